@@ -1,0 +1,200 @@
+"""Benchmark of the GLM-MHD line-march step on one MI355X.
+
+Workload (the turbulence configuration the metric is quoted on): 256^3 periodic box, L = 1,
+gamma = 5/3, rho = 1, p = 1 (cs ~ 1.29), B0 = (0.5, 0, 0), v = 0, psi = 0; solenoidal OU forcing
+in shells 1 <= |k| L / 2 pi <= 3, correlation time 0.5, Philox seed 1, fixed amplitude; PPM +
+full MUSCL-Hancock + HLLD + Dedner GLM; CFL 0.3.  A "step" is one driver step of run_turb:
+OU advance when due, dt from the CFL max folded into the previous step (one 8-byte read-back),
+and one fused march kernel.  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  value = cell-updates/s of the FP64 path (whole job, one GPU).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s + % HBM peak, 256^3 HLLD FP64 vs FP32; adjoint/forward cost"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+BYTES_PER_CELL = {"f64": 144, "f32": 72}   # 9 conserved variables, one read + one write
+AMP = 1.0e5                # OU amplitude (frozen)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def make_params(N):
+    from glmmhdturb_amd import Params
+    return Params(N=N, gamma=5.0 / 3.0, courant=0.3, rho0=1.0, cs0=math.sqrt(5.0 / 3.0), b0=0.5,
+                  turb_rms=AMP, turb_T=0.5, turb_Ndt=100, comp_frac=0.0, turb_seed=1,
+                  turb_gs=64, turb_kmin=1.0, turb_kmax=3.0)
+
+
+def run_gpu(N, steps, warmup, precision):
+    import torch
+    from glmmhdturb_amd import Solver, dxof, step_args
+
+    p = make_params(N)
+    s = Solver(p, precision)
+    u1, u2 = s.new_state(), s.new_state()
+    s.ic_uniform(u1, p.rho0, 1.0, (p.b0, 0.0, 0.0), p.gamma)     # rho = 1, p = 1
+    f = s.setup_forcing(p, AMP)
+    f.advance()
+    cmax = s.ctot_max(u1)
+    t = 0.0
+    stream = torch.cuda.current_stream()
+    ev = []
+
+    def one_step(timed):
+        nonlocal u1, u2, t, cmax
+        while t >= f.next_time:
+            f.advance()
+        dt = min(p.courant * dxof(p) / cmax, f.next_time - t)
+        a = step_args(p, dt, t, do_turb=True, riemann="hlld", recon="ppm", predictor="full", scheme="cube",
+                      want_cfl=True)
+        if timed:
+            e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        s.step(u1, u2, a)
+        if timed:
+            e1.record(stream)
+            ev.append((e0, e1))
+        u1, u2 = u2, u1
+        t += dt
+        cmax = s.cfl_max()
+
+    for _ in range(warmup):
+        one_step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
+    d = s.diagnostics(u1)
+    n = N ** 3
+    out = dict(value=n * steps / wall, ms_per_step=wall / steps * 1e3, kernel_ms=kern_ms, t=t, mach=d["mach"],
+               finite=d["finite"], divbmax=d["divbmax"])
+    s.close()
+    return out
+
+
+def run_gradient_ic(N, steps, precision):
+    """Throughput of the same scheme from the spike_mhd.cu gradient IC (no uniform-state regions)."""
+    import torch
+    from glmmhdturb_amd import Solver, step_args
+
+    p = make_params(N)
+    s = Solver(p, precision)
+    u1, u2 = s.new_state(), s.new_state()
+    s.ic_gradient(u1, p.gamma)
+    dt = p.courant * (1.0 / N) / s.ctot_max(u1)
+    a = step_args(p, dt, 0.0, riemann="hlld", recon="ppm", predictor="full", scheme="cube", want_cfl=True)
+    for _ in range(2):
+        s.step(u1, u2, a); u1, u2 = u2, u1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(u1, u2, a); u1, u2 = u2, u1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s.close()
+    return N ** 3 * steps / wall
+
+
+def cpu_baseline(seconds):
+    """The oracle (C restatement, OpenMP) on a bounded sample of the same workload: 128^3, same
+    IC, forcing and scheme, as many steps as fit in about `seconds`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import OUForcing, Oracle, Scheme
+    from glmmhdturb_amd import glm_speed, pfloor, ramp_of
+
+    M = 128
+    p = make_params(M)
+    orc = Oracle("f64")
+    u = orc.ic_uniform(M, M, M, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma)
+    fo = OUForcing(orc, g=64, kmin=1.0, kmax=3.0, T=0.5, ndt=100, comp_frac=0.0, amp=AMP, seed=1)
+    fo.advance()
+    t, steps, busy = 0.0, 0, 0.0
+    while busy < seconds or steps < 1:
+        while t >= fo.next_time:
+            fo.advance()
+        t0 = time.perf_counter()
+        cm = orc.ctot_max(u, p.gamma, p.smallr, pfloor(p))
+        dt = min(p.courant * (1.0 / M) / cm, fo.next_time - t)
+        ch, fac = glm_speed(p, dt)
+        sc = Scheme(gamma=p.gamma, smallr=p.smallr, smallc=p.smallc, dt=dt, dx=1.0 / M, ch=ch, glm_fac=fac,
+                    turb_min_rho=p.turb_min_rho, ramp=ramp_of(p, t), predictor="full", recon="ppm",
+                    riemann="hlld", mh_revert=True, face_guard="iflo", update_floor=True, do_turb=True, turb_gs=64)
+        u = orc.step(u, sc, afield=fo.afield)
+        busy += time.perf_counter() - t0
+        t += dt
+        steps += 1
+    return dict(value=M ** 3 * steps / busy, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"oracle C restatement (OpenMP, {threads} threads), 128^3 of the same workload, "
+                       f"{steps} steps incl. the CFL reduction, {busy:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU path is "
+              f"design-only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    N = args.N
+    r64 = run_gpu(N, args.steps, args.warmup, "f64")
+    rec = {
+        "metric": METRIC,
+        "value": r64["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (uniform IC + seeded solenoidal OU forcing)",
+        "config": {"workload": f"OU-driven GLM-MHD turbulence {N}^3 periodic, PPM + full Hancock + HLLD + GLM, "
+                               f"CFL 0.3, FP64", "N": N, "precision": "fp64", "parallelism": "single GPU"},
+    }
+    bpl = BYTES_PER_CELL["f64"] * N ** 3
+    ach = bpl / (r64["kernel_ms"] * 1e-3)
+    rec["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK, "traffic": None,
+                       "kernel": "glm::march_kernel<double, PRED_FULL, RIEM_HLLD>",
+                       "kernel_ms": r64["kernel_ms"], "bytes_per_cell": BYTES_PER_CELL["f64"]}
+    rec["state"] = {"t": r64["t"], "mach": r64["mach"], "divbmax": r64["divbmax"], "finite": r64["finite"]}
+    if not args.no_fp32:
+        r32 = run_gpu(N, args.steps, args.warmup, "f32")
+        ach32 = BYTES_PER_CELL["f32"] * N ** 3 / (r32["kernel_ms"] * 1e-3)
+        rec["fp32"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "kernel_ms": r32["kernel_ms"],
+                       "hbm_frac": ach32 / HBM_PEAK, "speedup_vs_fp64": r32["value"] / r64["value"]}
+    rec["gradient_ic_fp64"] = run_gradient_ic(N, max(5, args.steps // 3), "f64")
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

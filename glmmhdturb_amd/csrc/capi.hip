@@ -1,0 +1,560 @@
+// capi.hip -- extern "C" entry points of libglmmhd.so (declared in include/glmmhd.h).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/glmmhd.h"
+#include "aux_launch.hpp"
+#include "ct_launch.hpp"
+#include "launch.hpp"
+
+#ifndef GLM_DEFAULT_N
+#define GLM_DEFAULT_N 480
+#endif
+#ifndef GLM_LIB_NV
+#define GLM_LIB_NV 9
+#endif
+
+using namespace glm;
+
+// ------------------------------------------------------------------------------------------
+// error state (thread local)
+// ------------------------------------------------------------------------------------------
+static thread_local int t_last_error = GLM_OK;
+static thread_local char t_last_msg[256] = "";
+
+static int set_err(int code, const char* msg)
+{
+    t_last_error = code;
+    snprintf(t_last_msg, sizeof t_last_msg, "%s", msg);
+    return code;
+}
+static int hip_err(hipError_t e, const char* where)
+{
+    if (e == hipSuccess) return GLM_OK;
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
+    return set_err(GLM_ERR_HIP, buf);
+}
+#define HIPCHK(call)                                                   \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return hip_err(e_, #call);               \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------------------------
+struct glm_ctx {
+    int nx, ny, nz, prec;
+    size_t n;
+    hipStream_t stream;
+    unsigned long long* d_cmax;  // [0],[1] march CFL ping-pong, [2] CT step CFL, [3] standalone reductions
+    int cfl_parity;              // slot the next march step accumulates into
+    int cfl_slot;                // slot holding the CFL max of the last step with want_cfl (-1: none)
+    void* d_parts;
+    double* d_diag;
+    // OU forcing
+    int g;
+    double kmin, kmax, T;
+    int ndt;
+    double comp, amp;
+    unsigned long long seed;
+    long long nstep;
+    double *fre, *fim, *sqp, *unitk, *work, *cs, *sn;
+    void* afield;
+    // CT scratch (face fluxes / EMFs of the multi-pass CT path)
+    void* ct_scratch;
+};
+
+static void ctx_free(glm_ctx* c)
+{
+    if (!c) return;
+    void* ptrs[] = {c->d_cmax, c->d_parts, c->d_diag, c->fre, c->fim, c->sqp, c->unitk,
+                    c->work, c->cs, c->sn, c->afield, c->ct_scratch};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+extern "C" {
+
+void glm_params_default(glm_params* p)
+{
+    memset(p, 0, sizeof *p);
+    p->gamma = 1.01;                 // Params.gamma (glmmhd_turb.jl:31)
+    p->smallr = 1e-6;                // :33
+    p->smallc = 1e-4;                // :34
+    p->pfl = 0.0;                    // -> smallr*smallc^2/gamma (:55)
+    p->dt = 1e-4;
+    p->dx = 1.0 / 256;
+    p->ch = 1.0;
+    p->glm_fac = 1.0;
+    p->llf_dmin = 0.0;               // :39-40
+    p->llf_pmin = 0.0;
+    p->turb_min_rho = 1e-5;          // :50
+    p->ramp = 0.0;
+    p->predictor = GLM_PRED_FULL;
+    p->recon = GLM_RECON_PLM;
+    p->slope = GLM_SLOPE_MONCEN;
+    p->riemann = GLM_RIEMANN_HLLD;
+    p->mh_revert = 1;
+    p->face_guard = GLM_GUARD_IFLO;
+    p->update_floor = 1;
+    p->do_turb = 0;
+    p->zchunk = 0;
+    p->want_cfl = 1;
+}
+
+glm_ctx* glm_create(int nx, int ny, int nz, int precision)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || (precision != 8 && precision != 4)) {
+        set_err(GLM_ERR_ARG, "glm_create: grid must be >= 1 per axis and precision 8 or 4");
+        return nullptr;
+    }
+    glm_ctx* c = new glm_ctx();
+    memset(c, 0, sizeof *c);
+    c->nx = nx; c->ny = ny; c->nz = nz; c->prec = precision;
+    c->n = (size_t)nx * ny * nz;
+    c->stream = nullptr;
+    size_t np = diag_nparts(c->n);
+    c->cfl_parity = 0;
+    c->cfl_slot = -1;
+    if (hipMalloc(&c->d_cmax, 4 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_parts, np * diag_part_bytes()) != hipSuccess ||
+        hipMalloc(&c->d_diag, GLM_NDIAG * sizeof(double)) != hipSuccess) {
+        set_err(GLM_ERR_ALLOC, "glm_create: device allocation failed");
+        ctx_free(c);
+        return nullptr;
+    }
+    if (hipMemset(c->d_cmax, 0, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        set_err(GLM_ERR_HIP, "glm_create: hipMemset failed");
+        ctx_free(c);
+        return nullptr;
+    }
+    return c;
+}
+
+void glm_destroy(glm_ctx* ctx) { ctx_free(ctx); }
+
+int glm_set_stream(glm_ctx* ctx, void* stream)
+{
+    if (!ctx) return set_err(GLM_ERR_ARG, "glm_set_stream: null context");
+    ctx->stream = (hipStream_t)stream;
+    return GLM_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static Phys<T> make_phys(const glm_params* p)
+{
+    double pfl = p->pfl > 0.0 ? p->pfl : p->smallr * p->smallc * p->smallc / p->gamma;
+    Phys<T> s;
+    s.gamma = (T)p->gamma;
+    s.igm1 = (T)(1.0 / (p->gamma - 1.0));
+    s.smallr = (T)p->smallr;
+    s.pfl = (T)pfl;
+    s.ch = (T)p->ch;
+    s.ch2 = (T)(p->ch * p->ch);
+    s.half_ich = (T)(0.5 / p->ch);
+    s.llf_dmin = (T)p->llf_dmin;
+    s.llf_pmin = (T)p->llf_pmin;
+    return s;
+}
+
+static int check_params(const glm_params* p)
+{
+    if (!p) return set_err(GLM_ERR_ARG, "null glm_params");
+    if (!(p->gamma > 1.0) || !(p->dx > 0.0) || !(p->ch > 0.0) || !(p->smallr > 0.0))
+        return set_err(GLM_ERR_ARG, "glm_params: need gamma > 1, dx > 0, ch > 0, smallr > 0");
+    if (p->predictor < 0 || p->predictor > 1 || p->recon < 0 || p->recon > 2 || p->slope < 0 || p->slope > 1 ||
+        p->riemann < 0 || p->riemann > 3 || p->face_guard < 0 || p->face_guard > 2)
+        return set_err(GLM_ERR_ARG, "glm_params: scheme switch out of range");
+    return GLM_OK;
+}
+
+template <typename T>
+static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_params* p)
+{
+    MarchArgs<T> a;
+    memset(&a, 0, sizeof a);
+    for (int v = 0; v < 9; ++v) {
+        if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_step: null plane pointer");
+        if (q[v] == o[v]) return set_err(GLM_ERR_ARG, "glm_step: input and output planes must differ");
+        a.q[v] = (const T*)q[v];
+        a.o[v] = (T*)o[v];
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+    march_tiling(c->nx, c->ny, c->nz, p->zchunk, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
+    a.ph = make_phys<T>(p);
+    double dtdx = p->dt / p->dx;
+    a.dtdx = (T)dtdx;
+    a.half_dtdx = (T)(0.5 * dtdx);
+    a.glm_fac = (T)p->glm_fac;
+    a.smallc = (T)p->smallc;
+    a.turb_min_rho = (T)p->turb_min_rho;
+    a.ramp = (T)p->ramp;
+    a.dt = (T)p->dt;
+    a.dx = (T)p->dx;
+    a.lx = (T)(c->nx * p->dx); a.ly = (T)(c->ny * p->dx); a.lz = (T)(c->nz * p->dx);
+    a.recon = p->recon; a.slope = p->slope; a.mh_revert = p->mh_revert;
+    a.face_guard = p->face_guard; a.update_floor = p->update_floor;
+    a.do_turb = p->do_turb;
+    if (p->do_turb) {
+        if (!c->afield) return set_err(GLM_ERR_STATE, "glm_step: do_turb set but glm_ou_setup was not called");
+        a.afield = (const T*)c->afield;
+        a.tgs = c->g;
+    }
+    a.want_cfl = p->want_cfl;
+    a.cmax = c->d_cmax + c->cfl_parity;
+    a.cmax_next = c->d_cmax + (1 - c->cfl_parity);
+    HIPCHK(launch_march<T>(a, p->predictor, p->riemann, c->stream));
+    if (p->want_cfl) {
+        c->cfl_slot = c->cfl_parity;
+        c->cfl_parity = 1 - c->cfl_parity;
+    }
+    return GLM_OK;
+}
+
+extern "C" {
+
+int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
+{
+    if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, "glm_step: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    return ctx->prec == 8 ? step_impl<double>(ctx, q, o, p) : step_impl<float>(ctx, q, o, p);
+}
+
+int glm_cfl_max(glm_ctx* ctx, double* cmax)
+{
+    if (!ctx || !cmax) return set_err(GLM_ERR_ARG, "glm_cfl_max: null argument");
+    if (ctx->cfl_slot < 0) return set_err(GLM_ERR_STATE, "glm_cfl_max: no step with want_cfl has run");
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, ctx->d_cmax + ctx->cfl_slot, sizeof bits, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(cmax, &bits, sizeof bits);
+    return GLM_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static Ptr9<T> ptr9(void* const* q)
+{
+    Ptr9<T> r;
+    for (int v = 0; v < 9; ++v) r.p[v] = (T*)q[v];
+    return r;
+}
+
+template <typename T>
+static Ptr9<const T> cptr9(void* const* q)
+{
+    Ptr9<const T> r;
+    for (int v = 0; v < 9; ++v) r.p[v] = (const T*)q[v];
+    return r;
+}
+
+extern "C" {
+
+int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
+{
+    if (!ctx || !q || !cmax) return set_err(GLM_ERR_ARG, "glm_ctot_max: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
+    if (ctx->prec == 8) HIPCHK(launch_ctot<double>(cptr9<double>(q), ctx->n, make_phys<double>(p), ctx->d_cmax + 3, ctx->stream));
+    else HIPCHK(launch_ctot<float>(cptr9<float>(q), ctx->n, make_phys<float>(p), ctx->d_cmax + 3, ctx->stream));
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, ctx->d_cmax + 3, sizeof bits, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(cmax, &bits, sizeof bits);
+    return GLM_OK;
+}
+
+int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* out)
+{
+    if (!ctx || !q || !out) return set_err(GLM_ERR_ARG, "glm_diagnostics: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    if (ctx->prec == 8)
+        HIPCHK(launch_diag<double>(cptr9<double>(q), ctx->nx, ctx->ny, ctx->nz, p->dx, p->smallr, ctx->d_parts, ctx->d_diag, ctx->stream));
+    else
+        HIPCHK(launch_diag<float>(cptr9<float>(q), ctx->nx, ctx->ny, ctx->nz, p->dx, p->smallr, ctx->d_parts, ctx->d_diag, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, ctx->d_diag, GLM_NDIAG * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GLM_OK;
+}
+
+int glm_ic(glm_ctx* ctx, void* const* q, int kind, const double* args)
+{
+    if (!ctx || !q || !args) return set_err(GLM_ERR_ARG, "glm_ic: null argument");
+    if (kind < 0 || kind > 2) return set_err(GLM_ERR_ARG, "glm_ic: unknown kind");
+    if (ctx->prec == 8) HIPCHK(launch_ic<double>(ptr9<double>(q), ctx->nx, ctx->ny, ctx->nz, kind, args, ctx->stream));
+    else HIPCHK(launch_ic<float>(ptr9<float>(q), ctx->nx, ctx->ny, ctx->nz, kind, args, ctx->stream));
+    return GLM_OK;
+}
+
+int glm_ou_setup(glm_ctx* ctx, int g, double kmin, double kmax, double T, int ndt, double comp_frac, double amp,
+                 unsigned long long seed)
+{
+    if (!ctx) return set_err(GLM_ERR_ARG, "glm_ou_setup: null context");
+    if (g < 2 || g > 256 || !(T > 0.0) || ndt < 1 || !(kmax >= kmin))
+        return set_err(GLM_ERR_ARG, "glm_ou_setup: need 2 <= g <= 256, T > 0, ndt >= 1, kmax >= kmin");
+    void* olds[] = {ctx->fre, ctx->fim, ctx->sqp, ctx->unitk, ctx->work, ctx->cs, ctx->sn, ctx->afield};
+    for (void* p : olds)
+        if (p) (void)hipFree(p);
+    ctx->fre = ctx->fim = ctx->sqp = ctx->unitk = ctx->work = ctx->cs = ctx->sn = nullptr;
+    ctx->afield = nullptr;
+    size_t g3 = (size_t)g * g * g;
+    ctx->g = g; ctx->kmin = kmin; ctx->kmax = kmax; ctx->T = T; ctx->ndt = ndt;
+    ctx->comp = comp_frac; ctx->amp = amp; ctx->seed = seed; ctx->nstep = 0;
+    if (hipMalloc(&ctx->fre, 3 * g3 * 8) != hipSuccess || hipMalloc(&ctx->fim, 3 * g3 * 8) != hipSuccess ||
+        hipMalloc(&ctx->sqp, g3 * 8) != hipSuccess || hipMalloc(&ctx->unitk, 3 * g3 * 8) != hipSuccess ||
+        hipMalloc(&ctx->work, 4 * 3 * g3 * 8) != hipSuccess || hipMalloc(&ctx->cs, g * 8) != hipSuccess ||
+        hipMalloc(&ctx->sn, g * 8) != hipSuccess || hipMalloc(&ctx->afield, 3 * g3 * ctx->prec) != hipSuccess)
+        return set_err(GLM_ERR_ALLOC, "glm_ou_setup: device allocation failed");
+    std::vector<double> cs(g), sn(g);
+    for (int m = 0; m < g; ++m) {
+        cs[m] = std::cos(2.0 * M_PI * m / g);
+        sn[m] = std::sin(2.0 * M_PI * m / g);
+    }
+    HIPCHK(hipMemcpy(ctx->cs, cs.data(), g * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sn, sn.data(), g * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(ctx->fre, 0, 3 * g3 * 8, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->fim, 0, 3 * g3 * 8, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->afield, 0, 3 * g3 * ctx->prec, ctx->stream));
+    HIPCHK(launch_ou_init(g, kmin, kmax, ctx->sqp, ctx->unitk, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GLM_OK;
+}
+
+int glm_ou_advance(glm_ctx* ctx)
+{
+    if (!ctx || !ctx->fre) return set_err(GLM_ERR_STATE, "glm_ou_advance: call glm_ou_setup first");
+    double dt = ctx->T / ctx->ndt;             // glmmhd_turb.jl:2148-2150
+    double alpha = 1.0 - dt / ctx->T;
+    double beta = std::sqrt(2.0 * dt / ctx->T);
+    HIPCHK(launch_ou_step(ctx->g, ctx->fre, ctx->fim, ctx->sqp, ctx->unitk, alpha, beta, ctx->comp, 1.0 - ctx->comp,
+                          ctx->seed, (unsigned long long)ctx->nstep, ctx->stream));
+    ctx->nstep += 1;
+    if (ctx->prec == 8)
+        HIPCHK(launch_ou_afield<double>(ctx->g, ctx->fre, ctx->fim, ctx->work, ctx->cs, ctx->sn, ctx->amp,
+                                        (double*)ctx->afield, ctx->stream));
+    else
+        HIPCHK(launch_ou_afield<float>(ctx->g, ctx->fre, ctx->fim, ctx->work, ctx->cs, ctx->sn, ctx->amp,
+                                       (float*)ctx->afield, ctx->stream));
+    return GLM_OK;
+}
+
+const void* glm_ou_afield(glm_ctx* ctx) { return ctx ? ctx->afield : nullptr; }
+
+int glm_ou_copy_afield(glm_ctx* ctx, void* dst)
+{
+    if (!ctx || !ctx->afield || !dst) return set_err(GLM_ERR_STATE, "glm_ou_copy_afield: no forcing field or null dst");
+    size_t bytes = 3 * (size_t)ctx->g * ctx->g * ctx->g * (size_t)ctx->prec;
+    HIPCHK(hipMemcpyAsync(dst, ctx->afield, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return GLM_OK;
+}
+long long glm_ou_steps(glm_ctx* ctx) { return ctx ? ctx->nstep : -1; }
+
+int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
+{
+    if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, "glm_ct_step: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    if (p->riemann == GLM_RIEMANN_LLF_BOUND || p->recon == GLM_RECON_PPM)
+        return set_err(GLM_ERR_ARG, "glm_ct_step: riemann must be LLF/HLL/HLLD and recon PLM/PPM_SPIKE");
+    for (int v = 0; v < 8; ++v)
+        if (!q[v] || !o[v] || q[v] == o[v]) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers");
+    if (p->do_turb && !ctx->afield) return set_err(GLM_ERR_STATE, "glm_ct_step: do_turb without glm_ou_setup");
+    if (!ctx->ct_scratch) {
+        if (hipMalloc(&ctx->ct_scratch, ct_scratch_bytes(ctx->n, ctx->prec)) != hipSuccess)
+            return set_err(GLM_ERR_ALLOC, "glm_ct_step: scratch allocation failed");
+    }
+    CTArgsHost h;
+    h.nx = ctx->nx; h.ny = ctx->ny; h.nz = ctx->nz;
+    h.gamma = p->gamma; h.smallr = p->smallr;
+    h.smallp = p->pfl > 0.0 ? p->pfl : p->smallr * p->smallc * p->smallc / p->gamma;
+    h.dt = p->dt; h.dx = p->dx; h.smallc = p->smallc; h.turb_min_rho = p->turb_min_rho; h.ramp = p->ramp;
+    h.face_slope = ct_face_slope; h.riemann = p->riemann; h.recon = p->recon;
+    h.update_floor = p->update_floor; h.do_turb = p->do_turb; h.tgs = ctx->g; h.afield = ctx->afield;
+    h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2;
+    if (p->want_cfl) HIPCHK(launch_clear_u64(ctx->d_cmax + 2, 1, ctx->stream));
+    if (ctx->prec == 8) HIPCHK(launch_ct_step<double>(q, o, h, ctx->ct_scratch, ctx->stream));
+    else HIPCHK(launch_ct_step<float>(q, o, h, ctx->ct_scratch, ctx->stream));
+    if (p->want_cfl) ctx->cfl_slot = 2;
+    return GLM_OK;
+}
+
+int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
+{
+    if (!ctx || !q || !cmax) return set_err(GLM_ERR_ARG, "glm_ct_ctot_max: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    double smallp = p->pfl > 0.0 ? p->pfl : p->smallr * p->smallc * p->smallc / p->gamma;
+    HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
+    if (ctx->prec == 8)
+        HIPCHK(launch_ct_ctot<double>(q, ctx->nx, ctx->ny, ctx->nz, p->gamma, p->smallr, smallp, ctx->d_cmax + 3, ctx->stream));
+    else
+        HIPCHK(launch_ct_ctot<float>(q, ctx->nx, ctx->ny, ctx->nz, p->gamma, p->smallr, smallp, ctx->d_cmax + 3, ctx->stream));
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, ctx->d_cmax + 3, sizeof bits, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(cmax, &bits, sizeof bits);
+    return GLM_OK;
+}
+
+int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out)
+{
+    if (!ctx || !q || !out || !(dx > 0.0)) return set_err(GLM_ERR_ARG, "glm_ct_divb_max: bad argument");
+    HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
+    if (ctx->prec == 8) HIPCHK(launch_ct_divb<double>(q, ctx->nx, ctx->ny, ctx->nz, dx, ctx->d_cmax + 3, ctx->stream));
+    else HIPCHK(launch_ct_divb<float>(q, ctx->nx, ctx->ny, ctx->nz, dx, ctx->d_cmax + 3, ctx->stream));
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, ctx->d_cmax + 3, sizeof bits, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    memcpy(out, &bits, sizeof bits);
+    return GLM_OK;
+}
+
+int glm_last_error(void) { return t_last_error; }
+
+const char* glm_error_string(int code)
+{
+    if (code == t_last_error && t_last_msg[0]) return t_last_msg;
+    switch (code) {
+    case GLM_OK: return "ok";
+    case GLM_ERR_ARG: return "invalid argument";
+    case GLM_ERR_HIP: return "HIP runtime error";
+    case GLM_ERR_ALLOC: return "device allocation failed";
+    case GLM_ERR_STATE: return "invalid state";
+    default: return "unknown error";
+    }
+}
+
+int glm_kernel_info(int precision, int predictor, int riemann, int* vgprs, int* lds_bytes)
+{
+    hipFuncAttributes fa;
+    const void* fn = nullptr;
+    if (precision == 8) fn = march_kernel_ptr<double>(predictor, riemann);
+    else if (precision == 4) fn = march_kernel_ptr<float>(predictor, riemann);
+    if (!fn) return set_err(GLM_ERR_ARG, "glm_kernel_info: unknown variant");
+    HIPCHK(hipFuncGetAttributes(&fa, fn));
+    if (vgprs) *vgprs = fa.numRegs;
+    if (lds_bytes) *lds_bytes = (int)fa.sharedSizeBytes;
+    return GLM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Legacy MarchBridge ABI (spike_mhd.cu:319-341, spike_ctm.cu:299-329)
+// ------------------------------------------------------------------------------------------
+static std::mutex g_mu;
+static int g_nx = GLM_DEFAULT_N, g_ny = GLM_DEFAULT_N, g_nz = GLM_DEFAULT_N;
+static float g_dtdx = 0.02f, g_ch = 1.0f, g_glmfac = 1.0f, g_gamma = 1.4f;   // spike_mhd.cu:52
+static glm_ctx* g_ctx32 = nullptr;
+static glm_ctx* g_ctx64 = nullptr;
+
+int march_nv(void) { return GLM_LIB_NV; }
+int march_nx(void) { return g_nx; }
+int march_nsp(void) { return 0; }
+void march_set_dtdx(float v) { g_dtdx = v; }
+void march_set_glm(float ch, float fac) { g_ch = ch; g_glmfac = fac; }
+void march_set_gamma(float g) { g_gamma = g; }
+
+int march_set_grid(int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return set_err(GLM_ERR_ARG, "march_set_grid: bad grid");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_nx = nx; g_ny = ny; g_nz = nz;
+    glm_destroy(g_ctx32); g_ctx32 = nullptr;
+    glm_destroy(g_ctx64); g_ctx64 = nullptr;
+    return GLM_OK;
+}
+
+}  // extern "C"
+
+// The spike march scheme (spike_mhd.cu:257-288): normal-only Hancock, PLM MonCen, HLL, GLM pair,
+// psi *= GLMFAC, faces clamped at SMALLR/SMALLP = 1e-6, no predictor revert, no update floor.
+static glm_params legacy_params()
+{
+    glm_params p;
+    glm_params_default(&p);
+    p.gamma = g_gamma;
+    p.smallr = 1e-6;
+    p.pfl = 1e-6;
+    p.dx = 1.0;
+    p.dt = g_dtdx;
+    p.ch = g_ch;
+    p.glm_fac = g_glmfac;
+    p.predictor = GLM_PRED_ONED;
+    p.recon = GLM_RECON_PLM;
+    p.slope = GLM_SLOPE_MONCEN;
+    p.riemann = GLM_RIEMANN_HLL;
+    p.mh_revert = 0;
+    p.face_guard = GLM_GUARD_CLAMP;
+    p.update_floor = 0;
+    p.do_turb = 0;
+    p.want_cfl = 0;
+    return p;
+}
+
+template <typename T>
+static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, bool ct)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    glm_ctx*& c = sizeof(T) == 8 ? g_ctx64 : g_ctx32;
+    if (!c) c = glm_create(g_nx, g_ny, g_nz, (int)sizeof(T));
+    if (!c || !qv || !ov || nsteps < 0) return -1.0;
+    glm_params p = legacy_params();
+    std::vector<void*> q(qv, qv + nv), o(ov, ov + nv);
+    hipEvent_t t0, t1;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return -1.0;
+    (void)hipEventRecord(t0, c->stream);
+    int err = GLM_OK;
+    for (int it = 0; it < nsteps && !err; ++it) {
+        err = ct ? glm_ct_step(c, q.data(), o.data(), &p, 0) : glm_step(c, q.data(), o.data(), &p);
+        std::swap(q, o);
+    }
+    (void)hipEventRecord(t1, c->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (err) return -1.0;
+    if (q[0] != (void*)qv[0]) {   // odd step count: result is in ov; copy back (spike_mhd.cu:337)
+        for (int v = 0; v < nv; ++v)
+            if (hipMemcpyAsync(qv[v], q[v], c->n * sizeof(T), hipMemcpyDeviceToDevice, c->stream) != hipSuccess) return -1.0;
+        (void)hipStreamSynchronize(c->stream);
+    }
+    return (double)ms;
+}
+
+extern "C" {
+
+double march_run_dev(float* const* qv, float* const* ov, int nsteps) { return legacy_run<float>(qv, ov, nsteps, 9, false); }
+double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps) { return legacy_run<double>(qv, ov, nsteps, 9, false); }
+double ct_run(float* const* qv, float* const* ov, float* const* cc, int nsteps)
+{
+    (void)cc;
+    return legacy_run<float>(qv, ov, nsteps, 8, true);
+}
+int ct_regs(void)
+{
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, ct_kernel_ptr<float>()) != hipSuccess) return -1;
+    return fa.numRegs;
+}
+int ct_shmem(void)
+{
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, ct_kernel_ptr<float>()) != hipSuccess) return -1;
+    return (int)fa.sharedSizeBytes;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// ct.hip -- launchers of the constrained-transport kernels (ct_impl.hpp).
+#include "ct_impl.hpp"
+#include "ct_launch.hpp"
+
+namespace glm {
+
+size_t ct_scratch_bytes(size_t n, int prec) { return (size_t)(8 + 24 + 3) * n * (size_t)prec; }
+
+template <typename T>
+static CTArgs<T> make_ct_args(void* const* q, void* const* o, const CTArgsHost& h, void* scratch)
+{
+    CTArgs<T> a;
+    size_t n = (size_t)h.nx * h.ny * h.nz;
+    for (int v = 0; v < 9; ++v) {
+        a.q.p[v] = v < 8 ? (const T*)q[v] : nullptr;
+        a.o.p[v] = v < 8 ? (T*)o[v] : nullptr;
+    }
+    a.prim = (T*)scratch;
+    a.flx = a.prim + 8 * n;
+    a.emf = a.flx + 24 * n;
+    a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
+    a.gamma = (T)h.gamma; a.igm1 = (T)(1.0 / (h.gamma - 1.0)); a.smallr = (T)h.smallr; a.smallp = (T)h.smallp;
+    double dtdx = h.dt / h.dx;
+    a.dtdx = (T)dtdx; a.half_dtdx = (T)(0.5 * dtdx); a.dt = (T)h.dt; a.dx = (T)h.dx;
+    a.lx = (T)(h.nx * h.dx); a.ly = (T)(h.ny * h.dx); a.lz = (T)(h.nz * h.dx);
+    a.smallc = (T)h.smallc; a.turb_min_rho = (T)h.turb_min_rho; a.ramp = (T)h.ramp;
+    a.face_slope = h.face_slope; a.riemann = h.riemann; a.recon = h.recon; a.update_floor = h.update_floor;
+    a.do_turb = h.do_turb; a.tgs = h.tgs; a.want_cfl = h.want_cfl;
+    a.afield = (const T*)h.afield;
+    a.cmax = h.cmax;
+    return a;
+}
+
+template <typename T>
+hipError_t launch_ct_step(void* const* q, void* const* o, const CTArgsHost& h, void* scratch, hipStream_t st)
+{
+    CTArgs<T> a = make_ct_args<T>(q, o, h, scratch);
+    size_t n = (size_t)h.nx * h.ny * h.nz;
+    dim3 g((unsigned)((n + CT_NT - 1) / CT_NT)), b(CT_NT);
+    hipLaunchKernelGGL((ct_prim_kernel<T>), g, b, 0, st, a);
+    hipLaunchKernelGGL((ct_flux_kernel<T>), g, b, 0, st, a);
+    hipLaunchKernelGGL((ct_emf_kernel<T>), g, b, 0, st, a);
+    hipLaunchKernelGGL((ct_update_kernel<T>), g, b, 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !h.want_cfl) return e;
+    Ptr9<const T> o9;
+    for (int v = 0; v < 9; ++v) o9.p[v] = v < 8 ? (const T*)o[v] : nullptr;
+    hipLaunchKernelGGL((ct_ctot_kernel<T>), dim3(2048), b, 0, st, o9, h.nx, h.ny, h.nz, (T)h.gamma, (T)h.smallr,
+                       (T)h.smallp, h.cmax);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ct_ctot(void* const* q, int nx, int ny, int nz, double gamma, double smallr, double smallp,
+                          unsigned long long* out, hipStream_t st)
+{
+    Ptr9<const T> q9;
+    for (int v = 0; v < 9; ++v) q9.p[v] = v < 8 ? (const T*)q[v] : nullptr;
+    hipLaunchKernelGGL((ct_ctot_kernel<T>), dim3(2048), dim3(CT_NT), 0, st, q9, nx, ny, nz, (T)gamma, (T)smallr,
+                       (T)smallp, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ct_divb(void* const* q, int nx, int ny, int nz, double dx, unsigned long long* out, hipStream_t st)
+{
+    Ptr9<const T> q9;
+    for (int v = 0; v < 9; ++v) q9.p[v] = v < 8 ? (const T*)q[v] : nullptr;
+    hipLaunchKernelGGL((ct_divb_kernel<T>), dim3(2048), dim3(CT_NT), 0, st, q9, nx, ny, nz, dx, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+const void* ct_kernel_ptr()
+{
+    return (const void*)&ct_flux_kernel<T>;
+}
+
+#define GLM_CT_INST(T)                                                                                          \
+    template hipError_t launch_ct_step<T>(void* const*, void* const*, const CTArgsHost&, void*, hipStream_t);   \
+    template hipError_t launch_ct_ctot<T>(void* const*, int, int, int, double, double, double,                  \
+                                          unsigned long long*, hipStream_t);                                    \
+    template hipError_t launch_ct_divb<T>(void* const*, int, int, int, double, unsigned long long*, hipStream_t); \
+    template const void* ct_kernel_ptr<T>();
+GLM_CT_INST(double)
+GLM_CT_INST(float)
+#undef GLM_CT_INST
+
+}  // namespace glm

@@ -1,0 +1,359 @@
+// ct_impl.hpp -- constrained-transport MHD step on gfx950 (ct_mhd.jl:20-194, spike_ctm.cu).
+//
+// State: 8 SoA planes (rho, mx, my, mz, E, bxf, byf, bzf); bxf[i] is Bx on face i-1/2.
+// Passes: (1) cell-centred B (bcc_kernel!, ct_mhd.jl:85-96) folded into the primitive pass
+// (cellprim, :26-34); (2) the three lower-face fluxes of every cell, each solved once
+// (flux_kernel!, :99-122; fflm, spike_ctm.cu:223-245); (3) edge EMFs (emf_kernel!, :139-154);
+// (4) the hydro update (:124-136) and the face-B curl update (:156-171) in one kernel, with the
+// optional floor / OU kick / next-step CFL folded in.
+#pragma once
+#include "physics.hpp"
+
+namespace glm {
+
+constexpr int CT_NT = 256;
+
+template <typename T>
+struct CTArgs {
+    Ptr9<const T> q;   // 8 used
+    Ptr9<T> o;
+    T* prim;           // 8n
+    T* flx;            // 24n: flux at the lower-d face, plane (d*8 + t)
+    T* emf;            // 3n: ex, ey, ez
+    int nx, ny, nz;
+    T gamma, igm1, smallr, smallp, dtdx, half_dtdx, dt, dx, lx, ly, lz, smallc, turb_min_rho, ramp;
+    int face_slope, riemann, recon, update_floor, do_turb, tgs, want_cfl;
+    const T* afield;
+    unsigned long long* cmax;
+};
+
+GLM_DEV int cwrap(int i, int n)
+{
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+template <typename T>
+GLM_DEV size_t cidx(const CTArgs<T>& a, int i, int j, int k)
+{
+    return (size_t)i + (size_t)a.nx * ((size_t)j + (size_t)a.ny * (size_t)k);
+}
+
+// ct_mhd.jl:39-55 dflux in direction D (8 components, F[Bn] = 0)
+template <typename T, int D>
+GLM_DEV void ct_dflux(const T q[8], T f[8], T igm1)
+{
+    const T rho = q[0], vx = q[1], vy = q[2], vz = q[3], p = q[4], bx = q[5], by = q[6], bz = q[7];
+    T b2 = bx * bx + by * by + bz * bz, ptot = p + T(0.5) * b2;
+    T E = p * igm1 + T(0.5) * rho * (vx * vx + vy * vy + vz * vz) + T(0.5) * b2;
+    T vb = vx * bx + vy * by + vz * bz;
+    const T un = D == 0 ? vx : (D == 1 ? vy : vz);
+    const T bn = D == 0 ? bx : (D == 1 ? by : bz);
+    f[0] = rho * un;
+    f[1] = rho * un * vx - bn * bx + (D == 0 ? ptot : T(0));
+    f[2] = rho * un * vy - bn * by + (D == 1 ? ptot : T(0));
+    f[3] = rho * un * vz - bn * bz + (D == 2 ? ptot : T(0));
+    f[4] = (E + ptot) * un - bn * vb;
+    f[5] = D == 0 ? T(0) : un * bx - vx * bn;
+    f[6] = D == 1 ? T(0) : un * by - vy * bn;
+    f[7] = D == 2 ? T(0) : un * bz - vz * bn;
+}
+
+template <typename T>
+GLM_DEV void ct_toC(const T q[8], T u[8], T igm1)
+{
+    const T rho = q[0], vx = q[1], vy = q[2], vz = q[3], p = q[4], bx = q[5], by = q[6], bz = q[7];
+    u[0] = rho; u[1] = rho * vx; u[2] = rho * vy; u[3] = rho * vz;
+    u[4] = p * igm1 + T(0.5) * rho * (vx * vx + vy * vy + vz * vz) + T(0.5) * (bx * bx + by * by + bz * bz);
+    u[5] = bx; u[6] = by; u[7] = bz;
+}
+
+template <typename T>
+GLM_DEV T ct_cf(const T q[8], T bn, T gamma)
+{
+    T ir = T(1) / q[0];
+    T c2 = gamma * q[4] * ir, b2 = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir, dd = T(0.5) * (b2 + c2);
+    return tsqrt(dd + tsqrt(tmax(dd * dd - c2 * bn * bn * ir, T(0))));
+}
+
+// ct_mhd.jl:74-82 / spike_ctm.cu:170-184 hanc1d with offset hs (face = c -/+ hs)
+template <typename T, int D>
+GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const CTArgs<T>& a)
+{
+    T t[8], FL[8], FR[8], U[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = q[k] - hs[k];
+    ct_dflux<T, D>(t, FL, a.igm1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = q[k] + hs[k];
+    ct_dflux<T, D>(t, FR, a.igm1);
+    ct_toC(q, U, a.igm1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) U[k] = U[k] - a.half_dtdx * (FR[k] - FL[k]);
+    T rho = tmax(U[0], a.smallr), ir = T(1) / rho;
+    T vx = U[1] * ir, vy = U[2] * ir, vz = U[3] * ir;
+    T p = tmax((a.gamma - T(1)) * (U[4] - T(0.5) * rho * (vx * vx + vy * vy + vz * vz) -
+                                   T(0.5) * (U[5] * U[5] + U[6] * U[6] + U[7] * U[7])),
+               a.smallp);
+    mh[0] = rho; mh[1] = vx; mh[2] = vy; mh[3] = vz; mh[4] = p; mh[5] = U[5]; mh[6] = U[6]; mh[7] = U[7];
+}
+
+// Flux at the lower-D face from the stencil m2, m1, c0, p1 (ct_mhd.jl:99-122 / spike_ctm.cu:223-245).
+template <typename T, int D>
+GLM_DEV void ct_face_flux(const T m2[8], const T m1[8], const T c0[8], const T p1[8], T bn, const CTArgs<T>& a, T f[8])
+{
+    T hsL[8], hsR[8], mhL[8], mhR[8], L[8], R[8];
+    const T fac = a.face_slope == 1 ? T(0.25) : T(0.5);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        hsL[k] = fac * moncen(m1[k] - m2[k], c0[k] - m1[k]);
+        hsR[k] = fac * moncen(c0[k] - m1[k], p1[k] - c0[k]);
+    }
+    ct_hanc1d<T, D>(m1, hsL, mhL, a);
+    ct_hanc1d<T, D>(c0, hsR, mhR, a);
+    if (a.recon == RECON_PPM_SPIKE) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            T lo, hi;
+            ppm_edges_spike(m2[k], m1[k], c0[k], lo, hi);
+            L[k] = hi + mhL[k] - m1[k];
+            ppm_edges_spike(m1[k], c0[k], p1[k], lo, hi);
+            R[k] = lo + mhR[k] - c0[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { L[k] = mhL[k] + hsL[k]; R[k] = mhR[k] - hsR[k]; }
+    }
+    L[0] = tmax(L[0], a.smallr); L[4] = tmax(L[4], a.smallp);
+    R[0] = tmax(R[0], a.smallr); R[4] = tmax(R[4], a.smallp);
+    L[5 + D] = bn; R[5 + D] = bn;
+    if (a.riemann == RIEM_HLLD) {
+        // rotate D to x, Miyoshi-Kusano with the face-normal B and F[Bn] = 0, rotate back
+        constexpr int A = D % 3, B = (D + 1) % 3, C = (D + 2) % 3;
+        T Lr[9] = {L[0], L[1 + A], L[1 + B], L[1 + C], L[4], L[5 + A], L[5 + B], L[5 + C], T(0)};
+        T Rr[9] = {R[0], R[1 + A], R[1 + B], R[1 + C], R[4], R[5 + A], R[5 + B], R[5 + C], T(0)};
+        Phys<T> s;
+        s.gamma = a.gamma; s.igm1 = a.igm1; s.smallr = a.smallr; s.pfl = a.smallp;
+        s.ch = T(1); s.ch2 = T(1); s.half_ich = T(0.5); s.llf_dmin = T(0); s.llf_pmin = T(0);
+        T fx[9];
+        hlld_x(Lr, Rr, fx, s);
+        f[0] = fx[0]; f[4] = fx[4];
+        f[1 + A] = fx[1]; f[1 + B] = fx[2]; f[1 + C] = fx[3];
+        f[5 + A] = T(0); f[5 + B] = fx[6]; f[5 + C] = fx[7];
+        return;
+    }
+    T FL[8], FR[8], UL[8], UR[8];
+    ct_dflux<T, D>(L, FL, a.igm1);
+    ct_dflux<T, D>(R, FR, a.igm1);
+    ct_toC(L, UL, a.igm1);
+    ct_toC(R, UR, a.igm1);
+    const T uL = L[1 + D], uR = R[1 + D];
+    const T cL = ct_cf(L, L[5 + D], a.gamma), cR = ct_cf(R, R[5 + D], a.gamma);
+    if (a.riemann == RIEM_LLF) {
+        T smax = tmax(tabs(uL) + cL, tabs(uR) + cR);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = T(0.5) * (FL[k] + FR[k]) - T(0.5) * smax * (UR[k] - UL[k]);
+    } else {
+        T SL = tmin(tmin(uL - cL, uR - cR), T(0)), SR = tmax(tmax(uL + cL, uR + cR), T(0));
+        T ih = T(1) / (SR - SL);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * ih;
+    }
+}
+
+// Pass 1: cell-centred B and primitives (ct_mhd.jl:85-96 bcc, :26-34 cellprim)
+template <typename T>
+__global__ void __launch_bounds__(CT_NT) ct_prim_kernel(CTArgs<T> a)
+{
+    size_t n = (size_t)a.nx * a.ny * a.nz;
+    size_t c = (size_t)blockIdx.x * CT_NT + threadIdx.x;
+    if (c >= n) return;
+    int i = (int)(c % a.nx), j = (int)((c / a.nx) % a.ny), k = (int)(c / ((size_t)a.nx * a.ny));
+    T bx = T(0.5) * (a.q[5][c] + a.q[5][cidx(a, cwrap(i + 1, a.nx), j, k)]);
+    T by = T(0.5) * (a.q[6][c] + a.q[6][cidx(a, i, cwrap(j + 1, a.ny), k)]);
+    T bz = T(0.5) * (a.q[7][c] + a.q[7][cidx(a, i, j, cwrap(k + 1, a.nz))]);
+    T rho = tmax(a.q[0][c], a.smallr), ir = T(1) / rho;
+    T vx = a.q[1][c] * ir, vy = a.q[2][c] * ir, vz = a.q[3][c] * ir;
+    T p = tmax((a.gamma - T(1)) * (a.q[4][c] - T(0.5) * rho * (vx * vx + vy * vy + vz * vz) -
+                                   T(0.5) * (bx * bx + by * by + bz * bz)),
+               a.smallp);
+    T q8[8] = {rho, vx, vy, vz, p, bx, by, bz};
+#pragma unroll
+    for (int t = 0; t < 8; ++t) a.prim[(size_t)t * n + c] = q8[t];
+}
+
+// Pass 2: lower-face fluxes of each cell, one Riemann solve per face
+template <typename T>
+__global__ void __launch_bounds__(CT_NT) ct_flux_kernel(CTArgs<T> a)
+{
+    size_t n = (size_t)a.nx * a.ny * a.nz;
+    size_t c = (size_t)blockIdx.x * CT_NT + threadIdx.x;
+    if (c >= n) return;
+    int i = (int)(c % a.nx), j = (int)((c / a.nx) % a.ny), k = (int)(c / ((size_t)a.nx * a.ny));
+    auto ld = [&](size_t cc, T q[8]) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) q[t] = a.prim[(size_t)t * n + cc];
+    };
+    T m2[8], m1[8], c0[8], p1[8], f[8];
+    ld(c, c0);
+    // x
+    ld(cidx(a, cwrap(i - 2, a.nx), j, k), m2); ld(cidx(a, cwrap(i - 1, a.nx), j, k), m1);
+    ld(cidx(a, cwrap(i + 1, a.nx), j, k), p1);
+    ct_face_flux<T, 0>(m2, m1, c0, p1, a.q[5][c], a, f);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) a.flx[(size_t)(0 * 8 + t) * n + c] = f[t];
+    // y
+    ld(cidx(a, i, cwrap(j - 2, a.ny), k), m2); ld(cidx(a, i, cwrap(j - 1, a.ny), k), m1);
+    ld(cidx(a, i, cwrap(j + 1, a.ny), k), p1);
+    ct_face_flux<T, 1>(m2, m1, c0, p1, a.q[6][c], a, f);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) a.flx[(size_t)(1 * 8 + t) * n + c] = f[t];
+    // z
+    ld(cidx(a, i, j, cwrap(k - 2, a.nz)), m2); ld(cidx(a, i, j, cwrap(k - 1, a.nz)), m1);
+    ld(cidx(a, i, j, cwrap(k + 1, a.nz)), p1);
+    ct_face_flux<T, 2>(m2, m1, c0, p1, a.q[7][c], a, f);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) a.flx[(size_t)(2 * 8 + t) * n + c] = f[t];
+}
+
+// Pass 3: Balsara-Spicer edge EMFs (ct_mhd.jl:139-154)
+template <typename T>
+__global__ void __launch_bounds__(CT_NT) ct_emf_kernel(CTArgs<T> a)
+{
+    size_t n = (size_t)a.nx * a.ny * a.nz;
+    size_t c = (size_t)blockIdx.x * CT_NT + threadIdx.x;
+    if (c >= n) return;
+    int i = (int)(c % a.nx), j = (int)((c / a.nx) % a.ny), k = (int)(c / ((size_t)a.nx * a.ny));
+    size_t im = cidx(a, cwrap(i - 1, a.nx), j, k), jm = cidx(a, i, cwrap(j - 1, a.ny), k);
+    size_t km = cidx(a, i, j, cwrap(k - 1, a.nz));
+    const T* F = a.flx;
+#define CF(d, t, cc) F[(size_t)((d) * 8 + (t)) * n + (cc)]
+    a.emf[2 * n + c] = T(0.25) * (CF(1, 5, c) + CF(1, 5, im) - CF(0, 6, c) - CF(0, 6, jm));
+    a.emf[c] = T(0.25) * (CF(2, 6, c) + CF(2, 6, jm) - CF(1, 7, c) - CF(1, 7, km));
+    a.emf[n + c] = T(0.25) * (CF(0, 7, c) + CF(0, 7, km) - CF(2, 5, c) - CF(2, 5, im));
+#undef CF
+}
+
+// Pass 4: hydro update + face-B curl update (+ floor, OU kick, next-step CFL)
+template <typename T>
+__global__ void __launch_bounds__(CT_NT) ct_update_kernel(CTArgs<T> a)
+{
+    size_t n = (size_t)a.nx * a.ny * a.nz;
+    size_t c = (size_t)blockIdx.x * CT_NT + threadIdx.x;
+    T ctv = T(0);
+    if (c < n) {
+        int i = (int)(c % a.nx), j = (int)((c / a.nx) % a.ny), k = (int)(c / ((size_t)a.nx * a.ny));
+        size_t ip = cidx(a, cwrap(i + 1, a.nx), j, k), jp = cidx(a, i, cwrap(j + 1, a.ny), k);
+        size_t kp = cidx(a, i, j, cwrap(k + 1, a.nz));
+        const T* F = a.flx;
+#define CF(d, t, cc) F[(size_t)((d) * 8 + (t)) * n + (cc)]
+        T r[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+            r[t] = a.q[t][c] + a.dtdx * ((CF(0, t, c) - CF(0, t, ip)) + (CF(1, t, c) - CF(1, t, jp)) + (CF(2, t, c) - CF(2, t, kp)));
+#undef CF
+        const T *ex = a.emf, *ey = a.emf + n, *ez = a.emf + 2 * n;
+        T bxn = a.q[5][c] - a.dtdx * ((ez[jp] - ez[c]) - (ey[kp] - ey[c]));
+        T byn = a.q[6][c] - a.dtdx * ((ex[kp] - ex[c]) - (ez[ip] - ez[c]));
+        T bzn = a.q[7][c] - a.dtdx * ((ey[ip] - ey[c]) - (ex[jp] - ex[c]));
+        if (a.update_floor && r[0] < a.smallr) {
+            T bcx = a.prim[5 * n + c], bcy = a.prim[6 * n + c], bcz = a.prim[7 * n + c];
+            T emag = T(0.5) * (bcx * bcx + bcy * bcy + bcz * bcz);
+            r[0] = a.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0); r[4] = a.smallp * a.igm1 + emag;
+        }
+        if (a.do_turb && r[0] >= a.turb_min_rho) {
+            const int g = a.tgs;
+            const T gg = T(g);
+            T cx = ((T(i) + T(0.5)) * a.dx) / a.lx * gg, cy = ((T(j) + T(0.5)) * a.dx) / a.ly * gg;
+            T cz = ((T(k) + T(0.5)) * a.dx) / a.lz * gg;
+            T fx = floor(cx), fy = floor(cy), fz = floor(cz);
+            int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+            T wx = cx - fx, wy = cy - fy, wz = cz - fz;
+            int i0w = cwrap(i0, g), i1w = cwrap(i0 + 1, g), j0w = cwrap(j0, g), j1w = cwrap(j0 + 1, g);
+            int k0w = cwrap(k0, g), k1w = cwrap(k0 + 1, g);
+            size_t g3 = (size_t)g * g * g;
+            T acc[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const T* af = a.afield + cc * g3;
+                auto at = [&](int x, int y, int z) { return af[(size_t)x + (size_t)g * ((size_t)y + (size_t)g * z)]; };
+                T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
+                        (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
+                       ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
+                        (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
+                acc[cc] = a.ramp * t;
+            }
+            T rho = r[0], rhom = tmax(rho, a.smallr), irm = T(1) / rhom, fl = rho * a.smallc * a.smallc, e = r[4];
+            e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
+            e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
+            e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
+            T m2 = r[1] + rhom * acc[0] * a.dt, m3 = r[2] + rhom * acc[1] * a.dt, m4 = r[3] + rhom * acc[2] * a.dt;
+            e = tmax(e + T(0.5) * m2 * m2 * irm, fl);
+            e = tmax(e + T(0.5) * m3 * m3 * irm, fl);
+            e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
+            r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
+        }
+#pragma unroll
+        for (int t = 0; t < 5; ++t) a.o[t][c] = r[t];
+        a.o[5][c] = bxn; a.o[6][c] = byn; a.o[7][c] = bzn;
+    }
+    (void)ctv;
+}
+
+// CFL of a CT state from cell-centred B (the GLM rule, glmmhd_turb.jl:1963-1971)
+template <typename T>
+__global__ void __launch_bounds__(CT_NT) ct_ctot_kernel(Ptr9<const T> q, int nx, int ny, int nz, T gamma, T smallr,
+                                                        T smallp, unsigned long long* out)
+{
+    size_t n = (size_t)nx * ny * nz;
+    T cmax = T(0);
+    for (size_t c = (size_t)blockIdx.x * CT_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * CT_NT) {
+        int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+        size_t pl = (size_t)nx * ny;
+        T bx = T(0.5) * (q[5][c] + q[5][(size_t)cwrap(i + 1, nx) + (size_t)nx * j + pl * k]);
+        T by = T(0.5) * (q[6][c] + q[6][(size_t)i + (size_t)nx * cwrap(j + 1, ny) + pl * k]);
+        T bz = T(0.5) * (q[7][c] + q[7][(size_t)i + (size_t)nx * j + pl * cwrap(k + 1, nz)]);
+        T rho = tmax(q[0][c], smallr), ir = T(1) / rho;
+        T vx = q[1][c] * ir, vy = q[2][c] * ir, vz = q[3][c] * ir;
+        T p = tmax((gamma - T(1)) * (q[4][c] - T(0.5) * rho * (vx * vx + vy * vy + vz * vz) -
+                                     T(0.5) * (bx * bx + by * by + bz * bz)),
+                   smallp);
+        T p8[8] = {rho, vx, vy, vz, p, bx, by, bz};
+        T ct = (tabs(vx) + ct_cf(p8, bx, gamma)) + (tabs(vy) + ct_cf(p8, by, gamma)) + (tabs(vz) + ct_cf(p8, bz, gamma));
+        if (!(ct <= cmax)) cmax = ct;
+    }
+    unsigned long long bits = (unsigned long long)__double_as_longlong((double)cmax);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(out, bits);
+}
+
+// max |div B| from face fields (ct_mhd.jl:191-194)
+template <typename T>
+__global__ void __launch_bounds__(CT_NT) ct_divb_kernel(Ptr9<const T> q, int nx, int ny, int nz, double dx,
+                                                        unsigned long long* out)
+{
+    size_t n = (size_t)nx * ny * nz;
+    double m = 0.0;
+    for (size_t c = (size_t)blockIdx.x * CT_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * CT_NT) {
+        int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+        size_t pl = (size_t)nx * ny;
+        double d = (((double)q[5][(size_t)cwrap(i + 1, nx) + (size_t)nx * j + pl * k] - (double)q[5][c]) +
+                    ((double)q[6][(size_t)i + (size_t)nx * cwrap(j + 1, ny) + pl * k] - (double)q[6][c]) +
+                    ((double)q[7][(size_t)i + (size_t)nx * j + pl * cwrap(k + 1, nz)] - (double)q[7][c])) / dx;
+        d = fabs(d);
+        if (!(d <= m)) m = d;
+    }
+    unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0 && bits != 0ull) atomicMax(out, bits);
+}
+
+}  // namespace glm

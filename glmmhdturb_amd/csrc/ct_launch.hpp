@@ -1,0 +1,27 @@
+// ct_launch.hpp -- host launchers of the constrained-transport kernels (ct.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace glm {
+
+struct CTArgsHost {
+    int nx, ny, nz;
+    double gamma, smallr, smallp, dt, dx, smallc, turb_min_rho, ramp;
+    int face_slope, riemann, recon, update_floor, do_turb, tgs, want_cfl;
+    const void* afield;
+    unsigned long long* cmax;
+};
+
+size_t ct_scratch_bytes(size_t n, int prec);
+template <typename T>
+hipError_t launch_ct_step(void* const* q, void* const* o, const CTArgsHost& h, void* scratch, hipStream_t st);
+template <typename T>
+hipError_t launch_ct_ctot(void* const* q, int nx, int ny, int nz, double gamma, double smallr, double smallp,
+                          unsigned long long* out, hipStream_t st);
+template <typename T>
+hipError_t launch_ct_divb(void* const* q, int nx, int ny, int nz, double dx, unsigned long long* out, hipStream_t st);
+template <typename T>
+const void* ct_kernel_ptr();
+
+}  // namespace glm

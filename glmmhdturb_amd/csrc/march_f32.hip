@@ -1,0 +1,32 @@
+// march_f32.hip -- FP32 instantiations of the fused line-march kernel.
+#include "launch.hpp"
+
+namespace glm {
+
+template <>
+hipError_t launch_march<float>(const MarchArgs<float>& a, int pred, int riem, hipStream_t st)
+{
+    dim3 grid((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), block(NT);
+#define GLM_CASE(P, R)                                                                   \
+    if (pred == P && riem == R) {                                                        \
+        hipLaunchKernelGGL((march_kernel<float, P, R>), grid, block, 0, st, a);         \
+        return hipGetLastError();                                                        \
+    }
+    GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
+    GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
+#undef GLM_CASE
+    return hipErrorInvalidValue;
+}
+
+template <>
+const void* march_kernel_ptr<float>(int pred, int riem)
+{
+#define GLM_CASE(P, R) \
+    if (pred == P && riem == R) return (const void*)&march_kernel<float, P, R>;
+    GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
+    GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
+#undef GLM_CASE
+    return nullptr;
+}
+
+}  // namespace glm

@@ -1,0 +1,423 @@
+// march_impl.hpp -- fused GLM-MHD line-march step for gfx950 (one kernel per step).
+//
+// What it computes is the reference's one-step update  U1 = U0 + dt/dx * sum_d (F_lo - F_hi)
+// with the positivity floor, GLM psi damping and the fused OU kick (glmmhd_turb.jl:436-480,
+// :841-861), faces from PLM/PPM reconstruction with a MUSCL-Hancock predictor (full transverse,
+// glmmhd_turb.jl:165-172, :378-403, :931-1014; or normal-only, spike_mhd.cu:149-161, :257-285),
+// and LLF/HLL/HLLD Riemann fluxes with Dedner GLM cleaning (glmmhd_turb.jl:175-354).
+//
+// How it is laid out on the GPU:
+//   * A workgroup of TX x TY = 16 x 16 threads owns a 14 x 14 column tile and a z-chunk.  Each
+//     thread owns one (x,y) column of the tile grown by one halo cell and marches z.  Its
+//     column's primitives for planes q-1, q, q+1 live in registers.
+//   * LDS holds the primitives of the current plane q over the tile grown by two cells
+//     (18 x 18): the in-plane neighbours for slopes, PPM edges and the transverse predictor.
+//   * Every face is solved once.  The z face q-1/2 is solved by the column owner from the z+
+//     state it carried from plane q-1; x and y faces are solved after the +x/+y face states are
+//     exchanged through LDS, and the fluxes are exchanged back the same way.
+//   * The update of plane q-1 happens in the same pass, so face states and fluxes never touch
+//     HBM: one read and one write of the 9 conserved planes per cell and step (plus the halo).
+//   * The epilogue also folds the next step's CFL signal speed (glmmhd_turb.jl:1963-1971) into a
+//     per-thread max, reduced per wavefront and merged with one atomic per wavefront.
+#pragma once
+#include "physics.hpp"
+
+namespace glm {
+
+constexpr int TX = 16, TY = 16, NT = TX * TY;
+constexpr int OX = TX - 2, OY = TY - 2;     // owned columns per tile
+constexpr int PX = TX + 2, PY = TY + 2;     // LDS primitive plane (2-cell halo)
+constexpr int NPL = PX * PY;
+constexpr int NRING = NPL - NT;             // outer halo ring loaded by helper threads
+
+template <typename T>
+struct MarchArgs {
+    const T* q[9];
+    T* o[9];
+    int nx, ny, nz;
+    int tiles_x, tiles_y, nzc, zchunk;
+    Phys<T> ph;
+    T dtdx, half_dtdx, glm_fac;
+    T smallc, turb_min_rho, ramp, dt, dx, lx, ly, lz;
+    int recon, slope, mh_revert, face_guard, update_floor;
+    int do_turb, tgs, want_cfl;
+    const T* afield;
+    unsigned long long* cmax;       // this step's max CFL speed, as the bits of a non-negative double
+    unsigned long long* cmax_next;  // the slot the next step accumulates into (zeroed here)
+};
+
+GLM_DEV int wrapi(int i, int n)
+{
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+// Position of helper thread t on the outer ring of the PX x PY plane.
+GLM_DEV void ring_xy(int t, int& hx, int& hy)
+{
+    if (t < PX) { hx = t; hy = 0; }
+    else if (t < 2 * PX) { hx = t - PX; hy = PY - 1; }
+    else if (t < 2 * PX + (PY - 2)) { hx = 0; hy = t - 2 * PX + 1; }
+    else { hx = PX - 1; hy = t - 2 * PX - (PY - 2) + 1; }
+}
+
+template <typename T>
+GLM_DEV void load9(const MarchArgs<T>& a, size_t off, T c[9])
+{
+#pragma unroll
+    for (int v = 0; v < 9; ++v) c[v] = a.q[v][off];
+}
+
+// Face guard on one face state (glmmhd_turb.jl:618 iflo; spike_mhd.cu:145-147 clamp).
+template <typename T>
+GLM_DEV void guard(T f[9], const T m0[9], int g, const Phys<T>& s)
+{
+    if (g == GUARD_IFLO) {
+        bool bad = f[0] < s.smallr || f[4] < s.pfl;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) f[v] = bad ? m0[v] : f[v];
+    } else if (g == GUARD_CLAMP) {
+        f[0] = tmax(f[0], s.smallr);
+        f[4] = tmax(f[4], s.pfl);
+    }
+}
+
+// Normal-only Hancock half step of the cell centre in direction D (spike_mhd.cu:149-161); hs is
+// the half slope.  Returns the primitive state.
+template <typename T, int D>
+GLM_DEV void hanc1d(const T c0[9], const T hs[9], T mh[9], T half_dtdx, const Phys<T>& s)
+{
+    T t[9], fa[9], fb[9], u[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) t[v] = c0[v] - hs[v];
+    flux_dir<T, D>(t, fa, s);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) t[v] = c0[v] + hs[v];
+    flux_dir<T, D>(t, fb, s);
+    prim2cons(c0, u, s);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) u[v] -= half_dtdx * (fb[v] - fa[v]);
+    cons2prim(u, mh, s);
+}
+
+// Face states of one cell in direction D from its left/right neighbours l, r, the cell m0 and
+// the predicted mh (glmmhd_turb.jl:378-403, :955-973; spike_mhd.cu:262-275).
+template <typename T, int PRED, int D>
+GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9], bool up,
+                       const MarchArgs<T>& a, T lo[9], T hi[9])
+{
+    const Phys<T>& s = a.ph;
+    const bool mm = a.slope == SLOPE_MINMOD;
+    T hs[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) hs[v] = half_slope(l[v], m0[v], r[v], mm);
+    T mh[9];
+    if (PRED == PRED_ONED) {
+        hanc1d<T, D>(m0, hs, mh, a.half_dtdx, s);
+        if (a.mh_revert) {
+            bool bad = mh[0] <= s.smallr || mh[4] <= s.pfl;
+#pragma unroll
+            for (int v = 0; v < 9; ++v) mh[v] = bad ? m0[v] : mh[v];
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) mh[v] = mhf[v];
+    }
+    if (a.recon == RECON_PPM && up) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            T el, eh;
+            ppm_edges(l[v], m0[v], r[v], el, eh);
+            lo[v] = mh[v] + el - m0[v];
+            hi[v] = mh[v] + eh - m0[v];
+        }
+    } else if (a.recon == RECON_PPM_SPIKE) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) ppm_edges_spike(l[v], m0[v], r[v], lo[v], hi[v]);
+        lo[0] = tmax(lo[0], s.smallr); lo[4] = tmax(lo[4], s.pfl);
+        hi[0] = tmax(hi[0], s.smallr); hi[4] = tmax(hi[4], s.pfl);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) { lo[v] = lo[v] + mh[v] - m0[v]; hi[v] = hi[v] + mh[v] - m0[v]; }
+        lo[0] = tmax(lo[0], s.smallr); lo[4] = tmax(lo[4], s.pfl);
+        hi[0] = tmax(hi[0], s.smallr); hi[4] = tmax(hi[4], s.pfl);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) { lo[v] = mh[v] - hs[v]; hi[v] = mh[v] + hs[v]; }
+    }
+    guard(lo, m0, a.face_guard, s);
+    guard(hi, m0, a.face_guard, s);
+}
+
+// Update epilogue of one cell (glmmhd_turb.jl:440-476), CFL speed of the result
+// (glmmhd_turb.jl:1963-1971).  acc = (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) + (Fz_lo - Fz_hi).
+template <typename T>
+GLM_DEV T update_cell(T r[9], const T acc[9], const MarchArgs<T>& a, int gx, int gy, int gz)
+{
+    const Phys<T>& s = a.ph;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) r[v] = r[v] + a.dtdx * acc[v];
+    if (a.update_floor && r[0] < s.smallr) {
+        T emag = T(0.5) * (r[5] * r[5] + r[6] * r[6] + r[7] * r[7]);
+        r[0] = s.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0);
+        r[4] = s.pfl * s.igm1 + emag;
+    }
+    r[8] = r[8] * a.glm_fac;
+    if (a.do_turb && r[0] >= a.turb_min_rho) {
+        // glmmhd_turb.jl:483-505 turb_interp (cell centre (i+1/2) dx), periodic trilinear
+        const int g = a.tgs;
+        const T gg = T(g);
+        T cx = ((T(gx) + T(0.5)) * a.dx) / a.lx * gg;
+        T cy = ((T(gy) + T(0.5)) * a.dx) / a.ly * gg;
+        T cz = ((T(gz) + T(0.5)) * a.dx) / a.lz * gg;
+        T fx = floor(cx), fy = floor(cy), fz = floor(cz);
+        int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+        T wx = cx - fx, wy = cy - fy, wz = cz - fz;
+        int i0w = wrapi(i0, g), i1w = wrapi(i0 + 1, g);
+        int j0w = wrapi(j0, g), j1w = wrapi(j0 + 1, g);
+        int k0w = wrapi(k0, g), k1w = wrapi(k0 + 1, g);
+        const size_t g3 = (size_t)g * g * g;
+        T acc3[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T* af = a.afield + c * g3;
+            auto at = [&](int i, int j, int k) { return af[(size_t)i + (size_t)g * ((size_t)j + (size_t)g * k)]; };
+            T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
+                    (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
+                   ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
+                    (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
+            acc3[c] = a.ramp * t;
+        }
+        T rho = r[0], rhom = tmax(rho, s.smallr), irm = T(1) / rhom, fl = rho * a.smallc * a.smallc;
+        T e = r[4];
+        e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
+        e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
+        e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
+        T m2 = r[1] + rhom * acc3[0] * a.dt;
+        T m3 = r[2] + rhom * acc3[1] * a.dt;
+        T m4 = r[3] + rhom * acc3[2] * a.dt;
+        e = tmax(e + T(0.5) * m2 * m2 * irm, fl);
+        e = tmax(e + T(0.5) * m3 * m3 * irm, fl);
+        e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
+        r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
+    }
+    if (!a.want_cfl) return T(0);
+    T q[9];
+    cons2prim(r, q, s);
+    T b2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
+    return (tabs(q[1]) + fast_speed(q[0], q[4], b2, q[5], s)) + (tabs(q[2]) + fast_speed(q[0], q[4], b2, q[6], s)) +
+           (tabs(q[3]) + fast_speed(q[0], q[4], b2, q[7], s));
+}
+
+template <typename T, int PRED, int RIEM>
+__global__ void __launch_bounds__(NT) march_kernel(MarchArgs<T> a)
+{
+    __shared__ T P[9][NPL];
+    __shared__ T XB[9][NT];
+    __shared__ T YB[9][NT];
+
+    const Phys<T>& s = a.ph;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    int b = blockIdx.x;
+    const int tile_x = b % a.tiles_x; b /= a.tiles_x;
+    const int tile_y = b % a.tiles_y; b /= a.tiles_y;
+    const int zs = b * a.zchunk;
+    const int ze = min(zs + a.zchunk, a.nz);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const size_t plane = (size_t)nx * ny;
+    const int gxu = tile_x * OX - 1 + tx, gyu = tile_y * OY - 1 + ty;
+    const int gx = wrapi(gxu, nx), gy = wrapi(gyu, ny);
+    const bool owned = tx >= 1 && tx <= OX && ty >= 1 && ty <= OY && gxu < nx && gyu < ny;
+    const size_t col = (size_t)gx + (size_t)nx * gy;
+    const int lo_own = (ty + 1) * PX + (tx + 1);
+    const bool helper = tid < NRING;
+    int hx = 0, hy = 0;
+    ring_xy(helper ? tid : 0, hx, hy);
+    const size_t hcol = (size_t)wrapi(tile_x * OX - 2 + hx, nx) + (size_t)nx * wrapi(tile_y * OY - 2 + hy, ny);
+    const int lo_ring = hy * PX + hx;
+
+    if (a.want_cfl && blockIdx.x == 0 && tid == 0) *a.cmax_next = 0ull;
+
+    T pm[9], p0[9], pp[9];
+    {
+        T c[9];
+        load9(a, col + plane * wrapi(zs - 2, nz), c); cons2prim(c, pm, s);
+        load9(a, col + plane * wrapi(zs - 1, nz), c); cons2prim(c, p0, s);
+        load9(a, col + plane * wrapi(zs, nz), c); cons2prim(c, pp, s);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
+        if (helper) {
+            T h[9];
+            load9(a, hcol + plane * wrapi(zs - 1, nz), c); cons2prim(c, h, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) P[v][lo_ring] = h[v];
+        }
+    }
+    __syncthreads();
+
+    T zprev[9], fzprev[9], dxyprev[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) { zprev[v] = T(0); fzprev[v] = T(0); dxyprev[v] = T(0); }
+    T cmax = T(0);
+
+    for (int q = zs - 1; q <= ze; ++q) {
+        const bool last = q == ze;
+        const bool inplane = q >= zs && q < ze;
+
+        // ---- reconstruction of this column's cell in plane q --------------------------------
+        // PPM strong-pressure-jump flag (glmmhd_turb.jl:959)
+        bool up = true;
+        if (a.recon == RECON_PPM) {
+            up = !(spj(P[4][lo_own - 1], p0[4], P[4][lo_own + 1]) || spj(P[4][lo_own - PX], p0[4], P[4][lo_own + PX]) ||
+                   spj(pm[4], p0[4], pp[4]));
+        }
+        T mh[9];
+        if (PRED == PRED_FULL) {
+            // glmmhd_turb.jl:165-172 hancock, :391-395 revert
+            const bool mm = a.slope == SLOPE_MINMOD;
+            T u0c[9], acc[9], t[9], fa[9], fb[9], hs[9];
+            prim2cons(p0, u0c, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) hs[v] = half_slope(P[v][lo_own - 1], p0[v], P[v][lo_own + 1], mm);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
+            flux_dir<T, 0>(t, fa, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = p0[v] - hs[v];
+            flux_dir<T, 0>(t, fb, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] = fa[v] - fb[v];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) hs[v] = half_slope(P[v][lo_own - PX], p0[v], P[v][lo_own + PX], mm);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
+            flux_dir<T, 1>(t, fa, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = p0[v] - hs[v];
+            flux_dir<T, 1>(t, fb, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] = acc[v] + (fa[v] - fb[v]);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) hs[v] = half_slope(pm[v], p0[v], pp[v], mm);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
+            flux_dir<T, 2>(t, fa, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = p0[v] - hs[v];
+            flux_dir<T, 2>(t, fb, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] = acc[v] + (fa[v] - fb[v]);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) t[v] = u0c[v] - a.half_dtdx * acc[v];
+            cons2prim(t, mh, s);
+            if (a.mh_revert) {
+                bool bad = mh[0] <= s.smallr || mh[4] <= s.pfl;
+#pragma unroll
+                for (int v = 0; v < 9; ++v) mh[v] = bad ? p0[v] : mh[v];
+            }
+        }
+
+        // ---- z face q-1/2 and the update of plane q-1 ---------------------------------------
+        {
+            T zlo[9], zhi[9];
+            faces_dir<T, PRED, 2>(pm, p0, pp, mh, up, a, zlo, zhi);
+            if (q >= zs && owned) {
+                T fz[9];
+                riemann<T, RIEM, 2>(zprev, zlo, fz, s);
+                if (q >= zs + 1) {
+                    // update plane q-1: acc = dxy(q-1) + (Fz(q-3/2) - Fz(q-1/2))
+                    T r[9], acc[9];
+                    load9(a, col + plane * wrapi(q - 1, nz), r);
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) acc[v] = dxyprev[v] + (fzprev[v] - fz[v]);
+                    const int zq = wrapi(q - 1, nz);
+                    T ct = update_cell(r, acc, a, gx, gy, zq);
+                    if (!(ct <= cmax)) cmax = ct;
+                    const size_t off = col + plane * zq;
+#pragma unroll
+                    for (int v = 0; v < 9; ++v) __builtin_nontemporal_store(r[v], a.o[v] + off);
+                }
+#pragma unroll
+                for (int v = 0; v < 9; ++v) fzprev[v] = fz[v];
+            }
+#pragma unroll
+            for (int v = 0; v < 9; ++v) zprev[v] = zhi[v];
+        }
+        if (last) break;
+
+        // ---- prefetch the next planes ----------------------------------------------------------
+        T cn[9], hr[9];
+        load9(a, col + plane * wrapi(q + 2, nz), cn);
+        if (helper) load9(a, hcol + plane * wrapi(q + 1, nz), hr);
+
+        // ---- x and y faces of plane q (only for planes this chunk updates) ---------------------
+        if (inplane) {
+            T xlo[9], ylo[9];
+            {
+                T l[9], r[9], hi[9];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - 1]; r[v] = P[v][lo_own + 1]; }
+                faces_dir<T, PRED, 0>(l, p0, r, mh, up, a, xlo, hi);
+#pragma unroll
+                for (int v = 0; v < 9; ++v) XB[v][tid] = hi[v];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - PX]; r[v] = P[v][lo_own + PX]; }
+                faces_dir<T, PRED, 1>(l, p0, r, mh, up, a, ylo, hi);
+#pragma unroll
+                for (int v = 0; v < 9; ++v) YB[v][tid] = hi[v];
+            }
+            __syncthreads();
+            T fx[9], fy[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) { fx[v] = T(0); fy[v] = T(0); }
+            if (tx >= 1) {
+                T L[9];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) L[v] = XB[v][tid - 1];
+                riemann<T, RIEM, 0>(L, xlo, fx, s);
+            }
+            if (ty >= 1) {
+                T L[9];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) L[v] = YB[v][tid - TX];
+                riemann<T, RIEM, 1>(L, ylo, fy, s);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 9; ++v) { XB[v][tid] = fx[v]; YB[v][tid] = fy[v]; }
+            __syncthreads();
+            if (owned) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) dxyprev[v] = (fx[v] - XB[v][tid + 1]) + (fy[v] - YB[v][tid + TX]);
+            }
+        }
+
+        // ---- advance the column window and publish plane q+1 to LDS ------------------------------
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 9; ++v) { pm[v] = p0[v]; p0[v] = pp[v]; }
+        cons2prim(cn, pp, s);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
+        if (helper) {
+            T h[9];
+            cons2prim(hr, h, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) P[v][lo_ring] = h[v];
+        }
+        __syncthreads();
+    }
+
+    if (a.want_cfl) {
+        // wavefront max (NaN propagates as the largest bit pattern), one atomic per wavefront
+        double m = (double)cmax;
+        unsigned long long bits = __double_as_longlong(m);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            unsigned long long o = __shfl_xor(bits, off, 64);
+            bits = o > bits ? o : bits;
+        }
+        if ((tid & 63) == 0 && bits != 0ull) atomicMax(a.cmax, bits);
+    }
+}
+
+}  // namespace glm

@@ -1,0 +1,340 @@
+// physics.hpp -- GLM-MHD point physics for the gfx950 kernels.
+//
+// Device functions templated on the scalar type T (double or float).  A direction D in {0,1,2}
+// is a compile-time cyclic permutation (A,B,C) = (D, D+1, D+2) mod 3 of the vector components,
+// so the reference's rot_to / rot_flux_from (glmmhd_turb.jl:119-138) become register renaming.
+//
+// Conserved order (rho, mx, my, mz, E, Bx, By, Bz, psi); primitive order
+// (rho, vx, vy, vz, p, Bx, By, Bz, psi) -- glmmhd_turb.jl:61-64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GLM_DEV __device__ __forceinline__
+
+namespace glm {
+
+enum Pred { PRED_FULL = 0, PRED_ONED = 1 };
+enum Recon { RECON_PLM = 0, RECON_PPM = 1, RECON_PPM_SPIKE = 2 };
+enum Riem { RIEM_LLF = 0, RIEM_LLFB = 1, RIEM_HLL = 2, RIEM_HLLD = 3 };
+enum Slope { SLOPE_MONCEN = 0, SLOPE_MINMOD = 1 };
+enum FaceGuard { GUARD_NONE = 0, GUARD_IFLO = 1, GUARD_CLAMP = 2 };
+
+// Scalars every point function needs (precomputed on the host in double, cast to T).
+template <typename T>
+struct Phys {
+    T gamma, igm1;          // gamma, 1/(gamma-1)
+    T smallr, pfl;          // density and pressure floors (pfl = smallr*smallc^2/gamma)
+    T ch, ch2, half_ich;    // GLM speed, ch^2, 0.5/ch
+    T llf_dmin, llf_pmin;   // LLF robustness switch (<= 0 disables), glmmhd_turb.jl:287-290
+};
+
+// Nine plane pointers passed by value in kernel arguments.
+template <typename T>
+struct Ptr9 {
+    T* p[9];
+    GLM_DEV T* operator[](int i) const { return p[i]; }
+};
+
+template <typename T> GLM_DEV T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T> GLM_DEV T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T> GLM_DEV T tabs(T a) { return a < T(0) ? -a : a; }
+GLM_DEV double tsqrt(double a) { return __builtin_sqrt(a); }
+GLM_DEV float tsqrt(float a) { return __builtin_sqrtf(a); }
+
+// glmmhd_turb.jl:68-77 cons2prim
+template <typename T>
+GLM_DEV void cons2prim(const T c[9], T q[9], const Phys<T>& s)
+{
+    T rho = tmax(c[0], s.smallr);
+    T ir = T(1) / rho;
+    T ekin = T(0.5) * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
+    T emag = T(0.5) * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
+    q[0] = rho;
+    q[1] = c[1] * ir; q[2] = c[2] * ir; q[3] = c[3] * ir;
+    q[4] = tmax((s.gamma - T(1)) * (c[4] - ekin - emag), s.pfl);
+    q[5] = c[5]; q[6] = c[6]; q[7] = c[7]; q[8] = c[8];
+}
+
+// glmmhd_turb.jl:79-86 prim2cons
+template <typename T>
+GLM_DEV void prim2cons(const T q[9], T c[9], const Phys<T>& s)
+{
+    T ekin = T(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    T emag = T(0.5) * (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]);
+    c[0] = q[0];
+    c[1] = q[0] * q[1]; c[2] = q[0] * q[2]; c[3] = q[0] * q[3];
+    c[4] = q[4] * s.igm1 + ekin + emag;
+    c[5] = q[5]; c[6] = q[6]; c[7] = q[7]; c[8] = q[8];
+}
+
+// glmmhd_turb.jl:89-95 fast_speed; b2 = |B|^2 is frame independent.
+template <typename T>
+GLM_DEV T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
+{
+    T ir = T(1) / rho;
+    T c2 = s.gamma * p * ir;
+    T bb = b2 * ir;
+    T d2 = T(0.5) * (bb + c2);
+    return tsqrt(d2 + tsqrt(tmax(d2 * d2 - c2 * bn * bn * ir, T(0))));
+}
+
+// Ideal-MHD flux of a primitive state in direction D (glmmhd_turb.jl:98-115 evaluated in the
+// rotated frame and rotated back, :161-162).  F[Bn] = F[psi] = 0.
+template <typename T, int D>
+GLM_DEV void flux_dir(const T q[9], T f[9], const Phys<T>& s)
+{
+    constexpr int A = D % 3, B = (D + 1) % 3, C = (D + 2) % 3;
+    const T rho = q[0], un = q[1 + A], u1 = q[1 + B], u2 = q[1 + C], p = q[4];
+    const T bn = q[5 + A], b1 = q[5 + B], b2_ = q[5 + C];
+    T b2 = bn * bn + b1 * b1 + b2_ * b2_;
+    T ptot = p + T(0.5) * b2;
+    T ekin = T(0.5) * rho * (un * un + u1 * u1 + u2 * u2);
+    T E = p * s.igm1 + ekin + T(0.5) * b2;
+    T vdotb = un * bn + u1 * b1 + u2 * b2_;
+    f[0] = rho * un;
+    f[1 + A] = rho * un * un + ptot - bn * bn;
+    f[1 + B] = rho * un * u1 - bn * b1;
+    f[1 + C] = rho * un * u2 - bn * b2_;
+    f[4] = (E + ptot) * un - bn * vdotb;
+    f[5 + A] = T(0);
+    f[5 + B] = un * b1 - u1 * bn;
+    f[5 + C] = un * b2_ - u2 * bn;
+    f[8] = T(0);
+}
+
+// x-frame flux of a state already in the rotated (x-normal) order (phys_flux_x).
+template <typename T>
+GLM_DEV void flux_x(const T q[9], T f[9], const Phys<T>& s) { flux_dir<T, 0>(q, f, s); }
+
+// glmmhd_turb.jl:145-150 moncen and :300-301 minmod (each halved by the caller's prim_slope).
+template <typename T>
+GLM_DEV T moncen(T dl, T dr)
+{
+    T dc = T(0.5) * (dl + dr);
+    T m = tmin(T(2) * tmin(tabs(dl), tabs(dr)), tabs(dc));
+    T val = dc >= T(0) ? m : -m;
+    return (dl * dr <= T(0)) ? T(0) : val;
+}
+template <typename T>
+GLM_DEV T minmod(T a, T b)
+{
+    return (a * b <= T(0)) ? T(0) : (tabs(a) < tabs(b) ? a : b);
+}
+// Half slope (glmmhd_turb.jl:153-155 / :302-303); `mm` selects minmod.
+template <typename T>
+GLM_DEV T half_slope(T l, T m, T r, bool mm)
+{
+    T dl = m - l, dr = r - m;
+    return T(0.5) * (mm ? minmod(dl, dr) : moncen(dl, dr));
+}
+
+// glmmhd_turb.jl:917-923 ppm_mono / ppm_edges (Julia form).
+template <typename T>
+GLM_DEV void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
+{
+    T slope = T(0.25) * (qp - qm);
+    T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
+    T ql = q0 - slope + curve, qr = q0 + slope + curve;
+    T dq = qr - ql;
+    T diff = (q0 - T(0.5) * (ql + qr)) * dq;
+    T lim = dq * dq * T(1.0 / 6.0);
+    bool flat = (qr - q0) * (q0 - ql) <= T(0);
+    bool hi_os = diff > lim;
+    bool lo_os = diff < -lim;
+    lo = flat ? q0 : (hi_os ? T(3) * q0 - T(2) * qr : ql);
+    hi = flat ? q0 : (hi_os ? qr : (lo_os ? T(3) * q0 - T(2) * ql : qr));
+}
+
+// spike_mhd.cu:164-172 ppm1 (both edges).
+template <typename T>
+GLM_DEV void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
+{
+    T d = T(0.25) * (qp - qm), cu = (qm - T(2) * q0 + qp) * (T(1) / T(6));
+    T qr = q0 + d - cu, ql = q0 - d - cu;
+    if ((qr - q0) * (q0 - ql) <= T(0)) { qr = q0; ql = q0; }
+    else {
+        T dqe = qr - ql, d6 = T(6) * (q0 - T(0.5) * (ql + qr));
+        if (dqe * d6 > dqe * dqe) ql = T(3) * q0 - T(2) * qr;
+        if (dqe * d6 < -dqe * dqe) qr = T(3) * q0 - T(2) * ql;
+    }
+    lo = ql; hi = qr;
+}
+
+// glmmhd_turb.jl:926 spj
+template <typename T>
+GLM_DEV bool spj(T a, T b, T c)
+{
+    T hi = tmax(a, tmax(b, c));
+    T lo = tmax(tmin(a, tmin(b, c)), T(1e-20));
+    return hi / lo > T(2);
+}
+
+// ----------------------------------------------------------------------------------------
+// Riemann solvers in the x-normal frame.  L, R are floored, GLM-cleaned primitive states
+// (Bx = bn*); fbn/fpsi are the GLM fluxes placed in slots 5 and 8.
+// ----------------------------------------------------------------------------------------
+
+// glmmhd_turb.jl:182-190 llf_x (exact fast speed) and :311-324 llf_bound.
+template <typename T, bool BOUND>
+GLM_DEV void llf_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
+{
+    T b2L = L[5] * L[5] + L[6] * L[6] + L[7] * L[7];
+    T b2R = R[5] * R[5] + R[6] * R[6] + R[7] * R[7];
+    T cL, cR;
+    if (BOUND) {
+        cL = tsqrt((s.gamma * L[4] + b2L) * (T(1) / L[0]));
+        cR = tsqrt((s.gamma * R[4] + b2R) * (T(1) / R[0]));
+    } else {
+        cL = fast_speed(L[0], L[4], b2L, L[5], s);
+        cR = fast_speed(R[0], R[4], b2R, R[5], s);
+    }
+    T smax = tmax(tmax(tabs(L[1]) + cL, tabs(R[1]) + cR), s.ch);
+    T fL[9], fR[9], uL[9], uR[9];
+    flux_x(L, fL, s); flux_x(R, fR, s);
+    prim2cons(L, uL, s); prim2cons(R, uR, s);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v] = T(0.5) * (fL[v] + fR[v]) - T(0.5) * smax * (uR[v] - uL[v]);
+}
+
+// glmmhd_turb.jl:335-343 hll_x
+template <typename T>
+GLM_DEV void hll_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
+{
+    T bn = L[5];
+    T cfL = fast_speed(L[0], L[4], L[5] * L[5] + L[6] * L[6] + L[7] * L[7], bn, s);
+    T cfR = fast_speed(R[0], R[4], R[5] * R[5] + R[6] * R[6] + R[7] * R[7], bn, s);
+    T SL = tmin(tmin(L[1] - cfL, R[1] - cfR), T(0));
+    T SR = tmax(tmax(L[1] + cfL, R[1] + cfR), T(0));
+    T fL[9], fR[9], uL[9], uR[9];
+    flux_x(L, fL, s); flux_x(R, fR, s);
+    prim2cons(L, uL, s); prim2cons(R, uR, s);
+    T ihd = T(1) / (SR - SL);
+    T slsr = SL * SR;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) f[v] = (SR * fL[v] - SL * fR[v] + slsr * (uR[v] - uL[v])) * ihd;
+}
+
+// glmmhd_turb.jl:193-272 hlld_x (Miyoshi & Kusano 2005).  Only the side that the sampling
+// selects (SM >= 0 -> left) has its 9-vectors formed; the scalar star/double-star quantities
+// of both sides are computed as in the reference.
+template <typename T>
+GLM_DEV void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
+{
+    const T bn = L[5];
+    const T dL = L[0], uL = L[1], vL = L[2], wL = L[3], pL = L[4], byL = L[6], bzL = L[7];
+    const T dR = R[0], uR = R[1], vR = R[2], wR = R[3], pR = R[4], byR = R[6], bzR = R[7];
+    T b2L = bn * bn + byL * byL + bzL * bzL;
+    T b2R = bn * bn + byR * byR + bzR * bzR;
+    T ptL = pL + T(0.5) * b2L, ptR = pR + T(0.5) * b2R;
+    T cfL = fast_speed(dL, pL, b2L, bn, s);
+    T cfR = fast_speed(dR, pR, b2R, bn, s);
+    T cfm = tmax(cfL, cfR);
+    T SL = tmin(tmin(uL, uR) - cfm, T(0));
+    T SR = tmax(tmax(uL, uR) + cfm, T(0));
+    T aL = SL - uL, aR = SR - uR;
+    T iden = T(1) / (aR * dR - aL * dL);
+    T SM = (aR * dR * uR - aL * dL * uL - ptR + ptL) * iden;
+    T pts = (aR * dR * ptL - aL * dL * ptR + dL * dR * aR * aL * (uR - uL)) * iden;
+    T iL = T(1) / (SL - SM), iR = T(1) / (SR - SM);
+    T dLs = tmax(dL * aL * iL, T(1e-12));
+    T dRs = tmax(dR * aR * iR, T(1e-12));
+    T sqdLs = tsqrt(dLs), sqdRs = tsqrt(dRs);
+    T absbn = tabs(bn);
+    T SLs = SM - absbn / sqdLs;
+    T SRs = SM + absbn / sqdRs;
+    // star states (glmmhd_turb.jl:227-239)
+    T vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
+    {
+        T denK = dL * aL * (SL - SM) - bn * bn;
+        bool sm = tabs(denK) < T(1e-12);
+        T ik = T(1) / denK;
+        T fac = bn * (SM - uL) * ik;
+        T gb = (dL * aL * aL - bn * bn) * ik;
+        vyLs = sm ? vL : vL - byL * fac;
+        vzLs = sm ? wL : wL - bzL * fac;
+        byLs = sm ? byL : byL * gb;
+        bzLs = sm ? bzL : bzL * gb;
+    }
+    {
+        T denK = dR * aR * (SR - SM) - bn * bn;
+        bool sm = tabs(denK) < T(1e-12);
+        T ik = T(1) / denK;
+        T fac = bn * (SM - uR) * ik;
+        T gb = (dR * aR * aR - bn * bn) * ik;
+        vyRs = sm ? vR : vR - byR * fac;
+        vzRs = sm ? wR : wR - bzR * fac;
+        byRs = sm ? byR : byR * gb;
+        bzRs = sm ? bzR : bzR * gb;
+    }
+    T sgn = bn >= T(0) ? T(1) : T(-1);
+    T invsum = T(1) / (sqdLs + sqdRs);
+    T vyss = (sqdLs * vyLs + sqdRs * vyRs + (byRs - byLs) * sgn) * invsum;
+    T vzss = (sqdLs * vzLs + sqdRs * vzRs + (bzRs - bzLs) * sgn) * invsum;
+    T byss = (sqdLs * byRs + sqdRs * byLs + sqdLs * sqdRs * (vyRs - vyLs) * sgn) * invsum;
+    T bzss = (sqdLs * bzRs + sqdRs * bzLs + sqdLs * sqdRs * (vzRs - vzLs) * sgn) * invsum;
+    T vdotbss = SM * bn + vyss * byss + vzss * bzss;
+    // Select the sampled side: (SM >= 0) -> left fan, else right fan.
+    const bool left = SM >= T(0);
+    const T d = left ? dL : dR, u = left ? uL : uR, v = left ? vL : vR, w = left ? wL : wR;
+    const T p = left ? pL : pR, by = left ? byL : byR, bz = left ? bzL : bzR;
+    const T b2 = left ? b2L : b2R, pt = left ? ptL : ptR;
+    const T S = left ? SL : SR, a = left ? aL : aR, iS = left ? iL : iR;
+    const T ds = left ? dLs : dRs, sqd = left ? sqdLs : sqdRs, Ss = left ? SLs : SRs;
+    const T vys = left ? vyLs : vyRs, vzs = left ? vzLs : vzRs;
+    const T bys = left ? byLs : byRs, bzs = left ? bzLs : bzRs;
+    T E = p * s.igm1 + T(0.5) * d * (u * u + v * v + w * w) + T(0.5) * b2;
+    T vdotb = u * bn + v * by + w * bz;
+    T vdotbs = SM * bn + vys * bys + vzs * bzs;
+    T Es = (a * E - pt * u + pts * SM + bn * (vdotb - vdotbs)) * iS;
+    T sg = left ? -sgn : sgn;                         // ELss = ELs - ..., ERss = ERs + ...
+    T Ess = Es + sqd * (vdotbs - vdotbss) * sg;
+    bool outer = left ? (SLs >= T(0)) : !(SRs >= T(0));
+    T q[9] = {d, u, v, w, p, bn, by, bz, T(0)};
+    T F[9];
+    flux_x(q, F, s);
+    T U[9] = {d, d * u, d * v, d * w, E, bn, by, bz, T(0)};
+    T Us[9] = {ds, ds * SM, ds * vys, ds * vzs, Es, bn, bys, bzs, T(0)};
+    T Uss[9] = {ds, ds * SM, ds * vyss, ds * vzss, Ess, bn, byss, bzss, T(0)};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        T fo = F[k] + S * (Us[k] - U[k]);
+        T fi = F[k] + Ss * Uss[k] - (Ss - S) * Us[k] - S * U[k];
+        f[k] = outer ? fo : fi;
+    }
+}
+
+// Riemann dispatch in direction D: floors (glmmhd_turb.jl:280-281), GLM pair (:175-179,
+// :282-286), solver, rotation back.  RIEM: LLF / LLF-bound / HLL / HLLD (with the LLF switch).
+template <typename T, int RIEM, int D>
+GLM_DEV void riemann(const T Lq[9], const T Rq[9], T out[9], const Phys<T>& s)
+{
+    constexpr int A = D % 3, B = (D + 1) % 3, C = (D + 2) % 3;
+    T L[9] = {tmax(Lq[0], s.smallr), Lq[1 + A], Lq[1 + B], Lq[1 + C], tmax(Lq[4], s.pfl),
+              Lq[5 + A], Lq[5 + B], Lq[5 + C], Lq[8]};
+    T R[9] = {tmax(Rq[0], s.smallr), Rq[1 + A], Rq[1 + B], Rq[1 + C], tmax(Rq[4], s.pfl),
+              Rq[5 + A], Rq[5 + B], Rq[5 + C], Rq[8]};
+    T bns = T(0.5) * (L[5] + R[5]) - (R[8] - L[8]) * s.half_ich;
+    T psis = T(0.5) * (L[8] + R[8]) - T(0.5) * s.ch * (R[5] - L[5]);
+    L[5] = bns; R[5] = bns;
+    T f[9];
+    if (RIEM == RIEM_LLF) {
+        llf_x<T, false>(L, R, f, s);
+    } else if (RIEM == RIEM_LLFB) {
+        llf_x<T, true>(L, R, f, s);
+    } else if (RIEM == RIEM_HLL) {
+        hll_x(L, R, f, s);
+    } else {
+        bool uself = (s.llf_dmin > T(0) && tmin(L[0], R[0]) < s.llf_dmin) ||
+                     (s.llf_pmin > T(0) && tmin(L[4], R[4]) < s.llf_pmin);
+        if (uself) llf_x<T, false>(L, R, f, s);
+        else hlld_x(L, R, f, s);
+    }
+    out[0] = f[0];
+    out[1 + A] = f[1]; out[1 + B] = f[2]; out[1 + C] = f[3];
+    out[4] = f[4];
+    out[5 + A] = psis; out[5 + B] = f[6]; out[5 + C] = f[7];
+    out[8] = s.ch2 * bns;
+}
+
+}  // namespace glm

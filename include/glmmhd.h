@@ -1,0 +1,170 @@
+/*
+ * glmmhd.h -- C ABI of libglmmhd.so, the gfx950 (MI355X) GLM-MHD line-march library.
+ *
+ * Plain C: pointers, sizes and scalars only.  Device pointers are HIP device addresses
+ * (what AMDGPU.jl's pointer(::ROCArray), torch.Tensor.data_ptr() or hipMalloc return).
+ *
+ * State layout (all entry points): SoA planes, one contiguous plane per variable, cell (i,j,k)
+ * at i + nx*(j + ny*k) -- march_bridge/cu/spike_mhd.cu:48, MarchBridge.jl:64-81, and the memory
+ * layout of the Julia (N,N,N,9) array (glmmhd_turb.jl:2051).
+ *   GLM: 9 planes (rho, rho*vx, rho*vy, rho*vz, E, Bx, By, Bz, psi)       glmmhd_turb.jl:61-64
+ *   CT : 8 planes (rho, mx, my, mz, E, bxf, byf, bzf), bxf[i] = Bx on face i-1/2
+ *                                                                           ct_mhd.jl:10-11
+ * Paths in citations are relative to the reference repository root.
+ */
+#ifndef GLMMHD_H
+#define GLMMHD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================================== */
+/* 1. Legacy MarchBridge ABI (single precision state, library-global parameters).         */
+/*    Same symbols and C signatures as march_bridge/cu/spike_mhd.cu:319-341, so           */
+/*    MarchBridge.open_lib/set_dtdx/set_glm/set_gamma/run! (MarchBridge.jl:34-81) bind     */
+/*    unchanged.  The scheme is the spike march's: normal-only Hancock, PLM MonCen, HLL,   */
+/*    GLM cleaning and psi damping, smallr = smallp = 1e-6 (spike_mhd.cu:52, :257-288).    */
+/* ===================================================================================== */
+
+/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library) or 8 (CT library, spike_ctm.cu:301) */
+int march_nv(void);
+/* spike_mhd.cu:322  grid size N (compile-time NX in the reference; runtime here, see march_set_grid) */
+int march_nx(void);
+/* spike_mhd.cu:323  passive species count (0) */
+int march_nsp(void);
+/* spike_mhd.cu:324  fixed dt/dx of subsequent march_run_dev / ct_run calls */
+void march_set_dtdx(float v);
+/* spike_mhd.cu:325  Dedner cleaning speed ch and per-step psi decay factor */
+void march_set_glm(float ch, float fac);
+/* spike_mhd.cu:326  adiabatic index (default 1.4, spike_mhd.cu:52) */
+void march_set_gamma(float g);
+/* spike_mhd.cu:328-339  run nsteps periodic marches over device planes qv[0..8]; ov is scratch of
+ * the same shape; the result is always left in qv.  Returns elapsed GPU time in ms (HIP events). */
+double march_run_dev(float* const* qv, float* const* ov, int nsteps);
+
+/* No reference counterpart (the reference fixes NX/NY/NZ at compile time, build_mhd.sh:26-28):
+ * set the runtime grid used by march_run_dev / march_run_dev_f64 / ct_run.  Returns 0 or an
+ * error code. */
+int march_set_grid(int nx, int ny, int nz);
+/* FP64 twin of march_run_dev (BASELINE FP64 path; no reference counterpart). */
+double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps);
+
+/* spike_ctm.cu:305-325 CT march: qv/ov = 8 planes [U0..U4, bxf, byf, bzf]; cc = 3 Bcc scratch
+ * planes (accepted for ABI compatibility; the cell-centred B is formed in-kernel).  Uses the
+ * march_set_dtdx / march_set_gamma globals.  Returns elapsed GPU time in ms. */
+double ct_run(float* const* qv, float* const* ov, float* const* cc, int nsteps);
+/* spike_ctm.cu:326  registers of the CT march kernel (gfx950 arch VGPRs + AGPRs) */
+int ct_regs(void);
+/* spike_ctm.cu:327  LDS bytes per workgroup of the CT march kernel */
+int ct_shmem(void);
+
+/* ===================================================================================== */
+/* 2. Context API (re-entrant, per-context HIP stream, FP64 or FP32 state).               */
+/*    Mirrors the Julia step API: step_plm! (glmmhd_turb.jl:878-908), compute_dt          */
+/*    (:1973-1978), diagnostics (:2006-2018), the ICs (:2021-2036, :2164-2172) and the OU  */
+/*    forcing (Forcing / ou_advance!, :2117-2161).                                         */
+/* ===================================================================================== */
+
+typedef struct glm_ctx glm_ctx;
+
+/* scheme switches (values of glm_params fields) */
+enum { GLM_PRED_FULL = 0, GLM_PRED_ONED = 1 };                                  /* :165-172 | spike_mhd.cu:149-161 */
+enum { GLM_RECON_PLM = 0, GLM_RECON_PPM = 1, GLM_RECON_PPM_SPIKE = 2 };       /* :917-926 | spike_mhd.cu:162-192 */
+enum { GLM_SLOPE_MONCEN = 0, GLM_SLOPE_MINMOD = 1 };                           /* :145-155 | :300-303 */
+enum { GLM_RIEMANN_LLF = 0, GLM_RIEMANN_LLF_BOUND = 1, GLM_RIEMANN_HLL = 2, GLM_RIEMANN_HLLD = 3 };
+enum { GLM_GUARD_NONE = 0, GLM_GUARD_IFLO = 1, GLM_GUARD_CLAMP = 2 };          /* cpu_simd.jl | :618 | spike_mhd.cu:145 */
+enum { GLM_IC_ORSZAG_TANG = 0, GLM_IC_UNIFORM = 1, GLM_IC_GRADIENT = 2 };
+
+/* One step's scalars and switches; names follow Params (glmmhd_turb.jl:28-55). */
+typedef struct glm_params {
+    double gamma, smallr, smallc;
+    double pfl;                 /* pressure floor; <= 0 selects smallr*smallc^2/gamma (:55) */
+    double dt, dx;              /* time step and (cubic) cell size; box length per axis = n*dx */
+    double ch, glm_fac;         /* Dedner speed and psi decay factor (:884-885) */
+    double llf_dmin, llf_pmin;  /* HLLD -> LLF switch thresholds, <= 0 disables (:287-290) */
+    double turb_min_rho, ramp;  /* OU kick density floor and min(t/turb_T, 1) ramp (:886) */
+    int predictor, recon, slope, riemann;
+    int mh_revert;              /* predictor revert mh -> m0 (:393-395) */
+    int face_guard;             /* GLM_GUARD_* */
+    int update_floor;           /* static void-cell floor in the update (:448-453) */
+    int do_turb;                /* fused OU kick with the context's forcing field (:456-475) */
+    int zchunk;                 /* z planes per workgroup (0 = automatic) */
+    int want_cfl;               /* fold the next step's CFL max into the step (read with glm_cfl_max) */
+} glm_params;
+
+/* Fill a glm_params with the reference defaults (Params, glmmhd_turb.jl:28-52) and the cube
+ * scheme (full Hancock, PLM MonCen, HLLD, mh revert, iflo faces, update floor). */
+void glm_params_default(glm_params* p);
+
+/* Create a context for an nx*ny*nz grid; precision 8 (FP64 state) or 4 (FP32 state).
+ * Returns NULL on failure (see glm_last_error). */
+glm_ctx* glm_create(int nx, int ny, int nz, int precision);
+void glm_destroy(glm_ctx* ctx);
+/* Use a caller-owned HIP stream (hipStream_t passed as void*); NULL = the null stream. */
+int glm_set_stream(glm_ctx* ctx, void* stream);
+
+/* One step q -> o (step_plm!, glmmhd_turb.jl:878-908), asynchronous on the context stream.
+ * q, o: 9 device plane pointers each (distinct buffers).  Returns 0 or an error code. */
+int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
+/* Max over cells of sum_d(|v_d| + c_f) of the state written by the last glm_step that had
+ * want_cfl set (synchronises the stream).  dt = courant*dx/cmax (compute_dt, :1973-1978). */
+int glm_cfl_max(glm_ctx* ctx, double* cmax);
+/* Standalone CFL max of a state (ctot_kernel! + maximum, :1963-1976); synchronous. */
+int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
+
+#define GLM_NDIAG 10
+/* Diagnostics (:1996-2018 + energy budget), FP64 with compensated sums; synchronous.
+ * out: mass, sum m^2/max(rho,smallr), max|divB| (central), rho_min, finite, E_kin, E_mag,
+ *      sum m.B/rho (cross helicity), sum E, sum psi^2   (plain cell sums; times dV for integrals) */
+int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* out);
+
+/* Initial conditions: GLM_IC_ORSZAG_TANG args {gamma} (:2021-2036); GLM_IC_UNIFORM args
+ * {rho0, p0, b0x, b0y, b0z, gamma} (:2164-2172); GLM_IC_GRADIENT args {gamma}
+ * (spike_mhd.cu:304-317).  Asynchronous. */
+int glm_ic(glm_ctx* ctx, void* const* q, int kind, const double* args);
+
+/* OU forcing (Forcing/ou_advance!, :2117-2161) on a g^3 Fourier grid with a shell
+ * kmin <= |k| <= kmax, sqrt(P) = 1/|k|, compressive fraction comp_frac, correlation time T,
+ * ndt OU substeps per T, amplitude amp, seeded Philox4x32-10 increments. */
+int glm_ou_setup(glm_ctx* ctx, int g, double kmin, double kmax, double T, int ndt, double comp_frac, double amp,
+                 unsigned long long seed);
+/* One OU step of the Fourier field and a refresh of the real acceleration field (ou_advance!). */
+int glm_ou_advance(glm_ctx* ctx);
+/* Device pointer of the acceleration field (3 SoA planes of g^3, state precision). */
+const void* glm_ou_afield(glm_ctx* ctx);
+/* Copy the acceleration field (3*g^3 values, state precision) into the caller's device buffer dst,
+ * in stream order on the context stream. */
+int glm_ou_copy_afield(glm_ctx* ctx, void* dst);
+/* Number of OU steps taken so far. */
+long long glm_ou_steps(glm_ctx* ctx);
+
+/* CT step (ct_mhd.jl step!, :174-184; spike_ctm.cu ct_march): q, o = 8 planes.  Uses gamma,
+ * smallr, pfl (as SMALLP), dt, dx, riemann (HLL or HLLD or LLF), recon (PLM or PPM_SPIKE),
+ * update_floor, do_turb, turb_min_rho, smallc, ramp, want_cfl from p; ct_face_slope selects the
+ * spike_ctm.cu (0) or the ct_mhd.jl (1) face offset. */
+int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope);
+/* CFL max of a CT state (cell-centred B); synchronous. */
+int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
+/* max |div B| of a CT state from the face fields (ct_mhd.jl:191-194); synchronous. */
+int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
+
+/* Error handling (the reference has none: spike_mhd.cu's AS_LIB path never checks errors). */
+int glm_last_error(void);
+const char* glm_error_string(int code);
+
+/* Resource figures of the compiled march kernel for the given precision and scheme. */
+int glm_kernel_info(int precision, int predictor, int riemann, int* vgprs, int* lds_bytes);
+
+#define GLM_OK 0
+#define GLM_ERR_ARG 1
+#define GLM_ERR_HIP 2
+#define GLM_ERR_ALLOC 3
+#define GLM_ERR_STATE 4
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLMMHD_H */

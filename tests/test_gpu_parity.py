@@ -1,0 +1,319 @@
+"""GPU parity: the gfx950 kernels (through the C ABI) against the CPU oracle.
+
+Bar (FP64): relative L1 <= 1e-12 per conserved variable after N steps, with only reduction-order
+and FMA-contraction differences allowed.  FP32 paths are compared with the FP32 oracle at an
+FP32 tolerance.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import CTScheme, OUForcing, Scheme  # noqa: E402
+
+from glmmhdturb_amd import Params, Solver, _lib, glm_speed, pfloor, step_args  # noqa: E402
+
+TOL64 = 1e-12
+
+
+def rel_l1(a, b, nv=9, ch=None):
+    """Per-variable relative L1: sum|a_v - b_v| / max(sum|b_v|, floor_v).
+
+    The floor only matters for a variable that is small compared with the fluxes that update
+    it (momentum or transverse B generated from rest by the forcing, psi generated from
+    round-off-level div B): its error is then measured against the scale the scheme works at,
+    not against its own near-zero size.  Floors: momenta sum sqrt(rho |E|); field components
+    sum |B|; psi ch * sum |B| (psi has units of ch*B); anything else 1e-8 * sum rho.
+    For O(1) fields (every test state that starts away from rest) this is the plain relative L1."""
+    a = np.asarray(a, dtype=np.float64).reshape(nv, -1)
+    b = np.asarray(b, dtype=np.float64).reshape(nv, -1)
+    floor = np.full(nv, 1e-8 * np.abs(b[0]).sum())
+    babs = np.sqrt(b[5] ** 2 + b[6] ** 2 + b[7] ** 2).sum()
+    floor[1:4] = np.maximum(floor[1:4], np.sqrt(np.abs(b[0] * b[4])).sum())
+    floor[5:8] = np.maximum(floor[5:8], babs)
+    if nv == 9 and ch is not None:
+        floor[8] = max(floor[8], ch * babs)
+    return np.array([np.abs(a[v] - b[v]).sum() / max(np.abs(b[v]).sum(), floor[v]) for v in range(nv)])
+
+
+def smooth_state(orc, nx, ny, nz, seed=3, amp=0.3, bamp=0.4, gamma=5.0 / 3.0):
+    """A smooth, fully 3-D periodic state with all nine variables active."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    z = (np.arange(nz) + 0.5) / nz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    tp = 2 * np.pi
+
+    def f(k):
+        ph = rng.uniform(0, tp, 3)
+        return np.sin(tp * X + ph[0]) * np.cos(tp * Y + ph[1]) + 0.5 * np.sin(tp * (Y + Z) + ph[2]) + 0.3 * np.cos(tp * k * Z)
+
+    rho = 1.0 + amp * 0.5 * f(1)
+    vx, vy, vz = amp * f(1), amp * f(2), amp * f(1)
+    p = 1.0 + amp * 0.3 * f(2)
+    bx, by, bz = 0.5 + bamp * f(1), bamp * f(2), bamp * f(1)
+    psi = 0.01 * f(2)
+    q = np.stack([rho, vx, vy, vz, p, bx, by, bz, psi])
+    u = np.empty_like(q)
+    for idx in np.ndindex(nz, ny, nx):
+        u[(slice(None),) + idx] = orc.prim2cons(q[(slice(None),) + idx], gamma)
+    return u
+
+
+def to_dev(s, u):
+    t = s.new_state(u.shape[0])
+    t.copy_(torch.from_numpy(np.ascontiguousarray(u)).to(t.device, t.dtype))
+    return t
+
+
+def to_np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def oracle_scheme(a, **kw):
+    """Oracle scheme from the C parameter block the GPU step receives."""
+    inv = lambda d, v: [k for k, x in d.items() if x == v][0]  # noqa: E731
+    return Scheme(gamma=a.gamma, smallr=a.smallr, smallc=a.smallc, pfl=a.pfl, dt=a.dt, dx=a.dx, ch=a.ch,
+                  glm_fac=a.glm_fac, llf_dmin=a.llf_dmin, llf_pmin=a.llf_pmin, turb_min_rho=a.turb_min_rho,
+                  ramp=a.ramp, predictor=inv(_lib.PREDICTOR, a.predictor), recon=inv(_lib.RECON, a.recon),
+                  slope=inv(_lib.SLOPE, a.slope), riemann=inv(_lib.RIEMANN, a.riemann),
+                  mh_revert=bool(a.mh_revert), face_guard=inv(_lib.GUARD, a.face_guard),
+                  update_floor=bool(a.update_floor), do_turb=bool(a.do_turb), **kw)
+
+
+@pytest.fixture(scope="module")
+def grid3d(orc):
+    nx, ny, nz = 24, 20, 16          # not multiples of the 14x14 tile: partial tiles on both axes
+    p = Params(N=nx, nx=nx, ny=ny, nz=nz, gamma=5.0 / 3.0, courant=0.4)
+    s = Solver(p, "f64")
+    u0 = smooth_state(orc, nx, ny, nz)
+    return p, s, u0
+
+
+COMBOS = [(pr, rc, rm, sl) for pr in ("full", "oned") for rc in ("plm", "ppm", "ppm_spike")
+          for rm in ("llf", "llf_bound", "hll", "hlld") for sl in ("moncen",)]
+COMBOS += [("full", "plm", "hlld", "minmod"), ("oned", "ppm", "hll", "minmod")]
+
+
+@pytest.mark.parametrize("pred,recon,riem,slope", COMBOS)
+def test_step_matrix_fp64(orc, grid3d, pred, recon, riem, slope):
+    p, s, u0 = grid3d
+    dt = 0.3 * (1.0 / p.N) / orc.ctot_max(u0, p.gamma, p.smallr, pfloor(p))
+    uo = u0.copy()
+    ud = to_dev(s, u0)
+    un = s.new_state()
+    for it in range(3):
+        a = step_args(p, dt, 0.0, riemann=riem, recon=recon, predictor=pred, slope=slope, scheme="cube", want_cfl=False)
+        s.step(ud, un, a)
+        ud, un = un, ud
+        uo = orc.step(uo, oracle_scheme(a))
+    err = rel_l1(to_np(ud), uo, ch=a.ch)
+    assert np.all(err <= TOL64), (pred, recon, riem, slope, err)
+
+
+@pytest.mark.parametrize("scheme", ["multi", "cpu_simd", "spike"])
+def test_scheme_presets_fp64(orc, grid3d, scheme):
+    p, s, u0 = grid3d
+    dt = 0.3 * (1.0 / p.N) / orc.ctot_max(u0, p.gamma, p.smallr, pfloor(p))
+    a = step_args(p, dt, 0.0, scheme=scheme, riemann="hlld", recon="ppm", want_cfl=False)
+    ud, un = to_dev(s, u0), s.new_state()
+    uo = u0.copy()
+    for it in range(3):
+        s.step(ud, un, a)
+        ud, un = un, ud
+        uo = orc.step(uo, oracle_scheme(a))
+    assert np.all(rel_l1(to_np(ud), uo) <= TOL64)
+
+
+def test_zchunk_invariance(orc, grid3d):
+    """z-chunk boundaries recompute the same faces: results are bit-identical for any chunking."""
+    p, s, u0 = grid3d
+    dt = 1e-3
+    outs = []
+    for zc in (0, 1, 3, 5, 16):
+        a = step_args(p, dt, 0.0, riemann="hlld", recon="ppm", want_cfl=False, zchunk=zc)
+        ud, un = to_dev(s, u0), s.new_state()
+        s.step(ud, un, a)
+        outs.append(to_np(un))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
+def test_cfl_and_diagnostics(orc, grid3d):
+    p, s, u0 = grid3d
+    ud = to_dev(s, u0)
+    cm = s.ctot_max(ud)
+    co = orc.ctot_max(u0, p.gamma, p.smallr, pfloor(p))
+    assert abs(cm - co) <= 1e-14 * co
+    d = s.diagnostics(ud)
+    do = orc.diagnostics(u0, 1.0 / p.N, p.smallr)
+    for k in ("mass", "ke2", "ekin", "emag", "cross_helicity", "etot", "psi2"):
+        assert abs(d[k] - do[k]) <= 1e-13 * max(abs(do[k]), 1e-300), k
+    assert abs(d["divbmax"] - do["divbmax"]) <= 1e-13 * do["divbmax"]
+    assert d["rhomin"] == do["rhomin"] and d["finite"]
+    # fused CFL of the step equals the standalone CFL of the result
+    a = step_args(p, 1e-3, 0.0, riemann="hlld", recon="ppm", want_cfl=True)
+    un = s.new_state()
+    s.step(ud, un, a)
+    fused = s.cfl_max()
+    assert abs(fused - s.ctot_max(un)) <= 1e-14 * fused
+
+
+def test_initial_conditions(orc):
+    p = Params(N=32, nz=4, gamma=5.0 / 3.0)
+    s = Solver(p, "f64")
+    u = s.new_state()
+    s.ic_ot(u, p.gamma)
+    assert np.max(np.abs(to_np(u) - orc.ic_ot(32, 32, 4, p.gamma))) <= 1e-15
+    s.ic_uniform(u, 1.0, 0.6, (0.5, 0.1, -0.2), p.gamma)
+    assert np.array_equal(to_np(u), orc.ic_uniform(32, 32, 4, 1.0, 0.6, (0.5, 0.1, -0.2), p.gamma))
+    s.ic_gradient(u, p.gamma)
+    assert np.max(np.abs(to_np(u) - orc.ic_gradient(32, 32, 4, p.gamma))) <= 1e-15
+
+
+def test_orszag_tang_multistep(orc):
+    """Orszag-Tang 64x64x1, HLLD + PPM (cube scheme), 40 steps at the oracle's CFL dt."""
+    N = 64
+    p = Params(N=N, nz=1, gamma=5.0 / 3.0, courant=0.4)
+    s = Solver(p, "f64")
+    uo = orc.ic_ot(N, N, 1, p.gamma)
+    ud, un = to_dev(s, uo), s.new_state()
+    for it in range(40):
+        dt = p.courant * (1.0 / N) / orc.ctot_max(uo, p.gamma, p.smallr, pfloor(p))
+        a = step_args(p, dt, 0.0, riemann="hlld", recon="ppm", scheme="cube", want_cfl=False)
+        s.step(ud, un, a)
+        ud, un = un, ud
+        uo = orc.step(uo, oracle_scheme(a))
+    err = rel_l1(to_np(ud), uo, ch=a.ch)
+    assert np.all(err <= TOL64), err
+
+
+def test_forced_turbulence_parity(orc):
+    """OU-forced turbulence 24^3: forcing field and 12 steps (PPM + HLLD, kick on)."""
+    N = 24
+    p = Params(N=N, gamma=5.0 / 3.0, courant=0.3, turb_gs=16, turb_kmin=1.0, turb_kmax=3.0, comp_frac=0.0,
+               turb_T=0.5, turb_Ndt=100, turb_seed=7, b0=0.5)
+    s = Solver(p, "f64")
+    f = s.setup_forcing(p, amp=50.0)
+    fo = OUForcing(orc, g=16, kmin=1.0, kmax=3.0, T=0.5, ndt=100, comp_frac=0.0, amp=50.0, seed=7)
+    uo = orc.ic_uniform(N, N, N, 1.0, 0.6, (0.5, 0, 0), p.gamma)
+    ud, un = to_dev(s, uo), s.new_state()
+    t = 0.0
+    for it in range(12):
+        while t >= f.next_time:
+            f.advance(); fo.advance()
+        afd = s.afield().cpu().numpy().reshape(-1)
+        assert np.max(np.abs(afd - fo.afield)) <= 1e-12 * max(np.max(np.abs(fo.afield)), 1e-300)
+        dt = p.courant * (1.0 / N) / orc.ctot_max(uo, p.gamma, p.smallr, pfloor(p))
+        dt = min(dt, f.next_time - t)
+        a = step_args(p, dt, t + 0.05, do_turb=True, riemann="hlld", recon="ppm", want_cfl=False)
+        s.step(ud, un, a)
+        ud, un = un, ud
+        uo = orc.step(uo, oracle_scheme(a, turb_gs=16), afield=fo.afield)
+        t += dt
+    err = rel_l1(to_np(ud), uo, ch=a.ch)
+    assert np.all(err <= TOL64), err
+
+
+def test_fp32_step_vs_fp32_oracle(orc32, grid3d, orc):
+    p, _, u0 = grid3d
+    s = Solver(p, "f32")
+    dt = 0.3 * (1.0 / p.N) / orc.ctot_max(u0, p.gamma, p.smallr, pfloor(p))
+    a = step_args(p, dt, 0.0, riemann="hlld", recon="ppm", want_cfl=False)
+    u32 = u0.astype(np.float32)
+    ud, un = to_dev(s, u32), s.new_state()
+    uo = u32.copy()
+    for it in range(3):
+        s.step(ud, un, a)
+        ud, un = un, ud
+        uo = orc32.step(uo, oracle_scheme(a))
+    assert np.all(rel_l1(to_np(ud), uo) <= 2e-6)
+
+
+def test_legacy_march_run_dev(orc32, orc):
+    """march_run_dev (float**, spike scheme) against the FP32 oracle with the spike preset."""
+    from glmmhdturb_amd import march_bridge as MB
+    m = MB.open_lib(_lib.LIB_PATH)
+    assert m.NV == 9
+    N = 32
+    MB.set_grid(m, N, N, N)
+    MB.set_gamma(m, 5.0 / 3.0)
+    u0 = smooth_state(orc, N, N, N, seed=5).astype(np.float32)
+    dtdx, ch, fac = 0.05, 1.2, 0.99
+    MB.set_dtdx(m, dtdx)
+    MB.set_glm(m, ch, fac)
+    q = [torch.from_numpy(u0[v].copy()).cuda() for v in range(9)]
+    o = [torch.empty_like(x) for x in q]
+    ms = MB.run_(m, q, o, 3)
+    assert ms >= 0.0
+    sc = Scheme(gamma=5.0 / 3.0, smallr=1e-6, pfl=1e-6, dt=np.float32(dtdx), dx=1.0, ch=np.float32(ch),
+                glm_fac=np.float32(fac), predictor="oned", recon="plm", slope="moncen", riemann="hll",
+                mh_revert=False, face_guard="clamp", update_floor=False)
+    uo = u0.copy()
+    for it in range(3):
+        uo = orc32.step(uo, sc)
+    got = np.stack([x.cpu().numpy() for x in q])
+    assert np.all(rel_l1(got, uo) <= 2e-6)
+
+
+@pytest.mark.parametrize("riem,recon,fs", [("hll", "plm", 0), ("hll", "plm", 1), ("hlld", "plm", 0),
+                                           ("llf", "plm", 0), ("hll", "ppm_spike", 0)])
+def test_ct_step_parity(orc, riem, recon, fs):
+    nx, ny, nz = 24, 20, 16
+    p = Params(N=nx, nx=nx, ny=ny, nz=nz, gamma=5.0 / 3.0)
+    s = Solver(p, "f64")
+    rng = np.random.default_rng(11)
+    u = smooth_state(orc, nx, ny, nz, seed=11)[:8].copy()
+    # face B from a vector potential so that the face divergence is zero to round-off
+    x = np.arange(nx) / nx; y = np.arange(ny) / ny; z = np.arange(nz) / nz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    tp = 2 * np.pi
+    ax = 0.05 * np.sin(tp * (Y + Z)); ay = 0.05 * np.cos(tp * (X + 2 * Z)); az = 0.05 * np.sin(tp * (X - Y))
+    dx = 1.0 / nx
+    rl = lambda a, ax_: np.roll(a, -1, axis=ax_)  # noqa: E731  (index i+1)
+    # B = curl A with A on edges: bx = dAz/dy - dAy/dz etc. (face-centred, divergence-free)
+    u[5] = 0.5 + ((rl(az, 1) - az) - (rl(ay, 0) - ay)) / dx
+    u[6] = ((rl(ax, 0) - ax) - (rl(az, 2) - az)) / dx
+    u[7] = ((rl(ay, 2) - ay) - (rl(ax, 1) - ax)) / dx
+    assert orc.ct_divb_max(u, dx) < 1e-10
+    dt = 0.3 * dx / orc.ct_ctot_max(u, p.gamma, 1e-6, 1e-6)
+    a = step_args(p, dt, 0.0, riemann=riem, recon=recon, want_cfl=False)
+    a.pfl = 1e-6
+    a.update_floor = 0
+    cs = CTScheme(gamma=p.gamma, smallr=1e-6, smallp=1e-6, dt=dt, dx=dx, face_slope=fs, riemann=riem, recon=recon)
+    qd, qn = to_dev(s, u), s.new_state(8)
+    qo = u.copy()
+    for it in range(3):
+        s.ct_step(qd, qn, a, fs)
+        qd, qn = qn, qd
+        qo = orc.ct_step(qo, cs)
+    got = to_np(qd)
+    assert np.all(rel_l1(got, qo, 8) <= TOL64)
+    assert s.ct_divb_max(qd, dx) <= 1e-10
+
+
+def test_large_grid_invariants():
+    """256^3 FP64 PPM+HLLD: a uniform state is an exact fixed point, and on a smooth state the
+    totals of mass, momentum, energy and B are conserved to round-off (no floor or forcing hit)."""
+    N = 256
+    p = Params(N=N, gamma=5.0 / 3.0, courant=0.3)
+    s = Solver(p, "f64")
+    u = s.new_state()
+    s.ic_uniform(u, 1.0, 0.6, (0.5, 0.0, 0.0), p.gamma)
+    un = s.new_state()
+    a = step_args(p, 1e-4, 0.0, riemann="hlld", recon="ppm", want_cfl=True)
+    s.step(u, un, a)
+    assert torch.equal(u[:8], un[:8])
+    s.ic_gradient(u, p.gamma)
+    tot0 = u.reshape(9, -1).sum(dim=1)
+    dt = 0.3 * (1.0 / N) / s.ctot_max(u)
+    s.step(u, un, step_args(p, dt, 0.0, riemann="hlld", recon="ppm", want_cfl=False))
+    tot1 = un.reshape(9, -1).sum(dim=1)
+    scale = u.reshape(9, -1).abs().sum(dim=1)
+    for v in range(8):
+        assert abs(float(tot1[v] - tot0[v])) <= 1e-12 * float(scale[v]) + 1e-12, v

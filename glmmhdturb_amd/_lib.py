@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libglmmhd.so")
+# GLMMHD_LIB selects another build of the library (e.g. a libmhd<N>.so from build_mhd.sh).
+LIB_PATH = os.environ.get("GLMMHD_LIB") or os.path.join(HERE, "libglmmhd.so")
 
 PREDICTOR = {"full": 0, "oned": 1}
 RECON = {"plm": 0, "ppm": 1, "ppm_spike": 2}
@@ -75,7 +76,7 @@ def load(path: str | None = None):
         "glm_ct_ctot_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_ct_divb_max": (i, [vp, pp, d, C.POINTER(d)]),
         "glm_last_error": (i, []), "glm_error_string": (C.c_char_p, [i]),
-        "glm_kernel_info": (i, [i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "glm_kernel_info": (i, [i, i, i, i, i, C.POINTER(i), C.POINTER(i)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

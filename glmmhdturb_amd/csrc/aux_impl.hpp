@@ -25,9 +25,7 @@ __global__ void __launch_bounds__(AUX_NT) ctot_kernel(Ptr9<const T> q9, size_t n
 #pragma unroll
         for (int v = 0; v < 9; ++v) u[v] = q9[v][c];
         cons2prim(u, p, s);
-        T b2 = p[5] * p[5] + p[6] * p[6] + p[7] * p[7];
-        T ct = (tabs(p[1]) + fast_speed(p[0], p[4], b2, p[5], s)) + (tabs(p[2]) + fast_speed(p[0], p[4], b2, p[6], s)) +
-               (tabs(p[3]) + fast_speed(p[0], p[4], b2, p[7], s));
+        T ct = cfl_speed(p, s);
         if (!(ct <= cmax)) cmax = ct;
     }
     unsigned long long bits = (unsigned long long)__double_as_longlong((double)cmax);

@@ -212,7 +212,7 @@ static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_param
     a.want_cfl = p->want_cfl;
     a.cmax = c->d_cmax + c->cfl_parity;
     a.cmax_next = c->d_cmax + (1 - c->cfl_parity);
-    HIPCHK(launch_march<T>(a, p->predictor, p->riemann, c->stream));
+    HIPCHK(launch_march<T>(a, p->predictor, p->riemann, p->recon, p->slope, c->stream));
     if (p->want_cfl) {
         c->cfl_slot = c->cfl_parity;
         c->cfl_parity = 1 - c->cfl_parity;
@@ -437,12 +437,12 @@ const char* glm_error_string(int code)
     }
 }
 
-int glm_kernel_info(int precision, int predictor, int riemann, int* vgprs, int* lds_bytes)
+int glm_kernel_info(int precision, int predictor, int riemann, int recon, int slope, int* vgprs, int* lds_bytes)
 {
     hipFuncAttributes fa;
     const void* fn = nullptr;
-    if (precision == 8) fn = march_kernel_ptr<double>(predictor, riemann);
-    else if (precision == 4) fn = march_kernel_ptr<float>(predictor, riemann);
+    if (precision == 8) fn = march_kernel_ptr<double>(predictor, riemann, recon, slope);
+    else if (precision == 4) fn = march_kernel_ptr<float>(predictor, riemann, recon, slope);
     if (!fn) return set_err(GLM_ERR_ARG, "glm_kernel_info: unknown variant");
     HIPCHK(hipFuncGetAttributes(&fa, fn));
     if (vgprs) *vgprs = fa.numRegs;

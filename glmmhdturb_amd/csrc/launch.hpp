@@ -6,11 +6,35 @@
 
 namespace glm {
 
+// One (precision, predictor, Riemann) family of march kernels; the 3 reconstructions x 2 limiters
+// of a family are instantiated in one object built from march_inst.hip.
+template <typename T, int PRED, int RIEM>
+hipError_t launch_march_pr(const MarchArgs<T>& a, int recon, int slope, hipStream_t st);
+template <typename T, int PRED, int RIEM>
+const void* march_kernel_ptr_pr(int recon, int slope);
+
 template <typename T>
-hipError_t launch_march(const MarchArgs<T>& a, int pred, int riem, hipStream_t st);
+inline hipError_t launch_march(const MarchArgs<T>& a, int pred, int riem, int recon, int slope, hipStream_t st)
+{
+#define GLM_CASE(P, R) \
+    if (pred == P && riem == R) return launch_march_pr<T, P, R>(a, recon, slope, st);
+    GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
+    GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
+#undef GLM_CASE
+    return hipErrorInvalidValue;
+}
+
 // Kernel symbol of one march variant (for resource queries); nullptr if unknown.
 template <typename T>
-const void* march_kernel_ptr(int pred, int riem);
+inline const void* march_kernel_ptr(int pred, int riem, int recon, int slope)
+{
+#define GLM_CASE(P, R) \
+    if (pred == P && riem == R) return march_kernel_ptr_pr<T, P, R>(recon, slope);
+    GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
+    GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
+#undef GLM_CASE
+    return nullptr;
+}
 
 // Workgroup tiling of the march for an (nx, ny, nz) grid: tiles and z-chunks.
 inline void march_tiling(int nx, int ny, int nz, int zchunk_req, int& tiles_x, int& tiles_y, int& nzc, int& zchunk)

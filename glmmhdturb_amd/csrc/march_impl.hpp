@@ -30,6 +30,15 @@ constexpr int PX = TX + 2, PY = TY + 2;     // LDS primitive plane (2-cell halo)
 constexpr int NPL = PX * PY;
 constexpr int NRING = NPL - NT;             // outer halo ring loaded by helper threads
 
+// Minimum resident wavefronts per SIMD requested from the register allocator.
+#ifndef GLM_WAVES_F64
+#define GLM_WAVES_F64 1
+#endif
+#ifndef GLM_WAVES_F32
+#define GLM_WAVES_F32 2
+#endif
+#define GLM_WAVES_PER_EU(T) (sizeof(T) == 8 ? GLM_WAVES_F64 : GLM_WAVES_F32)
+
 template <typename T>
 struct MarchArgs {
     const T* q[9];
@@ -44,6 +53,17 @@ struct MarchArgs {
     const T* afield;
     unsigned long long* cmax;       // this step's max CFL speed, as the bits of a non-negative double
     unsigned long long* cmax_next;  // the slot the next step accumulates into (zeroed here)
+};
+
+// Arguments used only at the loads, stores and the update epilogue.  They are staged in LDS at
+// kernel entry so they do not hold scalar registers across the march loop.
+template <typename T>
+struct ColdArgs {
+    const T* q[9];
+    T* o[9];
+    const T* afield;
+    T dtdx, glm_fac, smallc, turb_min_rho, ramp, dt, dx, lx, ly, lz;
+    int update_floor, do_turb, tgs, pad_;
 };
 
 GLM_DEV int wrapi(int i, int n)
@@ -62,7 +82,7 @@ GLM_DEV void ring_xy(int t, int& hx, int& hy)
 }
 
 template <typename T>
-GLM_DEV void load9(const MarchArgs<T>& a, size_t off, T c[9])
+GLM_DEV void load9(const ColdArgs<T>& a, size_t off, T c[9])
 {
 #pragma unroll
     for (int v = 0; v < 9; ++v) c[v] = a.q[v][off];
@@ -73,9 +93,10 @@ template <typename T>
 GLM_DEV void guard(T f[9], const T m0[9], int g, const Phys<T>& s)
 {
     if (g == GUARD_IFLO) {
-        bool bad = f[0] < s.smallr || f[4] < s.pfl;
+        if (__builtin_expect(f[0] < s.smallr || f[4] < s.pfl, 0)) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) f[v] = bad ? m0[v] : f[v];
+            for (int v = 0; v < 9; ++v) f[v] = m0[v];
+        }
     } else if (g == GUARD_CLAMP) {
         f[0] = tmax(f[0], s.smallr);
         f[4] = tmax(f[4], s.pfl);
@@ -102,28 +123,29 @@ GLM_DEV void hanc1d(const T c0[9], const T hs[9], T mh[9], T half_dtdx, const Ph
 
 // Face states of one cell in direction D from its left/right neighbours l, r, the cell m0 and
 // the predicted mh (glmmhd_turb.jl:378-403, :955-973; spike_mhd.cu:262-275).
-template <typename T, int PRED, int D>
+template <typename T, int PRED, int RECON, int SLOPE, int D>
 GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9], bool up,
                        const MarchArgs<T>& a, T lo[9], T hi[9])
 {
     const Phys<T>& s = a.ph;
-    const bool mm = a.slope == SLOPE_MINMOD;
+    // half slopes: the PLM faces, the normal-only predictor, and the PPM shock fallback
     T hs[9];
+    if (RECON != RECON_PPM || PRED == PRED_ONED) {
 #pragma unroll
-    for (int v = 0; v < 9; ++v) hs[v] = half_slope(l[v], m0[v], r[v], mm);
+        for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(l[v], m0[v], r[v]);
+    }
     T mh[9];
     if (PRED == PRED_ONED) {
         hanc1d<T, D>(m0, hs, mh, a.half_dtdx, s);
-        if (a.mh_revert) {
-            bool bad = mh[0] <= s.smallr || mh[4] <= s.pfl;
+        if (a.mh_revert && __builtin_expect(mh[0] <= s.smallr || mh[4] <= s.pfl, 0)) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) mh[v] = bad ? m0[v] : mh[v];
+            for (int v = 0; v < 9; ++v) mh[v] = m0[v];
         }
     } else {
 #pragma unroll
         for (int v = 0; v < 9; ++v) mh[v] = mhf[v];
     }
-    if (a.recon == RECON_PPM && up) {
+    if (RECON == RECON_PPM) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
             T el, eh;
@@ -131,7 +153,15 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
             lo[v] = mh[v] + el - m0[v];
             hi[v] = mh[v] + eh - m0[v];
         }
-    } else if (a.recon == RECON_PPM_SPIKE) {
+        if (__builtin_expect(!up, 0)) {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) {
+                T h = PRED == PRED_ONED ? hs[v] : half_slope<T, SLOPE>(l[v], m0[v], r[v]);
+                lo[v] = mh[v] - h;
+                hi[v] = mh[v] + h;
+            }
+        }
+    } else if (RECON == RECON_PPM_SPIKE) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) ppm_edges_spike(l[v], m0[v], r[v], lo[v], hi[v]);
         lo[0] = tmax(lo[0], s.smallr); lo[4] = tmax(lo[4], s.pfl);
@@ -151,9 +181,9 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
 // Update epilogue of one cell (glmmhd_turb.jl:440-476), CFL speed of the result
 // (glmmhd_turb.jl:1963-1971).  acc = (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) + (Fz_lo - Fz_hi).
 template <typename T>
-GLM_DEV T update_cell(T r[9], const T acc[9], const MarchArgs<T>& a, int gx, int gy, int gz)
+GLM_DEV T update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl, int gx, int gy,
+                      int gz)
 {
-    const Phys<T>& s = a.ph;
 #pragma unroll
     for (int v = 0; v < 9; ++v) r[v] = r[v] + a.dtdx * acc[v];
     if (a.update_floor && r[0] < s.smallr) {
@@ -187,7 +217,7 @@ GLM_DEV T update_cell(T r[9], const T acc[9], const MarchArgs<T>& a, int gx, int
                     (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
             acc3[c] = a.ramp * t;
         }
-        T rho = r[0], rhom = tmax(rho, s.smallr), irm = T(1) / rhom, fl = rho * a.smallc * a.smallc;
+        T rho = r[0], rhom = tmax(rho, s.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc;
         T e = r[4];
         e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
         e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
@@ -200,23 +230,30 @@ GLM_DEV T update_cell(T r[9], const T acc[9], const MarchArgs<T>& a, int gx, int
         e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
         r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
     }
-    if (!a.want_cfl) return T(0);
+    if (!want_cfl) return T(0);
     T q[9];
     cons2prim(r, q, s);
-    T b2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
-    return (tabs(q[1]) + fast_speed(q[0], q[4], b2, q[5], s)) + (tabs(q[2]) + fast_speed(q[0], q[4], b2, q[6], s)) +
-           (tabs(q[3]) + fast_speed(q[0], q[4], b2, q[7], s));
+    return cfl_speed(q, s);
 }
 
-template <typename T, int PRED, int RIEM>
-__global__ void __launch_bounds__(NT) march_kernel(MarchArgs<T> a)
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) march_kernel(MarchArgs<T> a)
 {
     __shared__ T P[9][NPL];
     __shared__ T XB[9][NT];
     __shared__ T YB[9][NT];
+    __shared__ ColdArgs<T> ca;
 
     const Phys<T>& s = a.ph;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    if (tid == 0) {
+        for (int v = 0; v < 9; ++v) { ca.q[v] = a.q[v]; ca.o[v] = a.o[v]; }
+        ca.afield = a.afield;
+        ca.dtdx = a.dtdx; ca.glm_fac = a.glm_fac; ca.smallc = a.smallc; ca.turb_min_rho = a.turb_min_rho;
+        ca.ramp = a.ramp; ca.dt = a.dt; ca.dx = a.dx; ca.lx = a.lx; ca.ly = a.ly; ca.lz = a.lz;
+        ca.update_floor = a.update_floor; ca.do_turb = a.do_turb; ca.tgs = a.tgs;
+    }
+    __syncthreads();
     int b = blockIdx.x;
     const int tile_x = b % a.tiles_x; b /= a.tiles_x;
     const int tile_y = b % a.tiles_y; b /= a.tiles_y;
@@ -237,117 +274,119 @@ __global__ void __launch_bounds__(NT) march_kernel(MarchArgs<T> a)
 
     if (a.want_cfl && blockIdx.x == 0 && tid == 0) *a.cmax_next = 0ull;
 
+    // column window: primitives of planes q-1, q, q+1
     T pm[9], p0[9], pp[9];
     {
         T c[9];
-        load9(a, col + plane * wrapi(zs - 2, nz), c); cons2prim(c, pm, s);
-        load9(a, col + plane * wrapi(zs - 1, nz), c); cons2prim(c, p0, s);
-        load9(a, col + plane * wrapi(zs, nz), c); cons2prim(c, pp, s);
+        load9(ca, col + plane * wrapi(zs - 2, nz), c); cons2prim(c, pm, s);
+        load9(ca, col + plane * wrapi(zs - 1, nz), c); cons2prim(c, p0, s);
+        load9(ca, col + plane * wrapi(zs, nz), c); cons2prim(c, pp, s);
 #pragma unroll
         for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
         if (helper) {
             T h[9];
-            load9(a, hcol + plane * wrapi(zs - 1, nz), c); cons2prim(c, h, s);
+            load9(ca, hcol + plane * wrapi(zs - 1, nz), c); cons2prim(c, h, s);
 #pragma unroll
             for (int v = 0; v < 9; ++v) P[v][lo_ring] = h[v];
         }
     }
     __syncthreads();
 
-    T zprev[9], fzprev[9], dxyprev[9];
+    // zprev: +z face state of plane q-1.  pend: (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) of plane q-1 plus
+    // Fz(q-3/2), so the update of q-1 is U + dt/dx * (pend - Fz(q-1/2)).
+    T zprev[9], pend[9];
 #pragma unroll
-    for (int v = 0; v < 9; ++v) { zprev[v] = T(0); fzprev[v] = T(0); dxyprev[v] = T(0); }
+    for (int v = 0; v < 9; ++v) { zprev[v] = T(0); pend[v] = T(0); }
     T cmax = T(0);
 
     for (int q = zs - 1; q <= ze; ++q) {
         const bool last = q == ze;
         const bool inplane = q >= zs && q < ze;
+        const bool do_upd = owned && q >= zs + 1;
+
+        T u0[9];
+        if (do_upd) load9(ca, col + plane * wrapi(q - 1, nz), u0);
 
         // ---- reconstruction of this column's cell in plane q --------------------------------
-        // PPM strong-pressure-jump flag (glmmhd_turb.jl:959)
-        bool up = true;
-        if (a.recon == RECON_PPM) {
+        bool up = true;   // PPM strong-pressure-jump fallback flag (glmmhd_turb.jl:959)
+        if (RECON == RECON_PPM) {
             up = !(spj(P[4][lo_own - 1], p0[4], P[4][lo_own + 1]) || spj(P[4][lo_own - PX], p0[4], P[4][lo_own + PX]) ||
                    spj(pm[4], p0[4], pp[4]));
         }
         T mh[9];
         if (PRED == PRED_FULL) {
-            // glmmhd_turb.jl:165-172 hancock, :391-395 revert
-            const bool mm = a.slope == SLOPE_MINMOD;
-            T u0c[9], acc[9], t[9], fa[9], fb[9], hs[9];
-            prim2cons(p0, u0c, s);
+            // glmmhd_turb.jl:165-172 hancock, :391-395 revert: U0 - dt/(2dx) sum_d [F_d(m0+s_d) - F_d(m0-s_d)]
+            T acc[9], t[9], f[9], hs[9];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hs[v] = half_slope(P[v][lo_own - 1], p0[v], P[v][lo_own + 1], mm);
+            for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(P[v][lo_own - 1], p0[v], P[v][lo_own + 1]);
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
-            flux_dir<T, 0>(t, fa, s);
+            flux_dir<T, 0>(t, acc, s);
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] - hs[v];
-            flux_dir<T, 0>(t, fb, s);
+            flux_dir<T, 0>(t, f, s);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] = fa[v] - fb[v];
+            for (int v = 0; v < 9; ++v) acc[v] -= f[v];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hs[v] = half_slope(P[v][lo_own - PX], p0[v], P[v][lo_own + PX], mm);
+            for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(P[v][lo_own - PX], p0[v], P[v][lo_own + PX]);
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
-            flux_dir<T, 1>(t, fa, s);
+            flux_dir<T, 1>(t, f, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] += f[v];
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] - hs[v];
-            flux_dir<T, 1>(t, fb, s);
+            flux_dir<T, 1>(t, f, s);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] = acc[v] + (fa[v] - fb[v]);
+            for (int v = 0; v < 9; ++v) acc[v] -= f[v];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hs[v] = half_slope(pm[v], p0[v], pp[v], mm);
+            for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(pm[v], p0[v], pp[v]);
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
-            flux_dir<T, 2>(t, fa, s);
+            flux_dir<T, 2>(t, f, s);
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] += f[v];
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] - hs[v];
-            flux_dir<T, 2>(t, fb, s);
+            flux_dir<T, 2>(t, f, s);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] = acc[v] + (fa[v] - fb[v]);
+            for (int v = 0; v < 9; ++v) acc[v] -= f[v];
+            prim2cons(p0, t, s);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) t[v] = u0c[v] - a.half_dtdx * acc[v];
+            for (int v = 0; v < 9; ++v) t[v] = t[v] - a.half_dtdx * acc[v];
             cons2prim(t, mh, s);
-            if (a.mh_revert) {
-                bool bad = mh[0] <= s.smallr || mh[4] <= s.pfl;
+            if (a.mh_revert && __builtin_expect(mh[0] <= s.smallr || mh[4] <= s.pfl, 0)) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) mh[v] = bad ? p0[v] : mh[v];
+                for (int v = 0; v < 9; ++v) mh[v] = p0[v];
             }
         }
 
         // ---- z face q-1/2 and the update of plane q-1 ---------------------------------------
+        T fz[9];
         {
             T zlo[9], zhi[9];
-            faces_dir<T, PRED, 2>(pm, p0, pp, mh, up, a, zlo, zhi);
-            if (q >= zs && owned) {
-                T fz[9];
-                riemann<T, RIEM, 2>(zprev, zlo, fz, s);
-                if (q >= zs + 1) {
-                    // update plane q-1: acc = dxy(q-1) + (Fz(q-3/2) - Fz(q-1/2))
-                    T r[9], acc[9];
-                    load9(a, col + plane * wrapi(q - 1, nz), r);
-#pragma unroll
-                    for (int v = 0; v < 9; ++v) acc[v] = dxyprev[v] + (fzprev[v] - fz[v]);
-                    const int zq = wrapi(q - 1, nz);
-                    T ct = update_cell(r, acc, a, gx, gy, zq);
-                    if (!(ct <= cmax)) cmax = ct;
-                    const size_t off = col + plane * zq;
-#pragma unroll
-                    for (int v = 0; v < 9; ++v) __builtin_nontemporal_store(r[v], a.o[v] + off);
-                }
-#pragma unroll
-                for (int v = 0; v < 9; ++v) fzprev[v] = fz[v];
-            }
+            faces_dir<T, PRED, RECON, SLOPE, 2>(pm, p0, pp, mh, up, a, zlo, zhi);
+            if (q >= zs && owned) riemann<T, RIEM, 2>(zprev, zlo, fz, s);
 #pragma unroll
             for (int v = 0; v < 9; ++v) zprev[v] = zhi[v];
+        }
+        if (do_upd) {
+            T acc[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] = pend[v] - fz[v];
+            const int zq = wrapi(q - 1, nz);
+            T ct = update_cell(u0, acc, ca, s, a.want_cfl != 0, gx, gy, zq);
+            if (!(ct <= cmax)) cmax = ct;
+            const size_t off = col + plane * zq;
+#pragma unroll
+            for (int v = 0; v < 9; ++v) __builtin_nontemporal_store(u0[v], ca.o[v] + off);
         }
         if (last) break;
 
         // ---- prefetch the next planes ----------------------------------------------------------
         T cn[9], hr[9];
-        load9(a, col + plane * wrapi(q + 2, nz), cn);
-        if (helper) load9(a, hcol + plane * wrapi(q + 1, nz), hr);
+        load9(ca, col + plane * wrapi(q + 2, nz), cn);
+        if (helper) load9(ca, hcol + plane * wrapi(q + 1, nz), hr);
 
         // ---- x and y faces of plane q (only for planes this chunk updates) ---------------------
         if (inplane) {
@@ -356,12 +395,12 @@ __global__ void __launch_bounds__(NT) march_kernel(MarchArgs<T> a)
                 T l[9], r[9], hi[9];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - 1]; r[v] = P[v][lo_own + 1]; }
-                faces_dir<T, PRED, 0>(l, p0, r, mh, up, a, xlo, hi);
+                faces_dir<T, PRED, RECON, SLOPE, 0>(l, p0, r, mh, up, a, xlo, hi);
 #pragma unroll
                 for (int v = 0; v < 9; ++v) XB[v][tid] = hi[v];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - PX]; r[v] = P[v][lo_own + PX]; }
-                faces_dir<T, PRED, 1>(l, p0, r, mh, up, a, ylo, hi);
+                faces_dir<T, PRED, RECON, SLOPE, 1>(l, p0, r, mh, up, a, ylo, hi);
 #pragma unroll
                 for (int v = 0; v < 9; ++v) YB[v][tid] = hi[v];
             }
@@ -387,7 +426,7 @@ __global__ void __launch_bounds__(NT) march_kernel(MarchArgs<T> a)
             __syncthreads();
             if (owned) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) dxyprev[v] = (fx[v] - XB[v][tid + 1]) + (fy[v] - YB[v][tid + TX]);
+                for (int v = 0; v < 9; ++v) pend[v] = ((fx[v] - XB[v][tid + 1]) + (fy[v] - YB[v][tid + TX])) + fz[v];
             }
         }
 
@@ -409,8 +448,7 @@ __global__ void __launch_bounds__(NT) march_kernel(MarchArgs<T> a)
 
     if (a.want_cfl) {
         // wavefront max (NaN propagates as the largest bit pattern), one atomic per wavefront
-        double m = (double)cmax;
-        unsigned long long bits = __double_as_longlong(m);
+        unsigned long long bits = (unsigned long long)__double_as_longlong((double)cmax);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             unsigned long long o = __shfl_xor(bits, off, 64);

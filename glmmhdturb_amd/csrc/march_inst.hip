@@ -1,0 +1,37 @@
+// march_inst.hip -- one family of fused line-march kernels: precision GLM_T, predictor GLM_PRED,
+// Riemann solver GLM_RIEM (set by the build), all reconstructions x limiters.
+#include "launch.hpp"
+
+#ifndef GLM_T
+#error "build with -DGLM_T=double|float -DGLM_PRED=0|1 -DGLM_RIEM=0..3"
+#endif
+
+namespace glm {
+
+template <>
+hipError_t launch_march_pr<GLM_T, GLM_PRED, GLM_RIEM>(const MarchArgs<GLM_T>& a, int recon, int slope, hipStream_t st)
+{
+    dim3 grid((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), block(NT);
+#define GLM_CASE(RC, SL)                                                                             \
+    if (recon == RC && slope == SL) {                                                                \
+        hipLaunchKernelGGL((march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>), grid, block, 0, st, a); \
+        return hipGetLastError();                                                                    \
+    }
+    GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
+    GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
+#undef GLM_CASE
+    return hipErrorInvalidValue;
+}
+
+template <>
+const void* march_kernel_ptr_pr<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
+{
+#define GLM_CASE(RC, SL) \
+    if (recon == RC && slope == SL) return (const void*)&march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>;
+    GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
+    GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
+#undef GLM_CASE
+    return nullptr;
+}
+
+}  // namespace glm

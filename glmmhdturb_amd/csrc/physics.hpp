@@ -36,9 +36,34 @@ struct Ptr9 {
     GLM_DEV T* operator[](int i) const { return p[i]; }
 };
 
-template <typename T> GLM_DEV T tmax(T a, T b) { return a > b ? a : b; }
-template <typename T> GLM_DEV T tmin(T a, T b) { return a < b ? a : b; }
-template <typename T> GLM_DEV T tabs(T a) { return a < T(0) ? -a : a; }
+// fmax/fmin/fabs semantics (the C restatement's): one v_max/v_min instruction, |x| as a source
+// modifier, copysign as one bit-field insert.
+GLM_DEV double tmax(double a, double b) { return __builtin_fmax(a, b); }
+GLM_DEV float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+GLM_DEV double tmin(double a, double b) { return __builtin_fmin(a, b); }
+GLM_DEV float tmin(float a, float b) { return __builtin_fminf(a, b); }
+GLM_DEV double tabs(double a) { return __builtin_fabs(a); }
+GLM_DEV float tabs(float a) { return __builtin_fabsf(a); }
+GLM_DEV double tcopysign(double a, double b) { return __builtin_copysign(a, b); }
+GLM_DEV float tcopysign(float a, float b) { return __builtin_copysignf(a, b); }
+
+// Reciprocal from the hardware estimate refined by Newton steps (two for double, one for float):
+// within about 1 ulp of 1/x for the normal-range operands of this solver, at half the cost of
+// an IEEE divide sequence.  Used where the quotient feeds further arithmetic.
+GLM_DEV double trcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+GLM_DEV float trcp(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
 GLM_DEV double tsqrt(double a) { return __builtin_sqrt(a); }
 GLM_DEV float tsqrt(float a) { return __builtin_sqrtf(a); }
 
@@ -47,7 +72,7 @@ template <typename T>
 GLM_DEV void cons2prim(const T c[9], T q[9], const Phys<T>& s)
 {
     T rho = tmax(c[0], s.smallr);
-    T ir = T(1) / rho;
+    T ir = trcp(rho);
     T ekin = T(0.5) * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
     T emag = T(0.5) * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
     q[0] = rho;
@@ -72,11 +97,27 @@ GLM_DEV void prim2cons(const T q[9], T c[9], const Phys<T>& s)
 template <typename T>
 GLM_DEV T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
 {
-    T ir = T(1) / rho;
+    T ir = trcp(rho);
     T c2 = s.gamma * p * ir;
     T bb = b2 * ir;
     T d2 = T(0.5) * (bb + c2);
     return tsqrt(d2 + tsqrt(tmax(d2 * d2 - c2 * bn * bn * ir, T(0))));
+}
+
+// CFL signal speed sum_d (|v_d| + c_f(B_d)) of a primitive state (glmmhd_turb.jl:1963-1971),
+// sharing 1/rho, a^2 and the half sum between the three directions.
+template <typename T>
+GLM_DEV T cfl_speed(const T q[9], const Phys<T>& s)
+{
+    T ir = trcp(q[0]);
+    T c2 = s.gamma * q[4] * ir;
+    T bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
+    T d2 = T(0.5) * (bb + c2);
+    T dd = d2 * d2, k = c2 * ir;
+    T fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], T(0))));
+    T fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], T(0))));
+    T fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], T(0))));
+    return (tabs(q[1]) + fx) + (tabs(q[2]) + fy) + (tabs(q[3]) + fz);
 }
 
 // Ideal-MHD flux of a primitive state in direction D (glmmhd_turb.jl:98-115 evaluated in the
@@ -111,22 +152,23 @@ GLM_DEV void flux_x(const T q[9], T f[9], const Phys<T>& s) { flux_dir<T, 0>(q, 
 template <typename T>
 GLM_DEV T moncen(T dl, T dr)
 {
+    // sgn(dc) * m; at dc = -0 this gives -0 where the reference's sgn gives +0 (m = 0 there),
+    // which no later sum can tell apart
     T dc = T(0.5) * (dl + dr);
     T m = tmin(T(2) * tmin(tabs(dl), tabs(dr)), tabs(dc));
-    T val = dc >= T(0) ? m : -m;
-    return (dl * dr <= T(0)) ? T(0) : val;
+    return (dl * dr <= T(0)) ? T(0) : tcopysign(m, dc);
 }
 template <typename T>
 GLM_DEV T minmod(T a, T b)
 {
     return (a * b <= T(0)) ? T(0) : (tabs(a) < tabs(b) ? a : b);
 }
-// Half slope (glmmhd_turb.jl:153-155 / :302-303); `mm` selects minmod.
-template <typename T>
-GLM_DEV T half_slope(T l, T m, T r, bool mm)
+// Half slope (glmmhd_turb.jl:153-155 MonCen / :302-303 minmod).
+template <typename T, int SLOPE>
+GLM_DEV T half_slope(T l, T m, T r)
 {
     T dl = m - l, dr = r - m;
-    return T(0.5) * (mm ? minmod(dl, dr) : moncen(dl, dr));
+    return T(0.5) * (SLOPE == SLOPE_MINMOD ? minmod(dl, dr) : moncen(dl, dr));
 }
 
 // glmmhd_turb.jl:917-923 ppm_mono / ppm_edges (Julia form).
@@ -183,8 +225,8 @@ GLM_DEV void llf_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     T b2R = R[5] * R[5] + R[6] * R[6] + R[7] * R[7];
     T cL, cR;
     if (BOUND) {
-        cL = tsqrt((s.gamma * L[4] + b2L) * (T(1) / L[0]));
-        cR = tsqrt((s.gamma * R[4] + b2R) * (T(1) / R[0]));
+        cL = tsqrt((s.gamma * L[4] + b2L) * trcp(L[0]));
+        cR = tsqrt((s.gamma * R[4] + b2R) * trcp(R[0]));
     } else {
         cL = fast_speed(L[0], L[4], b2L, L[5], s);
         cR = fast_speed(R[0], R[4], b2R, R[5], s);
@@ -209,7 +251,7 @@ GLM_DEV void hll_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     T fL[9], fR[9], uL[9], uR[9];
     flux_x(L, fL, s); flux_x(R, fR, s);
     prim2cons(L, uL, s); prim2cons(R, uR, s);
-    T ihd = T(1) / (SR - SL);
+    T ihd = trcp(SR - SL);
     T slsr = SL * SR;
 #pragma unroll
     for (int v = 0; v < 9; ++v) f[v] = (SR * fL[v] - SL * fR[v] + slsr * (uR[v] - uL[v])) * ihd;
@@ -233,22 +275,22 @@ GLM_DEV void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     T SL = tmin(tmin(uL, uR) - cfm, T(0));
     T SR = tmax(tmax(uL, uR) + cfm, T(0));
     T aL = SL - uL, aR = SR - uR;
-    T iden = T(1) / (aR * dR - aL * dL);
+    T iden = trcp(aR * dR - aL * dL);
     T SM = (aR * dR * uR - aL * dL * uL - ptR + ptL) * iden;
     T pts = (aR * dR * ptL - aL * dL * ptR + dL * dR * aR * aL * (uR - uL)) * iden;
-    T iL = T(1) / (SL - SM), iR = T(1) / (SR - SM);
+    T iL = trcp(SL - SM), iR = trcp(SR - SM);
     T dLs = tmax(dL * aL * iL, T(1e-12));
     T dRs = tmax(dR * aR * iR, T(1e-12));
     T sqdLs = tsqrt(dLs), sqdRs = tsqrt(dRs);
     T absbn = tabs(bn);
-    T SLs = SM - absbn / sqdLs;
-    T SRs = SM + absbn / sqdRs;
+    T SLs = SM - absbn * trcp(sqdLs);
+    T SRs = SM + absbn * trcp(sqdRs);
     // star states (glmmhd_turb.jl:227-239)
     T vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
     {
         T denK = dL * aL * (SL - SM) - bn * bn;
         bool sm = tabs(denK) < T(1e-12);
-        T ik = T(1) / denK;
+        T ik = trcp(denK);
         T fac = bn * (SM - uL) * ik;
         T gb = (dL * aL * aL - bn * bn) * ik;
         vyLs = sm ? vL : vL - byL * fac;
@@ -259,7 +301,7 @@ GLM_DEV void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     {
         T denK = dR * aR * (SR - SM) - bn * bn;
         bool sm = tabs(denK) < T(1e-12);
-        T ik = T(1) / denK;
+        T ik = trcp(denK);
         T fac = bn * (SM - uR) * ik;
         T gb = (dR * aR * aR - bn * bn) * ik;
         vyRs = sm ? vR : vR - byR * fac;
@@ -268,7 +310,7 @@ GLM_DEV void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
         bzRs = sm ? bzR : bzR * gb;
     }
     T sgn = bn >= T(0) ? T(1) : T(-1);
-    T invsum = T(1) / (sqdLs + sqdRs);
+    T invsum = trcp(sqdLs + sqdRs);
     T vyss = (sqdLs * vyLs + sqdRs * vyRs + (byRs - byLs) * sgn) * invsum;
     T vzss = (sqdLs * vzLs + sqdRs * vzRs + (bzRs - bzLs) * sgn) * invsum;
     T byss = (sqdLs * byRs + sqdRs * byLs + sqdLs * sqdRs * (vyRs - vyLs) * sgn) * invsum;

@@ -156,7 +156,7 @@ int glm_last_error(void);
 const char* glm_error_string(int code);
 
 /* Resource figures of the compiled march kernel for the given precision and scheme. */
-int glm_kernel_info(int precision, int predictor, int riemann, int* vgprs, int* lds_bytes);
+int glm_kernel_info(int precision, int predictor, int riemann, int recon, int slope, int* vgprs, int* lds_bytes);
 
 #define GLM_OK 0
 #define GLM_ERR_ARG 1

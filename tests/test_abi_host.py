@@ -1,0 +1,141 @@
+"""CPU tests of the boundary and the host logic: the C-ABI library loads and exports every symbol
+include/glmmhd.h declares, the ctypes structs match the header, and the host-side step logic
+(Dedner speed, psi decay, forcing ramp, scheme presets, Params defaults) follows the reference.
+No GPU calls."""
+import ctypes as C
+import math
+import os
+import re
+import subprocess
+
+import pytest
+
+from glmmhdturb_amd import Params, SCHEMES, _lib, dxof, glm_speed, pfloor, ramp_of, step_args
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "glmmhd.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\([^;{]*\)\s*;", txt, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_reference_bridge_symbols():
+    names = header_functions()
+    # MarchBridge.jl:34-61 resolves these by name (spike_mhd.cu:319-341); spike_ctm.cu:299-329
+    for sym in ("march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_glm", "march_set_gamma",
+                "march_run_dev", "ct_run", "ct_regs", "ct_shmem"):
+        assert sym in names, sym
+    assert set(names) == set(_lib.EXPORTS), set(names) ^ set(_lib.EXPORTS)
+
+
+def test_library_exports_every_header_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "build libglmmhd.so first (make -C glmmhdturb_amd)"
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [n for n in header_functions() if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_binds_without_gpu_calls():
+    lib = _lib.load()
+    for name in _lib.EXPORTS:
+        assert getattr(lib, name) is not None
+    assert lib.march_nv() == 9 and lib.march_nsp() == 0
+
+
+def test_params_struct_matches_header():
+    """glm_params field order and types as declared in include/glmmhd.h."""
+    txt = open(HEADER).read()
+    body = txt[txt.index("typedef struct glm_params {"):txt.index("} glm_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for ctype, names in re.findall(r"\b(double|int)\s+([^;]+);", body):
+        for n in names.split(","):
+            fields.append((n.strip(), C.c_double if ctype == "double" else C.c_int))
+    assert [(n, t) for n, t in _lib.GLMParams._fields_] == fields
+    lib = _lib.load()
+    p = _lib.GLMParams()
+    lib.glm_params_default(C.byref(p))
+    # reference defaults (glmmhd_turb.jl:28-52)
+    assert p.gamma == 1.01 and p.smallr == 1e-6 and p.smallc == 1e-4 and p.turb_min_rho == 1e-5
+    assert p.riemann == 3 and p.predictor == 0 and p.face_guard == 1 and p.update_floor == 1
+
+
+def test_params_defaults_follow_reference():
+    p = Params()
+    assert (p.N, p.boxlen, p.gamma, p.courant, p.smallr, p.smallc) == (256, 1.0, 1.01, 0.7, 1e-6, 1e-4)
+    assert (p.glm_ch_scale, p.glm_cp_coef, p.turb_T, p.turb_Ndt, p.comp_frac) == (0.25, 0.18, 0.3, 100, 0.5)
+    assert dxof(p) == 1.0 / 256
+    assert pfloor(p) == pytest.approx(1e-6 * 1e-8 / 1.01, rel=1e-15)
+
+
+def test_glm_speed_and_ramp():
+    """ch = courant*dx/dt/NDIM*glm_ch_scale; glm_fac = exp(-(ch^2/(cp*L*ch))*dt) (glmmhd_turb.jl:884-886)."""
+    p = Params(N=128, courant=0.4)
+    dt = 1.3e-3
+    ch, fac = glm_speed(p, dt)
+    assert ch == pytest.approx(0.4 * (1 / 128) / dt / 3 * 0.25, rel=1e-15)
+    assert fac == pytest.approx(math.exp(-ch / 0.18 * dt), rel=1e-14)
+    assert glm_speed(Params(glm_cp_coef=0.0), dt)[1] == 1.0
+    assert ramp_of(p, 0.15) == pytest.approx(0.5) and ramp_of(p, 10.0) == 1.0
+
+
+@pytest.mark.parametrize("scheme", sorted(SCHEMES))
+def test_step_args_presets(scheme):
+    p = Params(N=64, gamma=5 / 3)
+    a = step_args(p, 1e-3, 0.05, scheme=scheme, riemann="hlld", recon="ppm", do_turb=True)
+    sc = SCHEMES[scheme]
+    assert a.predictor == _lib.PREDICTOR[sc["predictor"]]
+    assert a.face_guard == _lib.GUARD[sc["face_guard"]]
+    assert a.mh_revert == int(sc["mh_revert"]) and a.update_floor == int(sc["update_floor"])
+    if "riemann" in sc:
+        assert a.riemann == _lib.RIEMANN[sc["riemann"]]
+    else:
+        assert a.riemann == _lib.RIEMANN["hlld"] and a.recon == _lib.RECON["ppm"]
+    assert a.dt == 1e-3 and a.dx == 1 / 64 and a.do_turb == 1
+    assert a.ramp == pytest.approx(0.05 / 0.3)
+
+
+def test_solver_requires_gpu_and_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from glmmhdturb_amd import Solver
+    with pytest.raises(_lib.GLMError):
+        Solver(Params(N=8))
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(_lib.GLMError):
+        _lib.load(str(tmp_path / "libglmmhd.so"))
+
+
+def test_march_bridge_mirror_shapes():
+    """MarchBridge.run! asserts NV and (N,N,N) shapes before any device call (MarchBridge.jl:74-75)."""
+    from glmmhdturb_amd import march_bridge as MB
+    m = MB.open_lib(_lib.LIB_PATH)
+    assert m.NV == 9 and m.f_glm is not None
+
+    class Fake:
+        def __init__(self, shape):
+            self.shape = shape
+
+        def data_ptr(self):
+            return 0
+
+    with pytest.raises(AssertionError):
+        MB.run_(m, [Fake((m.N, m.N, m.N))] * 8, [Fake((m.N, m.N, m.N))] * 8, 1)
+    with pytest.raises(AssertionError):
+        MB.run_(m, [Fake((4, 4, 4))] * 9, [Fake((4, 4, 4))] * 9, 1)
+
+
+def test_build_scripts_reference_hipcc_for_gfx950():
+    mk = open(os.path.join(ROOT, "glmmhdturb_amd", "Makefile")).read()
+    assert "--offload-arch=$(ARCH)" in mk and "ARCH   ?= gfx950" in mk
+    for script in ("build_mhd.sh", "build_ct.sh"):
+        txt = open(os.path.join(ROOT, "glmmhdturb_amd", script)).read()
+        assert "legacy" in txt and "gfx950" in txt

@@ -1,0 +1,541 @@
+"""Pins of the CPU oracle (no GPU): closed forms, invariants, brute force and textbook results.
+
+Each test checks the oracle against something the mathematics fixes independently of the
+oracle's own formulas (eigenvalues of the flux Jacobian, exact Riemann solutions, cell averages
+of polynomials, brute-force sums and DFTs, published random-number known answers), chosen so a
+dropped term, a wrong sign or index, or a transposed operand fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import OUForcing, Scheme, twiddles
+
+G = 5.0 / 3.0
+SMALLR, PFL = 1e-6, 1e-6 * 1e-8 / G
+
+
+def rand_prim(rng, bn=None):
+    q = np.array([rng.uniform(0.5, 2.0), *rng.normal(0, 0.6, 3), rng.uniform(0.3, 2.0), *rng.normal(0, 0.8, 3),
+                  rng.normal(0, 0.1)])
+    if bn is not None:
+        q[5] = bn
+    return q
+
+
+def textbook_flux_x(q, g=G):
+    """Ideal-MHD x-flux (e.g. Toro / Stone et al. 2008) written from the conservation laws."""
+    r, u, v, w, p, bx, by, bz, psi = q
+    b2 = bx * bx + by * by + bz * bz
+    pt = p + 0.5 * b2
+    E = p / (g - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * b2
+    return np.array([r * u, r * u * u + pt - bx * bx, r * u * v - bx * by, r * u * w - bx * bz,
+                     (E + pt) * u - bx * (u * bx + v * by + w * bz), 0.0, u * by - v * bx, u * bz - w * bx, 0.0])
+
+
+# ----------------------------------------------------------------------------------------------
+# a3/a4 cons <-> prim
+# ----------------------------------------------------------------------------------------------
+def test_cons_prim_roundtrip_and_floors(orc):
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        q = rand_prim(rng)
+        c = orc.prim2cons(q, G)
+        q2 = orc.cons2prim(c, G, SMALLR, PFL)
+        assert np.allclose(q2, q, rtol=1e-13, atol=1e-14)
+    # density floor and pressure floor
+    q = orc.cons2prim([1e-9, 0, 0, 0, 1.0, 0, 0, 0, 0], G, SMALLR, PFL)
+    assert q[0] == SMALLR
+    q = orc.cons2prim([1.0, 2.0, 0, 0, 1.0, 0, 0, 0, 0], G, SMALLR, PFL)   # E < kinetic -> p floored
+    assert q[4] == PFL
+    # kinetic and magnetic parts of E
+    c = orc.prim2cons([2.0, 1.0, 0, 0, 0.0, 0, 3.0, 0, 0], G)
+    assert c[4] == pytest.approx(0.5 * 2.0 * 1.0 + 0.5 * 9.0, rel=1e-15)
+
+
+# ----------------------------------------------------------------------------------------------
+# a5/a6 fast speed and flux: closed forms, and the flux Jacobian's eigenvalues
+# ----------------------------------------------------------------------------------------------
+def test_fast_speed_special_cases(orc):
+    r, p = 1.3, 0.7
+    a = math.sqrt(G * p / r)
+    # B = 0: sound speed
+    assert orc.fast_speed([r, 0, 0, 0, p, 0, 0, 0, 0], G, 0.0) == pytest.approx(a, rel=1e-15)
+    # B along the normal: max(a, |B|/sqrt(rho))
+    for b in (0.3, 2.0):
+        assert orc.fast_speed([r, 0, 0, 0, p, b, 0, 0, 0], G, b) == pytest.approx(max(a, b / math.sqrt(r)), rel=1e-14)
+    # B perpendicular: sqrt(a^2 + b^2/rho)
+    assert orc.fast_speed([r, 0, 0, 0, p, 0, 0.9, 0.4, 0], G, 0.0) == pytest.approx(
+        math.sqrt(a * a + (0.81 + 0.16) / r), rel=1e-14)
+
+
+def test_flux_matches_textbook_and_rotations(orc):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        q = rand_prim(rng)
+        assert np.allclose(orc.phys_flux_x(q, G), textbook_flux_x(q), rtol=1e-14, atol=1e-14)
+        # y and z fluxes: rotate so the direction becomes x, take the x-flux, rotate back; compare
+        # with the textbook flux of the axis-permuted state mapped back by hand
+        for d, perm in ((1, [1, 2, 0]), (2, [2, 0, 1])):
+            f = orc.rot_flux_from(orc.phys_flux_x(orc.rot_to(q, d), G), d)
+            qp = q.copy()
+            qp[1:4] = q[1:4][perm]
+            qp[5:8] = q[5:8][perm]
+            ft = textbook_flux_x(qp)
+            back = np.empty(3); back[perm] = ft[1:4]
+            backb = np.empty(3); backb[perm] = ft[5:8]
+            assert np.allclose(f[1:4], back, rtol=1e-14, atol=1e-14)
+            assert np.allclose(f[5:8], backb, rtol=1e-14, atol=1e-14)
+            assert f[0] == pytest.approx(ft[0], rel=1e-14, abs=1e-14)
+            assert f[4] == pytest.approx(ft[4], rel=1e-14, abs=1e-14)
+
+
+def test_flux_jacobian_eigenvalues_are_mhd_wave_speeds(orc):
+    """The x-flux Jacobian in conserved variables (Bx fixed) has eigenvalues u, u+-c_a, u+-c_s,
+    u+-c_f with the textbook slow/Alfven/fast speeds; c_f must equal fast_speed."""
+    rng = np.random.default_rng(2)
+    for _ in range(10):
+        q = rand_prim(rng)
+        c0 = orc.prim2cons(q, G)
+        idx = [0, 1, 2, 3, 4, 6, 7]            # conserved variables evolved in x (Bx and psi aside)
+        J = np.zeros((7, 7))
+        for jj, k in enumerate(idx):
+            h = 1e-6 * max(1.0, abs(c0[k]))
+            cp, cm = c0.copy(), c0.copy()
+            cp[k] += h; cm[k] -= h
+            fp = orc.phys_flux_x(orc.cons2prim(cp, G, SMALLR, PFL), G)
+            fm = orc.phys_flux_x(orc.cons2prim(cm, G, SMALLR, PFL), G)
+            J[:, jj] = ((fp - fm) / (2 * h))[idx]
+        ev = np.sort(np.linalg.eigvals(J).real)
+        r, u, p, bx = q[0], q[1], q[4], q[5]
+        a2 = G * p / r
+        b2 = (q[5] ** 2 + q[6] ** 2 + q[7] ** 2) / r
+        ca = abs(bx) / math.sqrt(r)
+        disc = math.sqrt(max((a2 + b2) ** 2 - 4 * a2 * bx * bx / r, 0))
+        cf = math.sqrt(0.5 * (a2 + b2 + disc))
+        cs = math.sqrt(max(0.5 * (a2 + b2 - disc), 0))
+        ref = np.sort([u - cf, u - ca, u - cs, u, u + cs, u + ca, u + cf])
+        assert np.allclose(ev, ref, atol=2e-6 * (abs(u) + cf))
+        assert orc.fast_speed(q, G, bx) == pytest.approx(cf, rel=1e-13)
+
+
+# ----------------------------------------------------------------------------------------------
+# a7/a8 limiters
+# ----------------------------------------------------------------------------------------------
+def test_limiters_truth_tables(orc):
+    vals = [-3.0, -2.0, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0, 3.0]
+    for dl in vals:
+        for dr in vals:
+            mc = orc.moncen(dl, dr)
+            mm = orc.minmod(dl, dr)
+            if dl * dr <= 0:
+                assert mc == 0 and mm == 0
+            else:
+                s = 1.0 if dl > 0 else -1.0
+                assert mc == s * min(2 * abs(dl), 2 * abs(dr), abs(0.5 * (dl + dr)))
+                assert mm == (dl if abs(dl) < abs(dr) else dr)
+            assert orc.moncen(dl, dr) == orc.moncen(dr, dl)     # symmetric
+            assert abs(mc) <= 2 * min(abs(dl), abs(dr)) + 1e-15  # TVD region
+
+
+# ----------------------------------------------------------------------------------------------
+# a9/a10 predictors
+# ----------------------------------------------------------------------------------------------
+def test_hancock_uniform_fixed_point_and_taylor_term(orc):
+    q = np.array([1.2, 0.3, -0.2, 0.1, 0.8, 0.5, 0.2, -0.1, 0.01])
+    z = np.zeros(9)
+    uh = orc.hancock(q, z, z, z, 0.1, G)
+    assert np.array_equal(uh, orc.prim2cons(q, G))
+    for d in range(3):
+        assert np.array_equal(orc.hanc1d(q, z, d, 0.1, G, SMALLR, PFL), orc.cons2prim(orc.prim2cons(q, G), G, SMALLR, PFL))
+    # entropy wave: only rho varies linearly along x, u, p, B uniform, Bx only -> rho_half = rho - dtdx/2 * u * (2 s)
+    q = np.array([1.0, 0.7, 0, 0, 1.0, 0.4, 0, 0, 0])
+    s = np.zeros(9); s[0] = 0.05                 # half slope of rho
+    dtdx = 0.2
+    mh = orc.cons2prim(orc.hancock(q, s, z, z, dtdx, G), G, SMALLR, PFL)
+    assert mh[0] == pytest.approx(1.0 - 0.5 * dtdx * 0.7 * 2 * 0.05, rel=1e-13)
+    assert mh[1] == pytest.approx(0.7, rel=1e-12) and mh[4] == pytest.approx(1.0, rel=1e-12)
+    mh1 = orc.hanc1d(q, s, 0, dtdx, G, SMALLR, PFL)
+    assert np.allclose(mh1, mh, rtol=1e-13)
+
+
+# ----------------------------------------------------------------------------------------------
+# a11 PPM
+# ----------------------------------------------------------------------------------------------
+def test_ppm_julia_exact_for_quadratic_cell_averages(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        a, b, c = rng.normal(size=3)
+        b = abs(b) + 2 * abs(c) + 0.1          # monotone over the stencil: no limiting
+        avg = lambda x0: a + b * x0 + c * (x0 * x0 + 1.0 / 12.0)  # noqa: E731  average over [x0-1/2, x0+1/2]
+        lo, hi = orc.ppm_edges(avg(-1), avg(0), avg(1))
+        assert hi == pytest.approx(a + b / 2 + c / 4, abs=1e-13)
+        assert lo == pytest.approx(a - b / 2 + c / 4, abs=1e-13)
+
+
+def test_ppm_limiter_cases(orc):
+    # local extremum -> flat
+    assert orc.ppm_edges(1.0, 2.0, 1.0) == (2.0, 2.0)
+    # CW84 overshoot cases on the monotonize step itself
+    lo, hi = orc.ppm_mono(0.0, 0.9, 1.0)       # diff = 0.4 > 1/6 -> lo = 3 q0 - 2 qr
+    assert (lo, hi) == (pytest.approx(0.7), 1.0)
+    lo, hi = orc.ppm_mono(0.0, 0.1, 1.0)       # diff = -0.4 < -1/6 -> hi = 3 q0 - 2 ql
+    assert (lo, hi) == (0.0, pytest.approx(0.3))
+    assert orc.ppm_mono(0.0, 0.5, 1.0) == (0.0, 1.0)
+    # spike form (spike_mhd.cu:164-172): q0 +- (qp-qm)/4 - (qm-2q0+qp)/6 when unlimited
+    qm, q0, qp = 1.0, 1.5, 2.3
+    d, cu = 0.25 * (qp - qm), (qm - 2 * q0 + qp) / 6
+    assert orc.ppm1_spike(qm, q0, qp, 1.0) == pytest.approx(q0 + d - cu, rel=1e-15)
+    assert orc.ppm1_spike(qm, q0, qp, -1.0) == pytest.approx(q0 - d - cu, rel=1e-15)
+    assert orc.spj(1.0, 2.5, 1.0) and not orc.spj(1.0, 1.5, 1.2)
+
+
+# ----------------------------------------------------------------------------------------------
+# a12 GLM pair: exact solution of the linear (Bn, psi) Riemann problem
+# ----------------------------------------------------------------------------------------------
+def test_glm_pair_solves_linear_riemann_problem(orc):
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        bl, br, pl, pr = rng.normal(size=4)
+        ch = rng.uniform(0.2, 3.0)
+        # d/dt (B, psi) + A d/dx (B, psi) = 0, A = [[0, 1], [ch^2, 0]]; state at x/t = 0
+        A = np.array([[0.0, 1.0], [ch * ch, 0.0]])
+        w, R = np.linalg.eig(A)
+        Li = np.linalg.inv(R)
+        cl, cr = Li @ np.array([bl, pl]), Li @ np.array([br, pr])
+        star = R @ np.where(w > 0, cl, cr)
+        bns, psis = orc.glm_pair(bl, br, pl, pr, ch)
+        assert bns == pytest.approx(star[0], rel=1e-12, abs=1e-12)
+        assert psis == pytest.approx(star[1], rel=1e-12, abs=1e-12)
+
+
+# ----------------------------------------------------------------------------------------------
+# a13-a15 Riemann solvers
+# ----------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["llf", "llf_bound", "hll", "hlld"])
+def test_riemann_consistency(orc, kind):
+    """F(U, U) = F(U) in every direction (the GLM components give ch^2 Bn and psi)."""
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        q = rand_prim(rng)
+        ch = 1.7
+        for d in range(3):
+            f = orc.riemann(q, q, d, kind, G, ch, SMALLR, PFL)
+            phys = orc.rot_flux_from(orc.phys_flux_x(orc.rot_to(q, d), G), d)
+            mask = np.ones(9, bool); mask[5 + d] = False; mask[8] = False
+            assert np.allclose(f[mask], phys[mask], rtol=1e-12, atol=1e-13)
+            assert f[5 + d] == pytest.approx(q[8], rel=1e-14, abs=1e-15)
+            assert f[8] == pytest.approx(ch * ch * q[5 + d], rel=1e-14)
+
+
+def test_hlld_resolves_isolated_contact_and_alfven_wave(orc):
+    bx = 0.8
+    # stationary contact: density jump, everything else equal -> zero mass flux, pressure flux only
+    L = np.array([1.0, 0, 0, 0, 1.0, bx, 0.3, -0.2, 0])
+    R = np.array([0.2, 0, 0, 0, 1.0, bx, 0.3, -0.2, 0])
+    f = orc.hlld_x(L, R, G, bx)
+    fe = textbook_flux_x(L)
+    assert abs(f[0]) < 1e-14
+    assert np.allclose(f[:8], fe[:8], atol=1e-13)
+    hll = orc.riemann(L, R, 0, "hll", G, 1.0, SMALLR, PFL)
+    assert abs(hll[0]) > 1e-3                   # HLL smears the contact
+    # moving contact (u > 0): exact upwind flux F(U_L)
+    L[1] = R[1] = 0.3
+    f = orc.hlld_x(L, R, G, bx)
+    assert np.allclose(f[:8], textbook_flux_x(L)[:8], atol=1e-13)
+    # rotational (Alfven) discontinuity: |B_t|, p, rho continuous.  Rankine-Hugoniot with u = 0:
+    # lambda [B_t] = -Bx [v_t], so [v_t] = -sgn(Bx) [B_t]/sqrt(rho) moves at +c_a (x/t = 0 sees the
+    # left state) and [v_t] = +sgn(Bx) [B_t]/sqrt(rho) at -c_a (x/t = 0 sees the right state).
+    rho, p = 1.0, 0.5
+    bt_l = np.array([0.6, 0.0]); bt_r = np.array([0.0, 0.6])
+    for sgn, side in ((-1.0, "L"), (1.0, "R")):
+        vt_r = sgn * np.sign(bx) * (bt_r - bt_l) / math.sqrt(rho)
+        L = np.array([rho, 0, 0.0, 0.0, p, bx, *bt_l, 0])
+        R = np.array([rho, 0, *vt_r, p, bx, *bt_r, 0])
+        f = orc.hlld_x(L, R, G, bx)
+        assert np.allclose(f[:8], textbook_flux_x(L if side == "L" else R)[:8], atol=1e-12)
+        assert not np.allclose(orc.riemann(L, R, 0, "hll", G, 1.0, SMALLR, PFL)[:8], f[:8], atol=1e-6)
+
+
+@pytest.mark.parametrize("branch", ["outer_left", "inner_left", "inner_right", "outer_right"])
+def test_hlld_sampling_branches(orc, branch):
+    """Each sampling fan is reached and its flux equals the HLL-consistent states of that fan."""
+    u = {"outer_left": 4.0, "inner_left": 0.2, "inner_right": -0.2, "outer_right": -4.0}[branch]
+    L = np.array([1.0, u, 0.1, 0, 1.0, 0.7, 0.3, 0, 0])
+    R = np.array([0.8, u, -0.1, 0, 0.8, 0.7, -0.2, 0.1, 0])
+    f = orc.hlld_x(L, R, G, 0.7)
+    if branch == "outer_left":
+        assert np.allclose(f[:8], textbook_flux_x(L)[:8], atol=1e-12)
+    elif branch == "outer_right":
+        assert np.allclose(f[:8], textbook_flux_x(R)[:8], atol=1e-12)
+    else:
+        assert np.all(np.isfinite(f))
+        # mass flux lies between the two one-sided physical mass fluxes' HLL bracket
+        hll = orc.riemann(L, R, 0, "hll", G, 1.0, SMALLR, PFL)
+        assert abs(f[0] - hll[0]) < 0.5
+
+
+def test_llf_switch_and_hll_ordering(orc):
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        L, R = rand_prim(rng), rand_prim(rng)
+        # the HLLD -> LLF switch (glmmhd_turb.jl:287-290)
+        f_sw = orc.riemann(L, R, 0, "hlld", G, 1.0, SMALLR, PFL, llf_dmin=10.0)
+        f_llf = orc.riemann(L, R, 0, "llf", G, 1.0, SMALLR, PFL)
+        assert np.array_equal(f_sw, f_llf)
+
+
+# ----------------------------------------------------------------------------------------------
+# a16 whole-step invariants
+# ----------------------------------------------------------------------------------------------
+def smooth(orc, nx, ny, nz, seed=0):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx; y = (np.arange(ny) + 0.5) / ny; z = (np.arange(nz) + 0.5) / nz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    t = 2 * np.pi
+    f = lambda: np.sin(t * X + rng.uniform(0, t)) * np.cos(t * Y + rng.uniform(0, t)) + 0.5 * np.sin(t * Z + rng.uniform(0, t))  # noqa: E731
+    q = np.stack([1 + 0.2 * f(), 0.3 * f(), 0.3 * f(), 0.3 * f(), 1 + 0.2 * f(), 0.5 + 0.3 * f(), 0.3 * f(), 0.3 * f(),
+                  0.01 * f()])
+    u = np.empty_like(q)
+    for idx in np.ndindex(nz, ny, nx):
+        u[(slice(None),) + idx] = orc.prim2cons(q[(slice(None),) + idx], G)
+    return u
+
+
+SCHEMES = [dict(predictor="full", recon="ppm", riemann="hlld"), dict(predictor="oned", recon="plm", riemann="hll"),
+           dict(predictor="full", recon="ppm_spike", riemann="llf"), dict(predictor="full", recon="plm", riemann="llf_bound")]
+
+
+@pytest.mark.parametrize("kw", SCHEMES)
+def test_step_conservation_and_fixed_point(orc, kw):
+    u = smooth(orc, 12, 10, 8)
+    sc = Scheme(gamma=G, dt=2e-3, dx=1 / 12, ch=1.0, glm_fac=0.97, **kw)
+    v = orc.step(u, sc)
+    for k in range(8):   # psi is damped, everything else conserved
+        assert abs(v[k].sum() - u[k].sum()) <= 1e-13 * np.abs(u[k]).sum() + 1e-15
+    w = orc.ic_uniform(6, 5, 4, 1.0, 0.6, (0.5, 0.2, 0.1), G)
+    assert np.array_equal(orc.step(w, sc), w)
+
+
+def test_step_shift_and_permutation_symmetry(orc):
+    u = smooth(orc, 8, 8, 8, seed=2)
+    sc = Scheme(gamma=G, dt=2e-3, dx=1 / 8, ch=1.0, predictor="full", recon="ppm", riemann="hlld")
+    v = orc.step(u, sc)
+    # integer periodic shifts commute exactly
+    us = np.roll(u, (2, -3, 1), axis=(1, 2, 3))
+    assert np.array_equal(orc.step(us, sc), np.roll(v, (2, -3, 1), axis=(1, 2, 3)))
+    # cyclic x->y->z permutation of the state gives the permuted result (to round-off)
+    def perm(a):
+        b = np.transpose(a, (0, 2, 3, 1)).copy()          # (z,y,x) -> new axes so x->y, y->z, z->x
+        b[[1, 2, 3]] = b[[3, 1, 2]]
+        b[[5, 6, 7]] = b[[7, 5, 6]]
+        return b
+    assert np.allclose(orc.step(perm(u), sc), perm(v), rtol=1e-12, atol=1e-13)
+
+
+def test_psi_damping_closed_form(orc):
+    u = orc.ic_uniform(4, 4, 4, 1.0, 1.0, (0.3, 0, 0), G)
+    u[8] = 0.37
+    fac = math.exp(-(0.5 / (0.18 * 1.0)) * 0.01)
+    v = orc.step(u, Scheme(gamma=G, dt=0.01, dx=0.25, ch=0.5, glm_fac=fac))
+    assert np.allclose(v[8], 0.37 * fac, rtol=1e-15)
+
+
+# ----------------------------------------------------------------------------------------------
+# a17 OU kick and interpolation
+# ----------------------------------------------------------------------------------------------
+def test_turb_interp_linear_exact_and_periodic(orc):
+    g = 8
+    i, j, k = np.meshgrid(np.arange(g), np.arange(g), np.arange(g), indexing="ij")
+    af = np.zeros((3, g, g, g))
+    af[0] = (0.5 + 0.25 * k + 0.1 * j + 0.05 * i).transpose(2, 1, 0)   # stored [z][y][x]: 0.5 + 0.25 z + 0.1 y + 0.05 x
+    af[1] = 1.0
+    af[2] = np.cos(2 * np.pi * i / g).transpose(2, 1, 0)
+    n = 16   # two cells per forcing cell: cell centres at grid coordinate (i + 0.5) / 2
+    for (ci, cj, ck) in [(3, 4, 5), (6, 2, 9), (8, 8, 8)]:
+        a = orc.turb_interp(af.reshape(-1), g, ci, cj, ck, 1.0 / n, 1.0, 1.0, 1.0, 1.0)
+        x, y, z = (ci + 0.5) * g / n, (cj + 0.5) * g / n, (ck + 0.5) * g / n
+        assert a[0] == pytest.approx(0.5 + 0.25 * z + 0.1 * y + 0.05 * x, rel=1e-14)
+        assert a[1] == pytest.approx(1.0, rel=1e-15)
+    # periodic wrap: the last cell interpolates between index g-1 and 0
+    a = orc.turb_interp(af.reshape(-1), g, n - 1, 0, 0, 1.0 / n, 1.0, 1.0, 1.0, 1.0)
+    x = (n - 0.5) * g / n
+    w = x - math.floor(x)
+    assert a[2] == pytest.approx((1 - w) * math.cos(2 * np.pi * 7 / g) + w * 1.0, rel=1e-13)
+
+
+def test_kick_energy_bookkeeping(orc):
+    """Without floors the kick adds exactly the kinetic-energy change (glmmhd_turb.jl:464-474)."""
+    g = 4
+    af = np.full(3 * g ** 3, 0.0)
+    af[:g ** 3] = 2.0; af[g ** 3:2 * g ** 3] = -1.0; af[2 * g ** 3:] = 0.5
+    u = orc.ic_uniform(4, 4, 4, 1.3, 1.0, (0.2, 0, 0), G)
+    u[1] = 0.4; u[4] += 0.5 * 0.16 / 1.3
+    sc = Scheme(gamma=G, dt=1e-3, dx=0.25, ch=1.0, do_turb=True, turb_gs=g, ramp=0.5, turb_min_rho=1e-5)
+    v = orc.step(u, sc, afield=af)
+    m0 = u[1:4].reshape(3, -1)[:, 0]
+    m1 = v[1:4].reshape(3, -1)[:, 0]
+    assert np.allclose(m1 - m0, 1.3 * np.array([2.0, -1.0, 0.5]) * 0.5 * 1e-3, rtol=1e-12)
+    de = v[4].reshape(-1)[0] - u[4].reshape(-1)[0]
+    assert de == pytest.approx(0.5 * (m1 @ m1 - m0 @ m0) / 1.3, rel=1e-10)
+
+
+# ----------------------------------------------------------------------------------------------
+# a18 OU forcing (N5)
+# ----------------------------------------------------------------------------------------------
+def test_philox_known_answers(orc):
+    """Random123 known-answer vectors for Philox4x32-10."""
+    assert list(orc.philox([0, 0, 0, 0], [0, 0])) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(orc.philox([0xffffffff] * 4, [0xffffffff] * 2)) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert list(orc.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0])) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_gaussians_moments(orc):
+    z = np.concatenate([orc.ou_gauss6(1, s, m) for s in range(40) for m in range(500)])
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    assert abs(((z ** 4).mean()) - 3.0) < 0.06
+
+
+def test_ou_solenoidal_shells_and_real_field(orc):
+    g = 16
+    f = OUForcing(orc, g=g, kmin=1.0, kmax=3.0, T=0.5, ndt=100, comp_frac=0.0, amp=1.0, seed=9)
+    for _ in range(3):
+        f.advance()
+    F = (f.fre + 1j * f.fim).reshape(3, g, g, g)      # (component, kz, ky, kx)
+    k = np.fft.fftfreq(g, 1.0 / g)
+    KZ, KY, KX = np.meshgrid(k, k, k, indexing="ij")
+    kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    on = (kk >= 1) & (kk <= 3)
+    assert np.all(F[:, ~on] == 0)                     # shell support
+    kdot = KX * F[0] + KY * F[1] + KZ * F[2]
+    assert np.max(np.abs(kdot)) <= 1e-13 * np.max(np.abs(F))   # solenoidal
+    # acceleration field = real part of the unnormalised inverse DFT (brute-force check by numpy)
+    a = np.real(np.fft.ifftn(F, axes=(1, 2, 3))) * g ** 3 * (1.0 / g ** 3)
+    assert np.allclose(f.afield.reshape(3, g, g, g), a, rtol=1e-12, atol=1e-14)
+
+
+def test_ou_stationary_variance_and_correlation(orc):
+    """alpha = 1 - dt/T, beta = sqrt(2 dt/T): stationary <|F_k|^2> = beta^2 / (1 - alpha^2) * P(k) * 2
+    per component (before projection), lag-1 autocorrelation alpha."""
+    g = 8
+    f = OUForcing(orc, g=g, kmin=1.0, kmax=1.0, T=0.05, ndt=5, comp_frac=1.0, amp=1.0, seed=3)
+    dt = f.T / f.ndt
+    alpha = 1 - dt / f.T
+    beta2 = 2 * dt / f.T
+    samples, prev, cross = [], None, []
+    for it in range(600):
+        f.advance()
+        if it >= 100:
+            cur = (f.fre + 1j * f.fim).reshape(3, -1)
+            samples.append(np.abs(cur) ** 2)
+            if prev is not None:
+                cross.append(np.real(np.conj(prev) * cur))
+            prev = cur
+    S = np.mean(samples, axis=0)
+    on = S.sum(axis=0) > 0
+    # comp_frac = 1 keeps the k-parallel part: total over components = beta^2/(1-alpha^2) * P * 2
+    tot = S.sum(axis=0)[on]
+    expect = beta2 / (1 - alpha ** 2) * 1.0 * 2.0
+    assert abs(tot.mean() / expect - 1) < 0.1
+    rho1 = np.mean(cross, axis=0).sum(axis=0)[on].mean() / tot.mean()
+    assert abs(rho1 - alpha) < 0.05
+
+
+def test_twiddle_table():
+    cs, sn = twiddles(8)
+    assert np.allclose(cs + 1j * sn, np.exp(2j * np.pi * np.arange(8) / 8))
+
+
+# ----------------------------------------------------------------------------------------------
+# a19-a21 CFL, diagnostics, ICs (brute force / closed form)
+# ----------------------------------------------------------------------------------------------
+def test_ctot_and_diagnostics_brute_force(orc):
+    u = smooth(orc, 6, 5, 4, seed=7)
+    best, sums = 0.0, dict(mass=[], ekin=[], emag=[], hc=[])
+    for idx in np.ndindex(4, 5, 6):
+        c = u[(slice(None),) + idx]
+        q = orc.cons2prim(c, G, SMALLR, PFL)
+        b2 = q[5] ** 2 + q[6] ** 2 + q[7] ** 2
+        tot = 0.0
+        for d in range(3):
+            a2 = G * q[4] / q[0]
+            cf = math.sqrt(0.5 * (a2 + b2 / q[0]) + math.sqrt(max((0.5 * (a2 + b2 / q[0])) ** 2 - a2 * q[5 + d] ** 2 / q[0], 0)))
+            tot += abs(q[1 + d]) + cf
+        best = max(best, tot)
+        sums["mass"].append(c[0]); sums["ekin"].append(0.5 * (c[1:4] @ c[1:4]) / c[0])
+        sums["emag"].append(0.5 * (c[5:8] @ c[5:8])); sums["hc"].append((c[1:4] @ c[5:8]) / c[0])
+    assert orc.ctot_max(u, G, SMALLR, PFL) == pytest.approx(best, rel=1e-14)
+    d = orc.diagnostics(u, 0.25, SMALLR)
+    assert d["mass"] == pytest.approx(math.fsum(sums["mass"]), rel=1e-15)
+    assert d["ekin"] == pytest.approx(math.fsum(sums["ekin"]), rel=1e-14)
+    assert d["emag"] == pytest.approx(math.fsum(sums["emag"]), rel=1e-14)
+    assert d["cross_helicity"] == pytest.approx(math.fsum(sums["hc"]), rel=1e-13)
+    bx, by, bz = u[5], u[6], u[7]
+    div = ((np.roll(bx, -1, 2) - np.roll(bx, 1, 2)) + (np.roll(by, -1, 1) - np.roll(by, 1, 1)) +
+           (np.roll(bz, -1, 0) - np.roll(bz, 1, 0))) / (2 * 0.25)
+    assert d["divbmax"] == pytest.approx(np.abs(div).max(), rel=1e-14)
+
+
+def test_orszag_tang_ic_closed_form(orc):
+    n = 16
+    u = orc.ic_ot(n, n, 2, G)
+    x = (np.arange(n) + 0.5) / n
+    Y, X = np.meshgrid(x, x, indexing="ij")
+    rho = 25 / (36 * np.pi)
+    assert np.allclose(u[0], rho)
+    assert np.allclose(u[1, 0], -rho * np.sin(2 * np.pi * Y))
+    assert np.allclose(u[6, 1], np.sin(4 * np.pi * X) / np.sqrt(4 * np.pi))
+    e = 5 / (12 * np.pi) / (G - 1) + 0.5 * rho * (np.sin(2 * np.pi * Y) ** 2 + np.sin(2 * np.pi * X) ** 2) + \
+        0.5 * (np.sin(2 * np.pi * Y) ** 2 + np.sin(4 * np.pi * X) ** 2) / (4 * np.pi)
+    assert np.allclose(u[4, 0], e, rtol=1e-14)
+
+
+# ----------------------------------------------------------------------------------------------
+# a25 constrained transport
+# ----------------------------------------------------------------------------------------------
+def ct_state(orc, nx, ny, nz, seed=1):
+    u = smooth(orc, nx, ny, nz, seed)[:8].copy()
+    x = np.arange(nx) / nx; y = np.arange(ny) / ny; z = np.arange(nz) / nz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    t = 2 * np.pi
+    ax, ay, az = 0.05 * np.sin(t * (Y + Z)), 0.05 * np.cos(t * (X + 2 * Z)), 0.05 * np.sin(t * (X - Y))
+    h = 1.0 / nx
+    r = lambda a, k: np.roll(a, -1, axis=k)  # noqa: E731
+    u[5] = 0.5 + ((r(az, 1) - az) - (r(ay, 0) - ay)) / h
+    u[6] = ((r(ax, 0) - ax) - (r(az, 2) - az)) / h
+    u[7] = ((r(ay, 2) - ay) - (r(ax, 1) - ax)) / h
+    return u
+
+
+@pytest.mark.parametrize("riem,recon,fs", [("hll", "plm", 0), ("hll", "plm", 1), ("hlld", "plm", 0), ("llf", "ppm", 0)])
+def test_ct_preserves_divb_and_conserves(orc, riem, recon, fs):
+    from oracle import CTScheme
+    n = 10
+    q = ct_state(orc, n, n, n)
+    dx = 1.0 / n
+    assert orc.ct_divb_max(q, dx) < 1e-12
+    sc = CTScheme(gamma=G, dt=2e-3, dx=dx, face_slope=fs, riemann=riem, recon="ppm_spike" if recon == "ppm" else "plm")
+    for _ in range(3):
+        q2 = orc.ct_step(q, sc)
+        for k in range(5):
+            assert abs(q2[k].sum() - q[k].sum()) <= 1e-12 * np.abs(q[k]).sum()
+        q = q2
+    assert orc.ct_divb_max(q, dx) < 1e-11
+    assert np.all(np.isfinite(q))
+
+
+def test_ct_one_dimensional_reduction(orc):
+    """A state varying only along x: the face-B update leaves Bx (the normal field) untouched and
+    the y and z fluxes cancel, so only the x-sweep acts."""
+    from oracle import CTScheme
+    n = 12
+    q = np.zeros((8, 2, 2, n))
+    x = (np.arange(n) + 0.5) / n
+    rho = 1 + 0.2 * np.sin(2 * np.pi * x)
+    q[0] = rho; q[1] = 0.1 * rho; q[4] = 1.0 / (G - 1) + 0.5 * 0.01 * rho + 0.5 * (0.49 + 0.09)
+    q[5] = 0.7; q[6] = 0.3
+    q2 = orc.ct_step(q, CTScheme(gamma=G, dt=1e-3, dx=1.0 / n))
+    assert np.array_equal(q2[5], q[5])
+    assert np.allclose(q2[:, 0], q2[:, 1]) and np.allclose(q2[:, :, 0], q2[:, :, 1])

@@ -34,7 +34,23 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run 3 plain device copies of 256 MiB (HBM counter calibration)")
+    ap.add_argument("--no-gradient", action="store_true")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Three device-to-device copies of a 256 MiB buffer: a kernel of known traffic (256 MiB read,
+    256 MiB written per launch) to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 256 * 2 ** 20 // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8, "launches": 3}
 
 
 def make_params(N):
@@ -190,7 +206,10 @@ def main():
         ach32 = BYTES_PER_CELL["f32"] * N ** 3 / (r32["kernel_ms"] * 1e-3)
         rec["fp32"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "kernel_ms": r32["kernel_ms"],
                        "hbm_frac": ach32 / HBM_PEAK, "speedup_vs_fp64": r32["value"] / r64["value"]}
-    rec["gradient_ic_fp64"] = run_gradient_ic(N, max(5, args.steps // 3), "f64")
+    if not args.no_gradient:
+        rec["gradient_ic_fp64"] = run_gradient_ic(N, max(5, args.steps // 3), "f64")
+    if args.copy_calib:
+        rec["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(rec))

@@ -7,11 +7,15 @@
 // and LLF/HLL/HLLD Riemann fluxes with Dedner GLM cleaning (glmmhd_turb.jl:175-354).
 //
 // How it is laid out on the GPU:
-//   * A workgroup of TX x TY = 16 x 16 threads owns a 14 x 14 column tile and a z-chunk.  Each
-//     thread owns one (x,y) column of the tile grown by one halo cell and marches z.  Its
-//     column's primitives for planes q-1, q, q+1 live in registers.
+//   * A workgroup of TX x TY = 18 x 14 threads owns a 16 x 12 column tile and a z-chunk (16
+//     doubles = one 128-byte line per row, so row stores are whole lines).  Each thread owns one
+//     (x,y) column of the tile grown by one halo cell and marches z.  Its column's primitives for
+//     planes q-1, q, q+1 live in registers.
 //   * LDS holds the primitives of the current plane q over the tile grown by two cells
-//     (18 x 18): the in-plane neighbours for slopes, PPM edges and the transverse predictor.
+//     (20 x 16): the in-plane neighbours for slopes, PPM edges and the transverse predictor.
+//   * Workgroup ids are renumbered so that each XCD (the hardware deals ids round-robin over the
+//     8 XCDs) receives a contiguous run of tiles: neighbouring tiles then share their halo
+//     reads in the same L2.
 //   * Every face is solved once.  The z face q-1/2 is solved by the column owner from the z+
 //     state it carried from plane q-1; x and y faces are solved after the +x/+y face states are
 //     exchanged through LDS, and the fluxes are exchanged back the same way.
@@ -24,7 +28,14 @@
 
 namespace glm {
 
-constexpr int TX = 16, TY = 16, NT = TX * TY;
+#ifndef GLM_TX
+#define GLM_TX 18
+#endif
+#ifndef GLM_TY
+#define GLM_TY 14
+#endif
+constexpr int TX = GLM_TX, TY = GLM_TY, NT = TX * TY;
+constexpr int NXCD = 8;                     // XCDs of an MI355X
 constexpr int OX = TX - 2, OY = TY - 2;     // owned columns per tile
 constexpr int PX = TX + 2, PY = TY + 2;     // LDS primitive plane (2-cell halo)
 constexpr int NPL = PX * PY;
@@ -65,6 +76,15 @@ struct ColdArgs {
     T dtdx, glm_fac, smallc, turb_min_rho, ramp, dt, dx, lx, ly, lz;
     int update_floor, do_turb, tgs, pad_;
 };
+
+// Logical tile of hardware workgroup id w when ids are dealt round-robin over NXCD XCDs: XCD x
+// gets the contiguous logical range [start_x, start_x + count_x).  A bijection on [0, total).
+GLM_DEV int xcd_block(int w, int total)
+{
+    const int q = total / NXCD, r = total % NXCD;
+    const int xcd = w % NXCD, slot = w / NXCD;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+}
 
 GLM_DEV int wrapi(int i, int n)
 {
@@ -178,11 +198,64 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
     guard(hi, m0, a.face_guard, s);
 }
 
+// Bilinear (x,y) part of the trilinear forcing interpolation (glmmhd_turb.jl:483-505) at forcing
+// plane kw for component c, in the reference's association order.
+template <typename T>
+GLM_DEV T force_bilinear(const T* af, int g, int i0w, int i1w, int j0w, int j1w, int kw, T wx, T wy)
+{
+    auto at = [&](int i, int j) { return af[(size_t)i + (size_t)g * ((size_t)j + (size_t)g * kw)]; };
+    return (at(i0w, j0w) * (T(1) - wx) + at(i1w, j0w) * wx) * (T(1) - wy) +
+           (at(i0w, j1w) * (T(1) - wx) + at(i1w, j1w) * wx) * wy;
+}
+
+// Per-column cache of the forcing interpolation: the (x,y) bilinear values on the two forcing
+// planes that bracket the current cell.  A z-march crosses a forcing plane only every n/g cells,
+// so the 24 corner loads of the reference's trilinear interpolation happen once per crossing.
+// The bilinear values live in LDS (one slot per thread): they are touched once per cell only when
+// the kick is on, and keeping them out of registers leaves the unforced march unchanged.
+template <typename T>
+struct ForceCache {
+    int k0;        // lower forcing plane of b0, or a sentinel when empty
+    T* b;          // this thread's 6 LDS slots: b0[3], b1[3], stride NT
+    GLM_DEV T& b0(int c) { return b[(size_t)c * NT]; }
+    GLM_DEV T& b1(int c) { return b[(size_t)(3 + c) * NT]; }
+};
+
+template <typename T>
+GLM_DEV void force_accel(ForceCache<T>& fc, const ColdArgs<T>& a, int gx, int gy, int gz, T acc3[3])
+{
+    const int g = a.tgs;
+    const T gg = T(g);
+    T cx = ((T(gx) + T(0.5)) * a.dx) / a.lx * gg;
+    T cy = ((T(gy) + T(0.5)) * a.dx) / a.ly * gg;
+    T cz = ((T(gz) + T(0.5)) * a.dx) / a.lz * gg;
+    T fz = floor(cz);
+    int k0 = (int)fz;
+    T wz = cz - fz;
+    if (k0 != fc.k0) {
+        T fx = floor(cx), fy = floor(cy);
+        int i0 = (int)fx, j0 = (int)fy;
+        T wx = cx - fx, wy = cy - fy;
+        int i0w = wrapi(i0, g), i1w = wrapi(i0 + 1, g), j0w = wrapi(j0, g), j1w = wrapi(j0 + 1, g);
+        const size_t g3 = (size_t)g * g * g;
+        const bool shift = k0 == fc.k0 + 1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T* af = a.afield + c * g3;
+            fc.b0(c) = shift ? fc.b1(c) : force_bilinear(af, g, i0w, i1w, j0w, j1w, wrapi(k0, g), wx, wy);
+            fc.b1(c) = force_bilinear(af, g, i0w, i1w, j0w, j1w, wrapi(k0 + 1, g), wx, wy);
+        }
+        fc.k0 = k0;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc3[c] = a.ramp * (fc.b0(c) * (T(1) - wz) + fc.b1(c) * wz);
+}
+
 // Update epilogue of one cell (glmmhd_turb.jl:440-476), CFL speed of the result
 // (glmmhd_turb.jl:1963-1971).  acc = (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) + (Fz_lo - Fz_hi).
 template <typename T>
-GLM_DEV T update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl, int gx, int gy,
-                      int gz)
+GLM_DEV T update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl, ForceCache<T>& fc,
+                      int gx, int gy, int gz)
 {
 #pragma unroll
     for (int v = 0; v < 9; ++v) r[v] = r[v] + a.dtdx * acc[v];
@@ -193,30 +266,8 @@ GLM_DEV T update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T
     }
     r[8] = r[8] * a.glm_fac;
     if (a.do_turb && r[0] >= a.turb_min_rho) {
-        // glmmhd_turb.jl:483-505 turb_interp (cell centre (i+1/2) dx), periodic trilinear
-        const int g = a.tgs;
-        const T gg = T(g);
-        T cx = ((T(gx) + T(0.5)) * a.dx) / a.lx * gg;
-        T cy = ((T(gy) + T(0.5)) * a.dx) / a.ly * gg;
-        T cz = ((T(gz) + T(0.5)) * a.dx) / a.lz * gg;
-        T fx = floor(cx), fy = floor(cy), fz = floor(cz);
-        int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-        T wx = cx - fx, wy = cy - fy, wz = cz - fz;
-        int i0w = wrapi(i0, g), i1w = wrapi(i0 + 1, g);
-        int j0w = wrapi(j0, g), j1w = wrapi(j0 + 1, g);
-        int k0w = wrapi(k0, g), k1w = wrapi(k0 + 1, g);
-        const size_t g3 = (size_t)g * g * g;
         T acc3[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T* af = a.afield + c * g3;
-            auto at = [&](int i, int j, int k) { return af[(size_t)i + (size_t)g * ((size_t)j + (size_t)g * k)]; };
-            T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
-                    (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
-                   ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
-                    (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
-            acc3[c] = a.ramp * t;
-        }
+        force_accel(fc, a, gx, gy, gz, acc3);
         T rho = r[0], rhom = tmax(rho, s.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc;
         T e = r[4];
         e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
@@ -243,6 +294,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     __shared__ T XB[9][NT];
     __shared__ T YB[9][NT];
     __shared__ ColdArgs<T> ca;
+    __shared__ T FCB[6][NT];
 
     const Phys<T>& s = a.ph;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -254,7 +306,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
         ca.update_floor = a.update_floor; ca.do_turb = a.do_turb; ca.tgs = a.tgs;
     }
     __syncthreads();
-    int b = blockIdx.x;
+    int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int tile_x = b % a.tiles_x; b /= a.tiles_x;
     const int tile_y = b % a.tiles_y; b /= a.tiles_y;
     const int zs = b * a.zchunk;
@@ -298,6 +350,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
 #pragma unroll
     for (int v = 0; v < 9; ++v) { zprev[v] = T(0); pend[v] = T(0); }
     T cmax = T(0);
+    ForceCache<T> fc;
+    fc.k0 = -0x7fffffff;
+    fc.b = &FCB[0][tid];
 
     for (int q = zs - 1; q <= ze; ++q) {
         const bool last = q == ze;
@@ -375,7 +430,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
 #pragma unroll
             for (int v = 0; v < 9; ++v) acc[v] = pend[v] - fz[v];
             const int zq = wrapi(q - 1, nz);
-            T ct = update_cell(u0, acc, ca, s, a.want_cfl != 0, gx, gy, zq);
+            T ct = update_cell(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
             if (!(ct <= cmax)) cmax = ct;
             const size_t off = col + plane * zq;
 #pragma unroll
@@ -447,14 +502,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     }
 
     if (a.want_cfl) {
-        // wavefront max (NaN propagates as the largest bit pattern), one atomic per wavefront
-        unsigned long long bits = (unsigned long long)__double_as_longlong((double)cmax);
+        // workgroup max through LDS (the last wavefront of a TX*TY group may be partial), then one
+        // atomic per workgroup on the bits of a non-negative double (NaN compares largest)
+        __syncthreads();
+        unsigned long long* red = reinterpret_cast<unsigned long long*>(&XB[0][0]);
+        red[tid] = (unsigned long long)__double_as_longlong((double)cmax);
+        __syncthreads();
+        if (tid < 64) {
+            unsigned long long bits = 0ull;
+            for (int t = tid; t < NT; t += 64) bits = red[t] > bits ? red[t] : bits;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            unsigned long long o = __shfl_xor(bits, off, 64);
-            bits = o > bits ? o : bits;
+            for (int off = 32; off >= 1; off >>= 1) {
+                unsigned long long o = __shfl_xor(bits, off, 64);
+                bits = o > bits ? o : bits;
+            }
+            if (tid == 0 && bits != 0ull) atomicMax(a.cmax, bits);
         }
-        if ((tid & 63) == 0 && bits != 0ull) atomicMax(a.cmax, bits);
     }
 }
 

@@ -41,16 +41,20 @@ def parse():
 
 
 def copy_calibration():
-    """Three device-to-device copies of a 256 MiB buffer: a kernel of known traffic (256 MiB read,
-    256 MiB written per launch) to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    """Three elementwise copies (dst = src * 1, one compute kernel each) of a 1 GiB buffer, with a
+    1 GiB fill in between to push src out of the caches: a kernel of known traffic (1 GiB read,
+    1 GiB written per launch) to scale the FETCH_SIZE / WRITE_SIZE counters."""
     import torch
-    n = 256 * 2 ** 20 // 8
+    n = 2 ** 30 // 8
     src = torch.rand(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
+    flush = torch.empty_like(src)
     for _ in range(3):
-        dst.copy_(src)
+        flush.fill_(1.0)
+        torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    return {"bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8, "launches": 3}
+    return {"kernel": "elementwise mul by 1 (MulFunctor)", "bytes_read_per_launch": n * 8,
+            "bytes_written_per_launch": n * 8, "launches": 3}
 
 
 def make_params(N):

@@ -28,18 +28,26 @@
 
 namespace glm {
 
-#ifndef GLM_TX
-#define GLM_TX 18
+// Owned tile OX x OY (16 doubles = one 128-byte line per row).  Lanes 0..NO-1 own the tile row by
+// row (4 rows per wavefront, so each row is stored by one instruction as a whole line); lanes
+// NO..NT-1 are the tile's one-cell halo without its corners and form the last wavefront, which
+// therefore skips the z solve and the update as a whole.
+#ifndef GLM_OX
+#define GLM_OX 16
 #endif
-#ifndef GLM_TY
-#define GLM_TY 14
+#ifndef GLM_OY
+#define GLM_OY 12
 #endif
-constexpr int TX = GLM_TX, TY = GLM_TY, NT = TX * TY;
-constexpr int NXCD = 8;                     // XCDs of an MI355X
-constexpr int OX = TX - 2, OY = TY - 2;     // owned columns per tile
-constexpr int PX = TX + 2, PY = TY + 2;     // LDS primitive plane (2-cell halo)
+constexpr int OX = GLM_OX, OY = GLM_OY;
+constexpr int NO = OX * OY;                 // owned lanes
+constexpr int NH = 2 * OX + 2 * OY;         // halo lanes (one-cell ring without corners)
+constexpr int NT = NO + NH;                 // threads per workgroup
+constexpr int PX = OX + 4, PY = OY + 4;     // LDS primitive plane (two-cell ring)
 constexpr int NPL = PX * PY;
-constexpr int NRING = NPL - NT;             // outer halo ring loaded by helper threads
+constexpr int NHELP = 4 + NH;               // plane cells no lane owns: the 4 one-cell corners and the
+                                            // two-cell ring without corners (in-plane neighbours of halo lanes)
+constexpr int NXCD = 8;                     // XCDs of an MI355X
+static_assert(NT <= 256 && NT >= NHELP && NO % 64 == 0, "tile shape");
 
 // Minimum resident wavefronts per SIMD requested from the register allocator.
 #ifndef GLM_WAVES_F64
@@ -92,13 +100,26 @@ GLM_DEV int wrapi(int i, int n)
     return m < 0 ? m + n : m;
 }
 
-// Position of helper thread t on the outer ring of the PX x PY plane.
-GLM_DEV void ring_xy(int t, int& hx, int& hy)
+// Cell (tx, ty) of lane t in the tile grown by one cell: tx in 0..OX+1, ty in 0..OY+1.
+GLM_DEV void lane_cell(int t, int& tx, int& ty)
 {
-    if (t < PX) { hx = t; hy = 0; }
-    else if (t < 2 * PX) { hx = t - PX; hy = PY - 1; }
-    else if (t < 2 * PX + (PY - 2)) { hx = 0; hy = t - 2 * PX + 1; }
-    else { hx = PX - 1; hy = t - 2 * PX - (PY - 2) + 1; }
+    if (t < NO) { tx = 1 + t % OX; ty = 1 + t / OX; return; }
+    const int h = t - NO;
+    if (h < OX) { tx = 1 + h; ty = 0; }
+    else if (h < 2 * OX) { tx = 1 + h - OX; ty = OY + 1; }
+    else if (h < 2 * OX + OY) { tx = 0; ty = 1 + h - 2 * OX; }
+    else { tx = OX + 1; ty = 1 + h - 2 * OX - OY; }
+}
+
+// LDS plane position (px, py) of helper cell h (0 <= h < NHELP).
+GLM_DEV void helper_cell(int h, int& px, int& py)
+{
+    if (h < 4) { px = (h & 1) ? PX - 2 : 1; py = (h & 2) ? PY - 2 : 1; return; }
+    const int r = h - 4;
+    if (r < OX) { px = 2 + r; py = 0; }
+    else if (r < 2 * OX) { px = 2 + r - OX; py = PY - 1; }
+    else if (r < 2 * OX + OY) { px = 0; py = 2 + r - 2 * OX; }
+    else { px = PX - 1; py = 2 + r - 2 * OX - OY; }
 }
 
 template <typename T>
@@ -291,13 +312,15 @@ template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) march_kernel(MarchArgs<T> a)
 {
     __shared__ T P[9][NPL];
-    __shared__ T XB[9][NT];
-    __shared__ T YB[9][NT];
+    __shared__ T XB[9][NPL];   // +x face states, then x fluxes, at the LDS plane position of the cell
+    __shared__ T YB[9][NPL];
     __shared__ ColdArgs<T> ca;
     __shared__ T FCB[6][NT];
 
     const Phys<T>& s = a.ph;
-    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int tid = threadIdx.x;
+    int tx, ty;
+    lane_cell(tid, tx, ty);
     if (tid == 0) {
         for (int v = 0; v < 9; ++v) { ca.q[v] = a.q[v]; ca.o[v] = a.o[v]; }
         ca.afield = a.afield;
@@ -315,12 +338,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     const size_t plane = (size_t)nx * ny;
     const int gxu = tile_x * OX - 1 + tx, gyu = tile_y * OY - 1 + ty;
     const int gx = wrapi(gxu, nx), gy = wrapi(gyu, ny);
-    const bool owned = tx >= 1 && tx <= OX && ty >= 1 && ty <= OY && gxu < nx && gyu < ny;
+    const bool owned = tid < NO && gxu < nx && gyu < ny;
+    const bool need_x = ty >= 1 && ty <= OY && tx >= 1;      // this lane solves the x face at tx - 1/2
+    const bool need_y = tx >= 1 && tx <= OX && ty >= 1;      // this lane solves the y face at ty - 1/2
     const size_t col = (size_t)gx + (size_t)nx * gy;
     const int lo_own = (ty + 1) * PX + (tx + 1);
-    const bool helper = tid < NRING;
+    const bool helper = tid < NHELP;
     int hx = 0, hy = 0;
-    ring_xy(helper ? tid : 0, hx, hy);
+    helper_cell(helper ? tid : 0, hx, hy);
     const size_t hcol = (size_t)wrapi(tile_x * OX - 2 + hx, nx) + (size_t)nx * wrapi(tile_y * OY - 2 + hy, ny);
     const int lo_ring = hy * PX + hx;
 
@@ -418,10 +443,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
 
         // ---- z face q-1/2 and the update of plane q-1 ---------------------------------------
         T fz[9];
-        {
+        if (owned) {
             T zlo[9], zhi[9];
             faces_dir<T, PRED, RECON, SLOPE, 2>(pm, p0, pp, mh, up, a, zlo, zhi);
-            if (q >= zs && owned) riemann<T, RIEM, 2>(zprev, zlo, fz, s);
+            if (q >= zs) riemann<T, RIEM, 2>(zprev, zlo, fz, s);
 #pragma unroll
             for (int v = 0; v < 9; ++v) zprev[v] = zhi[v];
         }
@@ -452,36 +477,37 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
                 for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - 1]; r[v] = P[v][lo_own + 1]; }
                 faces_dir<T, PRED, RECON, SLOPE, 0>(l, p0, r, mh, up, a, xlo, hi);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) XB[v][tid] = hi[v];
+                for (int v = 0; v < 9; ++v) XB[v][lo_own] = hi[v];
 #pragma unroll
                 for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - PX]; r[v] = P[v][lo_own + PX]; }
                 faces_dir<T, PRED, RECON, SLOPE, 1>(l, p0, r, mh, up, a, ylo, hi);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) YB[v][tid] = hi[v];
+                for (int v = 0; v < 9; ++v) YB[v][lo_own] = hi[v];
             }
             __syncthreads();
             T fx[9], fy[9];
 #pragma unroll
             for (int v = 0; v < 9; ++v) { fx[v] = T(0); fy[v] = T(0); }
-            if (tx >= 1) {
+            if (need_x) {
                 T L[9];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) L[v] = XB[v][tid - 1];
+                for (int v = 0; v < 9; ++v) L[v] = XB[v][lo_own - 1];
                 riemann<T, RIEM, 0>(L, xlo, fx, s);
             }
-            if (ty >= 1) {
+            if (need_y) {
                 T L[9];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) L[v] = YB[v][tid - TX];
+                for (int v = 0; v < 9; ++v) L[v] = YB[v][lo_own - PX];
                 riemann<T, RIEM, 1>(L, ylo, fy, s);
             }
             __syncthreads();
 #pragma unroll
-            for (int v = 0; v < 9; ++v) { XB[v][tid] = fx[v]; YB[v][tid] = fy[v]; }
+            for (int v = 0; v < 9; ++v) { XB[v][lo_own] = fx[v]; YB[v][lo_own] = fy[v]; }
             __syncthreads();
             if (owned) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) pend[v] = ((fx[v] - XB[v][tid + 1]) + (fy[v] - YB[v][tid + TX])) + fz[v];
+                for (int v = 0; v < 9; ++v)
+                    pend[v] = ((fx[v] - XB[v][lo_own + 1]) + (fy[v] - YB[v][lo_own + PX])) + fz[v];
             }
         }
 

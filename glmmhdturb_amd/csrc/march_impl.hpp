@@ -316,6 +316,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     __shared__ T YB[9][NPL];
     __shared__ ColdArgs<T> ca;
     __shared__ T FCB[6][NT];
+    __shared__ T UR[3][9][NO];   // owned lanes' conserved state of the last three planes loaded
+                                 // (plane k in slot k mod 3): the update base, read once from HBM
 
     const Phys<T>& s = a.ph;
     const int tid = threadIdx.x;
@@ -358,6 +360,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
         load9(ca, col + plane * wrapi(zs - 2, nz), c); cons2prim(c, pm, s);
         load9(ca, col + plane * wrapi(zs - 1, nz), c); cons2prim(c, p0, s);
         load9(ca, col + plane * wrapi(zs, nz), c); cons2prim(c, pp, s);
+        if (tid < NO) {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) UR[zs % 3][v][tid] = c[v];
+        }
 #pragma unroll
         for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
         if (helper) {
@@ -385,7 +391,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
         const bool do_upd = owned && q >= zs + 1;
 
         T u0[9];
-        if (do_upd) load9(ca, col + plane * wrapi(q - 1, nz), u0);
 
         // ---- reconstruction of this column's cell in plane q --------------------------------
         bool up = true;   // PPM strong-pressure-jump fallback flag (glmmhd_turb.jl:959)
@@ -451,6 +456,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
             for (int v = 0; v < 9; ++v) zprev[v] = zhi[v];
         }
         if (do_upd) {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) u0[v] = UR[(q - 1) % 3][v][tid];
             T acc[9];
 #pragma unroll
             for (int v = 0; v < 9; ++v) acc[v] = pend[v] - fz[v];
@@ -516,6 +523,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
 #pragma unroll
         for (int v = 0; v < 9; ++v) { pm[v] = p0[v]; p0[v] = pp[v]; }
         cons2prim(cn, pp, s);
+        if (tid < NO) {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) UR[(q + 2) % 3][v][tid] = cn[v];
+        }
 #pragma unroll
         for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
         if (helper) {

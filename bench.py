@@ -28,7 +28,7 @@ AMP = 1.0e5                # OU amplitude (frozen)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-fp32", action="store_true")
@@ -109,7 +109,7 @@ def run_gpu(N, steps, warmup, precision):
     d = s.diagnostics(u1)
     n = N ** 3
     out = dict(value=n * steps / wall, ms_per_step=wall / steps * 1e3, kernel_ms=kern_ms, t=t, mach=d["mach"],
-               finite=d["finite"], divbmax=d["divbmax"])
+               finite=d["finite"], divbmax=d["divbmax"], diag=d)
     s.close()
     return out
 
@@ -135,6 +135,54 @@ def run_gradient_ic(N, steps, precision):
     wall = time.perf_counter() - t0
     s.close()
     return N ** 3 * steps / wall
+
+
+def diag_errors(d64, d32):
+    """Relative error of each FP32 diagnostic against FP64.  The cross helicity sum v.B can be near
+    zero, so its error is also given against the Cauchy-Schwarz bound sqrt(sum m^2/rho * sum B^2)."""
+    rel = {k: abs(d32[k] - d64[k]) / max(abs(d64[k]), 1e-300) for k in ("ekin", "emag", "cross_helicity", "mass", "etot")}
+    rel["cross_helicity_vs_bound"] = abs(d32["cross_helicity"] - d64["cross_helicity"]) / math.sqrt(
+        d64["ke2"] * 2.0 * d64["emag"])
+    rel["divbmax_abs"] = abs(d32["divbmax"] - d64["divbmax"])
+    return rel
+
+
+def fp32_developed(N, steps):
+    """FP32 against FP64 on a developed state: the gradient IC (spike_mhd.cu:304-317) with the OU kick
+    on, `steps` steps, each precision with its own CFL dt."""
+    import torch
+    from glmmhdturb_amd import Solver, dxof, step_args
+    p = make_params(N)
+    out = {}
+    for prec in ("f64", "f32"):
+        s = Solver(p, prec)
+        u1, u2 = s.new_state(), s.new_state()
+        s.ic_gradient(u1, p.gamma)
+        f = s.setup_forcing(p, AMP)
+        t, cmax = 0.0, s.ctot_max(u1)
+        for _ in range(steps):
+            while t >= f.next_time:
+                f.advance()
+            dt = min(p.courant * dxof(p) / cmax, f.next_time - t)
+            s.step(u1, u2, step_args(p, dt, t, do_turb=True, riemann="hlld", recon="ppm", want_cfl=True))
+            u1, u2 = u2, u1
+            t += dt
+            cmax = s.cfl_max()
+        torch.cuda.synchronize()
+        out[prec] = s.diagnostics(u1)
+        s.close()
+    return diag_errors(out["f64"], out["f32"])
+
+
+def measured_traffic(N):
+    """HBM bytes per launch of the FP64 march kernel from the committed, calibrated PMC summary."""
+    path = os.path.join(ROOT, "profiles", "march_fp64_256_ou_hbm_traffic.txt")
+    if N != 256 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(seconds):
@@ -201,15 +249,22 @@ def main():
     bpl = BYTES_PER_CELL["f64"] * N ** 3
     ach = bpl / (r64["kernel_ms"] * 1e-3)
     rec["roofline"] = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": ach / HBM_PEAK, "traffic": None,
+                       "frac": ach / HBM_PEAK, "traffic": measured_traffic(N),
+                       "traffic_source": "profiles/march_fp64_256_ou_hbm_traffic.txt (FETCH_SIZE x2.0 and WRITE_SIZE "
+                                         "x1.0 calibrated on a 1 GiB elementwise copy)",
                        "kernel": "glm::march_kernel<double, PRED_FULL, RIEM_HLLD>",
                        "kernel_ms": r64["kernel_ms"], "bytes_per_cell": BYTES_PER_CELL["f64"]}
     rec["state"] = {"t": r64["t"], "mach": r64["mach"], "divbmax": r64["divbmax"], "finite": r64["finite"]}
     if not args.no_fp32:
         r32 = run_gpu(N, args.steps, args.warmup, "f32")
         ach32 = BYTES_PER_CELL["f32"] * N ** 3 / (r32["kernel_ms"] * 1e-3)
+        rel = diag_errors(r64["diag"], r32["diag"])
         rec["fp32"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "kernel_ms": r32["kernel_ms"],
-                       "hbm_frac": ach32 / HBM_PEAK, "speedup_vs_fp64": r32["value"] / r64["value"]}
+                       "hbm_frac": ach32 / HBM_PEAK, "speedup_vs_fp64": r32["value"] / r64["value"],
+                       "steps_compared": args.steps + args.warmup,
+                       "diag_rel_err_vs_fp64_from_rest": rel,
+                       "diag_rel_err_vs_fp64_gradient_ic_100_steps": fp32_developed(N, 100),
+                       "note": "FP32 state and fluxes; FP64 pressure recovery, CFL speed and diagnostics"}
     if not args.no_gradient:
         rec["gradient_ic_fp64"] = run_gradient_ic(N, max(5, args.steps // 3), "f64")
     if args.copy_calib:

@@ -19,16 +19,16 @@ GLM_DEV int awrap(int i, int n)
 template <typename T>
 __global__ void __launch_bounds__(AUX_NT) ctot_kernel(Ptr9<const T> q9, size_t n, Phys<T> s, unsigned long long* out)
 {
-    T cmax = T(0);
+    double cmax = 0.0;
     for (size_t c = (size_t)blockIdx.x * AUX_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * AUX_NT) {
         T u[9], p[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) u[v] = q9[v][c];
         cons2prim(u, p, s);
-        T ct = cfl_speed(p, s);
+        double ct = cfl_speed(p, s);
         if (!(ct <= cmax)) cmax = ct;
     }
-    unsigned long long bits = (unsigned long long)__double_as_longlong((double)cmax);
+    unsigned long long bits = (unsigned long long)__double_as_longlong(cmax);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         unsigned long long o = __shfl_xor(bits, off, 64);

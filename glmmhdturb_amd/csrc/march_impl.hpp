@@ -275,8 +275,8 @@ GLM_DEV void force_accel(ForceCache<T>& fc, const ColdArgs<T>& a, int gx, int gy
 // Update epilogue of one cell (glmmhd_turb.jl:440-476), CFL speed of the result
 // (glmmhd_turb.jl:1963-1971).  acc = (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) + (Fz_lo - Fz_hi).
 template <typename T>
-GLM_DEV T update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl, ForceCache<T>& fc,
-                      int gx, int gy, int gz)
+GLM_DEV double update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl,
+                           ForceCache<T>& fc, int gx, int gy, int gz)
 {
 #pragma unroll
     for (int v = 0; v < 9; ++v) r[v] = r[v] + a.dtdx * acc[v];
@@ -302,7 +302,7 @@ GLM_DEV T update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T
         e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
         r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
     }
-    if (!want_cfl) return T(0);
+    if (!want_cfl) return 0.0;
     T q[9];
     cons2prim(r, q, s);
     return cfl_speed(q, s);
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     T zprev[9], pend[9];
 #pragma unroll
     for (int v = 0; v < 9; ++v) { zprev[v] = T(0); pend[v] = T(0); }
-    T cmax = T(0);
+    double cmax = 0.0;
     ForceCache<T> fc;
     fc.k0 = -0x7fffffff;
     fc.b = &FCB[0][tid];
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
 #pragma unroll
             for (int v = 0; v < 9; ++v) acc[v] = pend[v] - fz[v];
             const int zq = wrapi(q - 1, nz);
-            T ct = update_cell(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
+            double ct = update_cell(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
             if (!(ct <= cmax)) cmax = ct;
             const size_t off = col + plane * zq;
 #pragma unroll
@@ -543,7 +543,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
         // atomic per workgroup on the bits of a non-negative double (NaN compares largest)
         __syncthreads();
         unsigned long long* red = reinterpret_cast<unsigned long long*>(&XB[0][0]);
-        red[tid] = (unsigned long long)__double_as_longlong((double)cmax);
+        red[tid] = (unsigned long long)__double_as_longlong(cmax);
         __syncthreads();
         if (tid < 64) {
             unsigned long long bits = 0ull;

@@ -67,17 +67,32 @@ GLM_DEV float trcp(float x)
 GLM_DEV double tsqrt(double a) { return __builtin_sqrt(a); }
 GLM_DEV float tsqrt(float a) { return __builtin_sqrtf(a); }
 
+// Gas pressure (gamma-1)(E - |m|^2/(2 rho) - |B|^2/2), floored (glmmhd_turb.jl:75).  With FP32 state
+// the subtraction is carried out in FP64: in low-beta or high-Mach cells E is dominated by the
+// magnetic or kinetic part and the FP32 difference would lose the thermal part.
+GLM_DEV double gas_pressure(const double c[9], double ir, double gm1, double pfl)
+{
+    double ekin = 0.5 * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
+    double emag = 0.5 * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
+    return __builtin_fmax(gm1 * (c[4] - ekin - emag), pfl);
+}
+GLM_DEV float gas_pressure(const float c[9], float ir, float gm1, float pfl)
+{
+    double m1 = c[1], m2 = c[2], m3 = c[3], b1 = c[5], b2 = c[6], b3 = c[7];
+    double ekin = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * (double)ir;
+    double emag = 0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
+    return __builtin_fmaxf((float)((double)gm1 * ((double)c[4] - ekin - emag)), pfl);
+}
+
 // glmmhd_turb.jl:68-77 cons2prim
 template <typename T>
 GLM_DEV void cons2prim(const T c[9], T q[9], const Phys<T>& s)
 {
     T rho = tmax(c[0], s.smallr);
     T ir = trcp(rho);
-    T ekin = T(0.5) * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
-    T emag = T(0.5) * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
     q[0] = rho;
     q[1] = c[1] * ir; q[2] = c[2] * ir; q[3] = c[3] * ir;
-    q[4] = tmax((s.gamma - T(1)) * (c[4] - ekin - emag), s.pfl);
+    q[4] = gas_pressure(c, ir, s.gamma - T(1), s.pfl);
     q[5] = c[5]; q[6] = c[6]; q[7] = c[7]; q[8] = c[8];
 }
 
@@ -105,18 +120,23 @@ GLM_DEV T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
 }
 
 // CFL signal speed sum_d (|v_d| + c_f(B_d)) of a primitive state (glmmhd_turb.jl:1963-1971),
-// sharing 1/rho, a^2 and the half sum between the three directions.
+// sharing 1/rho, a^2 and the half sum between the three directions.  Evaluated in FP64 for either
+// state precision: the step size is a global FP64 reduction.
 template <typename T>
-GLM_DEV T cfl_speed(const T q[9], const Phys<T>& s)
+GLM_DEV double cfl_speed(const T qt[9], const Phys<T>& st)
 {
-    T ir = trcp(q[0]);
-    T c2 = s.gamma * q[4] * ir;
-    T bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
-    T d2 = T(0.5) * (bb + c2);
-    T dd = d2 * d2, k = c2 * ir;
-    T fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], T(0))));
-    T fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], T(0))));
-    T fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], T(0))));
+    double q[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) q[v] = (double)qt[v];
+    struct { double gamma; } s = {(double)st.gamma};
+    double ir = trcp(q[0]);
+    double c2 = s.gamma * q[4] * ir;
+    double bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
+    double d2 = 0.5 * (bb + c2);
+    double dd = d2 * d2, k = c2 * ir;
+    double fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], 0.0)));
+    double fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], 0.0)));
+    double fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], 0.0)));
     return (tabs(q[1]) + fx) + (tabs(q[2]) + fy) + (tabs(q[3]) + fz);
 }
 

@@ -6,10 +6,14 @@ the host-side mirror of the reference's Julia interfaces:
 * ``march_bridge`` -- march_bridge/MarchBridge.jl (open_lib, set_dtdx, set_glm, set_gamma, run!)
 * ``turb``         -- glmmhd_turb.jl's Params / step_plm! / compute_dt / diagnostics / ICs /
                       Forcing / run_ot / run_turb
+* ``ensemble``     -- many run_turb boxes of uneven size advanced together (no reference
+                      counterpart): Ensemble, BoxSpec, random_boxes
 """
 from . import _lib  # noqa: F401
 from .turb import (Params, Solver, Forcing, dxof, pfloor, glm_speed, ramp_of, step_args, run_ot,  # noqa: F401
                    run_turb, SCHEMES)
 
+from .ensemble import BoxSpec, Ensemble, random_boxes, single_box_run  # noqa: F401
+
 __all__ = ["Params", "Solver", "Forcing", "dxof", "pfloor", "glm_speed", "ramp_of", "step_args", "run_ot",
-           "run_turb", "SCHEMES"]
+           "run_turb", "SCHEMES", "BoxSpec", "Ensemble", "random_boxes", "single_box_run"]

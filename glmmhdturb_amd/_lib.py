@@ -29,7 +29,13 @@ EXPORTS = (
     "glm_ctot_max", "glm_diagnostics", "glm_ic", "glm_ou_setup", "glm_ou_advance", "glm_ou_afield",
     "glm_ou_steps", "glm_ou_copy_afield", "glm_ct_step", "glm_ct_ctot_max", "glm_ct_divb_max", "glm_last_error", "glm_error_string",
     "glm_kernel_info",
+    "glm_ens_create", "glm_ens_destroy", "glm_ens_stream", "glm_ens_reset", "glm_ens_step", "glm_ens_graph",
+    "glm_ens_sync", "glm_ens_sweeps", "glm_ens_current", "glm_ens_log", "glm_ens_timing", "glm_ens_configure",
+    "glm_ens_tune",
 )
+ENS_NLOG = 8
+ENS_LOG_LEN = 256
+ENS_LOG_KEYS = ("sweep", "t", "dt", "ch", "glm_fac", "ramp", "cmax", "ou_due")
 
 
 class GLMParams(C.Structure):
@@ -38,6 +44,16 @@ class GLMParams(C.Structure):
         "turb_min_rho", "ramp")] + [(n, C.c_int) for n in (
         "predictor", "recon", "slope", "riemann", "mh_revert", "face_guard", "update_floor", "do_turb",
         "zchunk", "want_cfl")]
+
+
+class GLMEnsBox(C.Structure):
+    """glm_ens_box (include/glmmhd.h, section 3)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("do_turb", C.c_int),
+                ("u0", C.c_void_p * 9), ("u1", C.c_void_p * 9),
+                ("boxlen", C.c_double), ("courant", C.c_double), ("ch_scale", C.c_double), ("cp_coef", C.c_double),
+                ("g", C.c_int), ("ndt", C.c_int),
+                ("kmin", C.c_double), ("kmax", C.c_double), ("turb_T", C.c_double), ("comp_frac", C.c_double),
+                ("amp", C.c_double), ("seed", C.c_ulonglong)]
 
 
 class GLMError(RuntimeError):
@@ -77,6 +93,12 @@ def load(path: str | None = None):
         "glm_ct_divb_max": (i, [vp, pp, d, C.POINTER(d)]),
         "glm_last_error": (i, []), "glm_error_string": (C.c_char_p, [i]),
         "glm_kernel_info": (i, [i, i, i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "glm_ens_create": (vp, [i, C.POINTER(GLMEnsBox), C.POINTER(GLMParams), i]),
+        "glm_ens_destroy": (None, [vp]), "glm_ens_stream": (vp, [vp]), "glm_ens_reset": (i, [vp]),
+        "glm_ens_step": (i, [vp, i]), "glm_ens_graph": (i, [vp, i]), "glm_ens_sync": (i, [vp]),
+        "glm_ens_sweeps": (C.c_longlong, [vp]), "glm_ens_current": (i, [vp, i, pp]),
+        "glm_ens_log": (i, [vp, i, i, C.POINTER(d)]), "glm_ens_timing": (i, [vp, i, C.POINTER(d)]),
+        "glm_ens_configure": (i, [vp, i, i, i]), "glm_ens_tune": (i, [vp, i, C.POINTER(i), C.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,4 +1,4 @@
-// aux.hip -- instantiations of the reduction / initial-condition / OU kernels (aux_impl.hpp).
+// aux.hip -- instantiations of the reduction and initial-condition kernels (aux_impl.hpp).
 #include "aux_impl.hpp"
 #include "aux_launch.hpp"
 
@@ -52,44 +52,10 @@ hipError_t launch_ic(Ptr9<T> q9dev, int nx, int ny, int nz, int kind, const doub
     return hipGetLastError();
 }
 
-hipError_t launch_ou_init(int g, double kmin, double kmax, double* sqp, double* unitk, hipStream_t st)
-{
-    size_t g3 = (size_t)g * g * g;
-    hipLaunchKernelGGL(ou_init_kernel, dim3((unsigned)((g3 + 255) / 256)), dim3(256), 0, st, g, kmin, kmax, sqp, unitk);
-    return hipGetLastError();
-}
-
-hipError_t launch_ou_step(int g, double* fre, double* fim, const double* sqp, const double* unitk, double alpha,
-                          double beta, double comp, double sol, unsigned long long seed, unsigned long long step,
-                          hipStream_t st)
-{
-    size_t g3 = (size_t)g * g * g;
-    hipLaunchKernelGGL(ou_step_kernel, dim3((unsigned)((g3 + 255) / 256)), dim3(256), 0, st, g, fre, fim, sqp, unitk,
-                       alpha, beta, comp, sol, seed, step);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_ou_afield(int g, const double* fre, const double* fim, double* work, const double* cs,
-                            const double* sn, double amp, T* afield, hipStream_t st)
-{
-    size_t g3 = (size_t)g * g * g, n3 = 3 * g3;
-    double *ar = work, *ai = work + n3, *br = work + 2 * n3, *bi = work + 3 * n3;
-    dim3 gr((unsigned)((n3 + 255) / 256)), bl(256);
-    hipLaunchKernelGGL(idft_pass_kernel, gr, bl, 0, st, g, 0, fre, fim, ar, ai, cs, sn);
-    hipLaunchKernelGGL(idft_pass_kernel, gr, bl, 0, st, g, 1, ar, ai, br, bi, cs, sn);
-    hipLaunchKernelGGL(idft_pass_kernel, gr, bl, 0, st, g, 2, br, bi, ar, ai, cs, sn);
-    double scale = amp / ((double)g * (double)g * (double)g);
-    hipLaunchKernelGGL((afield_kernel<T>), gr, bl, 0, st, g, ar, scale, afield);
-    return hipGetLastError();
-}
-
 #define GLM_INST(T)                                                                                                   \
     template hipError_t launch_ctot<T>(Ptr9<const T>, size_t, const Phys<T>&, unsigned long long*, hipStream_t);     \
     template hipError_t launch_diag<T>(Ptr9<const T>, int, int, int, double, double, void*, double*, hipStream_t);  \
-    template hipError_t launch_ic<T>(Ptr9<T>, int, int, int, int, const double*, hipStream_t);                      \
-    template hipError_t launch_ou_afield<T>(int, const double*, const double*, double*, const double*, const double*, \
-                                            double, T*, hipStream_t);
+    template hipError_t launch_ic<T>(Ptr9<T>, int, int, int, int, const double*, hipStream_t);
 GLM_INST(double)
 GLM_INST(float)
 #undef GLM_INST

@@ -1,4 +1,4 @@
-// aux_impl.hpp -- reductions, initial conditions and OU forcing kernels for gfx950.
+// aux_impl.hpp -- reductions and initial-condition kernels for gfx950.
 #pragma once
 #include "physics.hpp"
 
@@ -205,116 +205,6 @@ __global__ void ic_gradient_kernel(Ptr9<T> q9, size_t n, T gamma)
         q9[4][c] = p / (gamma - T(1)) + ekin + emag;
         q9[5][c] = bx; q9[6][c] = by; q9[7][c] = bz; q9[8][c] = T(0);
     }
-}
-
-// ---------------------------------------------------------------------------------------------
-// OU forcing (glmmhd_turb.jl:2070-2161) with a seeded Philox4x32-10 generator and a
-// kmin <= |k| <= kmax shell; all in double.  Fourier layout: component planes of g^3 modes,
-// mode (a,b,d) at a + g*(b + g*d); re and im in separate arrays.
-// ---------------------------------------------------------------------------------------------
-GLM_DEV void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-        uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
-        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
-    }
-}
-
-GLM_DEV void ou_gauss6(unsigned long long seed, unsigned long long step, uint32_t mode, double g[6])
-{
-    double z[8];
-#pragma unroll
-    for (int blk = 0; blk < 2; ++blk) {
-        uint32_t c[4] = {mode, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)blk};
-        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            double u1 = ((double)c[2 * p] + 0.5) * 2.3283064365386963e-10;
-            double u2 = ((double)c[2 * p + 1] + 0.5) * 2.3283064365386963e-10;
-            double r = sqrt(-2.0 * log(u1));
-            double th = 6.283185307179586 * u2;
-            z[4 * blk + 2 * p] = r * cos(th);
-            z[4 * blk + 2 * p + 1] = r * sin(th);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) g[q] = z[q];
-}
-
-// glmmhd_turb.jl:2078-2091 ou_init_kernel! on a shell
-__global__ void ou_init_kernel(int g, double kmin, double kmax, double* sqp, double* unitk)
-{
-    size_t g3 = (size_t)g * g * g;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g3) return;
-    int a = (int)(m % g), b = (int)((m / g) % g), d = (int)(m / ((size_t)g * g));
-    auto kf = [&](int i) { return i <= g / 2 ? (double)i : (double)(i - g); };
-    double kx = kf(a), ky = kf(b), kz = kf(d);
-    double kk = sqrt(kx * kx + ky * ky + kz * kz);
-    bool on = kk >= kmin && kk <= kmax;
-    sqp[m] = on ? 1.0 / kk : 0.0;
-    unitk[m] = on ? kx / kk : 0.0;
-    unitk[g3 + m] = on ? ky / kk : 0.0;
-    unitk[2 * g3 + m] = on ? kz / kk : 0.0;
-}
-
-// glmmhd_turb.jl:2094-2113 ou_step_kernel!
-__global__ void ou_step_kernel(int g, double* fre, double* fim, const double* sqp, const double* unitk, double alpha,
-                               double beta, double comp, double sol, unsigned long long seed, unsigned long long step)
-{
-    size_t g3 = (size_t)g * g * g;
-    size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g3) return;
-    double s = sqp[m];
-    double gg[6] = {0, 0, 0, 0, 0, 0};
-    if (s != 0.0) ou_gauss6(seed, step, (uint32_t)m, gg);
-    double wr[3] = {gg[0] * s, gg[2] * s, gg[4] * s};
-    double wi[3] = {gg[1] * s, gg[3] * s, gg[5] * s};
-    double u[3] = {unitk[m], unitk[g3 + m], unitk[2 * g3 + m]};
-    double kr = u[0] * wr[0] + u[1] * wr[1] + u[2] * wr[2];
-    double ki = u[0] * wi[0] + u[1] * wi[1] + u[2] * wi[2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double cr = u[c] * kr, ci = u[c] * ki;
-        double pr = comp * cr + sol * (wr[c] - cr);
-        double pi = comp * ci + sol * (wi[c] - ci);
-        fre[c * g3 + m] = alpha * fre[c * g3 + m] + beta * pr;
-        fim[c * g3 + m] = alpha * fim[c * g3 + m] + beta * pi;
-    }
-}
-
-// One 1-D unnormalised inverse DFT pass along axis ax over 3 stacked g^3 components.
-__global__ void idft_pass_kernel(int g, int ax, const double* ir, const double* ii, double* orr, double* oi,
-                                 const double* cs, const double* sn)
-{
-    size_t g3 = (size_t)g * g * g;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * g3) return;
-    size_t comp = t / g3, base = t % g3;
-    size_t stride = ax == 0 ? 1 : (ax == 1 ? (size_t)g : (size_t)g * g);
-    int pos = (int)((base / stride) % (size_t)g);
-    size_t line0 = comp * g3 + base - (size_t)pos * stride;
-    double sr = 0.0, si = 0.0;
-    for (int a = 0; a < g; ++a) {
-        int tw = (a * pos) % g;
-        double xr = ir[line0 + (size_t)a * stride], xi = ii[line0 + (size_t)a * stride];
-        sr += xr * cs[tw] - xi * sn[tw];
-        si += xr * sn[tw] + xi * cs[tw];
-    }
-    orr[t] = sr;
-    oi[t] = si;
-}
-
-template <typename T>
-__global__ void afield_kernel(int g, const double* re, double scale, T* afield)
-{
-    size_t n = 3 * (size_t)g * g * g;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) afield[t] = (T)(re[t] * scale);
 }
 
 }  // namespace glm

@@ -17,12 +17,5 @@ hipError_t launch_diag(Ptr9<const T> q9dev, int nx, int ny, int nz, double dx, d
                        double* out, hipStream_t st);
 template <typename T>
 hipError_t launch_ic(Ptr9<T> q9dev, int nx, int ny, int nz, int kind, const double* args, hipStream_t st);
-hipError_t launch_ou_init(int g, double kmin, double kmax, double* sqp, double* unitk, hipStream_t st);
-hipError_t launch_ou_step(int g, double* fre, double* fim, const double* sqp, const double* unitk, double alpha,
-                          double beta, double comp, double sol, unsigned long long seed, unsigned long long step,
-                          hipStream_t st);
-template <typename T>
-hipError_t launch_ou_afield(int g, const double* fre, const double* fim, double* work, const double* cs,
-                            const double* sn, double amp, T* afield, hipStream_t st);
 
 }  // namespace glm

@@ -9,7 +9,9 @@
 #include "../../include/glmmhd.h"
 #include "aux_launch.hpp"
 #include "ct_launch.hpp"
+#include "host_common.hpp"
 #include "launch.hpp"
+#include "ou_launch.hpp"
 
 #ifndef GLM_DEFAULT_N
 #define GLM_DEFAULT_N 480
@@ -26,24 +28,21 @@ using namespace glm;
 static thread_local int t_last_error = GLM_OK;
 static thread_local char t_last_msg[256] = "";
 
-static int set_err(int code, const char* msg)
+namespace glm {
+int set_err(int code, const char* msg)
 {
     t_last_error = code;
     snprintf(t_last_msg, sizeof t_last_msg, "%s", msg);
     return code;
 }
-static int hip_err(hipError_t e, const char* where)
+int hip_err(hipError_t e, const char* where)
 {
     if (e == hipSuccess) return GLM_OK;
     char buf[256];
     snprintf(buf, sizeof buf, "%s: %s", where, hipGetErrorString(e));
     return set_err(GLM_ERR_HIP, buf);
 }
-#define HIPCHK(call)                                                   \
-    do {                                                               \
-        hipError_t e_ = (call);                                        \
-        if (e_ != hipSuccess) return hip_err(e_, #call);               \
-    } while (0)
+}  // namespace glm
 
 // ------------------------------------------------------------------------------------------
 // context
@@ -57,14 +56,11 @@ struct glm_ctx {
     int cfl_slot;                // slot holding the CFL max of the last step with want_cfl (-1: none)
     void* d_parts;
     double* d_diag;
-    // OU forcing
+    // OU forcing: one band-limited box (host copy of the descriptor, and the device copy)
     int g;
-    double kmin, kmax, T;
-    int ndt;
-    double comp, amp;
-    unsigned long long seed;
     long long nstep;
-    double *fre, *fim, *sqp, *unitk, *work, *cs, *sn;
+    OUBox ou;
+    OUBox* d_ou;
     void* afield;
     // CT scratch (face fluxes / EMFs of the multi-pass CT path)
     void* ct_scratch;
@@ -73,10 +69,10 @@ struct glm_ctx {
 static void ctx_free(glm_ctx* c)
 {
     if (!c) return;
-    void* ptrs[] = {c->d_cmax, c->d_parts, c->d_diag, c->fre, c->fim, c->sqp, c->unitk,
-                    c->work, c->cs, c->sn, c->afield, c->ct_scratch};
+    void* ptrs[] = {c->d_cmax, c->d_parts, c->d_diag, c->d_ou, c->ct_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    ou_box_free(c->ou);
     delete c;
 }
 
@@ -148,34 +144,6 @@ int glm_set_stream(glm_ctx* ctx, void* stream)
 }
 
 }  // extern "C"
-
-template <typename T>
-static Phys<T> make_phys(const glm_params* p)
-{
-    double pfl = p->pfl > 0.0 ? p->pfl : p->smallr * p->smallc * p->smallc / p->gamma;
-    Phys<T> s;
-    s.gamma = (T)p->gamma;
-    s.igm1 = (T)(1.0 / (p->gamma - 1.0));
-    s.smallr = (T)p->smallr;
-    s.pfl = (T)pfl;
-    s.ch = (T)p->ch;
-    s.ch2 = (T)(p->ch * p->ch);
-    s.half_ich = (T)(0.5 / p->ch);
-    s.llf_dmin = (T)p->llf_dmin;
-    s.llf_pmin = (T)p->llf_pmin;
-    return s;
-}
-
-static int check_params(const glm_params* p)
-{
-    if (!p) return set_err(GLM_ERR_ARG, "null glm_params");
-    if (!(p->gamma > 1.0) || !(p->dx > 0.0) || !(p->ch > 0.0) || !(p->smallr > 0.0))
-        return set_err(GLM_ERR_ARG, "glm_params: need gamma > 1, dx > 0, ch > 0, smallr > 0");
-    if (p->predictor < 0 || p->predictor > 1 || p->recon < 0 || p->recon > 2 || p->slope < 0 || p->slope > 1 ||
-        p->riemann < 0 || p->riemann > 3 || p->face_guard < 0 || p->face_guard > 2)
-        return set_err(GLM_ERR_ARG, "glm_params: scheme switch out of range");
-    return GLM_OK;
-}
 
 template <typename T>
 static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_params* p)
@@ -305,49 +273,26 @@ int glm_ou_setup(glm_ctx* ctx, int g, double kmin, double kmax, double T, int nd
     if (!ctx) return set_err(GLM_ERR_ARG, "glm_ou_setup: null context");
     if (g < 2 || g > 256 || !(T > 0.0) || ndt < 1 || !(kmax >= kmin))
         return set_err(GLM_ERR_ARG, "glm_ou_setup: need 2 <= g <= 256, T > 0, ndt >= 1, kmax >= kmin");
-    void* olds[] = {ctx->fre, ctx->fim, ctx->sqp, ctx->unitk, ctx->work, ctx->cs, ctx->sn, ctx->afield};
-    for (void* p : olds)
-        if (p) (void)hipFree(p);
-    ctx->fre = ctx->fim = ctx->sqp = ctx->unitk = ctx->work = ctx->cs = ctx->sn = nullptr;
-    ctx->afield = nullptr;
-    size_t g3 = (size_t)g * g * g;
-    ctx->g = g; ctx->kmin = kmin; ctx->kmax = kmax; ctx->T = T; ctx->ndt = ndt;
-    ctx->comp = comp_frac; ctx->amp = amp; ctx->seed = seed; ctx->nstep = 0;
-    if (hipMalloc(&ctx->fre, 3 * g3 * 8) != hipSuccess || hipMalloc(&ctx->fim, 3 * g3 * 8) != hipSuccess ||
-        hipMalloc(&ctx->sqp, g3 * 8) != hipSuccess || hipMalloc(&ctx->unitk, 3 * g3 * 8) != hipSuccess ||
-        hipMalloc(&ctx->work, 4 * 3 * g3 * 8) != hipSuccess || hipMalloc(&ctx->cs, g * 8) != hipSuccess ||
-        hipMalloc(&ctx->sn, g * 8) != hipSuccess || hipMalloc(&ctx->afield, 3 * g3 * ctx->prec) != hipSuccess)
-        return set_err(GLM_ERR_ALLOC, "glm_ou_setup: device allocation failed");
-    std::vector<double> cs(g), sn(g);
-    for (int m = 0; m < g; ++m) {
-        cs[m] = std::cos(2.0 * M_PI * m / g);
-        sn[m] = std::sin(2.0 * M_PI * m / g);
-    }
-    HIPCHK(hipMemcpy(ctx->cs, cs.data(), g * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->sn, sn.data(), g * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemsetAsync(ctx->fre, 0, 3 * g3 * 8, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->fim, 0, 3 * g3 * 8, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->afield, 0, 3 * g3 * ctx->prec, ctx->stream));
-    HIPCHK(launch_ou_init(g, kmin, kmax, ctx->sqp, ctx->unitk, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ou_box_free(ctx->ou);
+    ctx->afield = nullptr;
+    ctx->g = g;
+    ctx->nstep = 0;
+    if (ou_box_alloc(ctx->ou, g, kmin, kmax, T, ndt, comp_frac, amp, seed, ctx->prec, ctx->stream) != hipSuccess)
+        return set_err(GLM_ERR_ALLOC, "glm_ou_setup: device allocation failed");
+    if (!ctx->d_ou && hipMalloc(&ctx->d_ou, sizeof(OUBox)) != hipSuccess)
+        return set_err(GLM_ERR_ALLOC, "glm_ou_setup: device allocation failed");
+    HIPCHK(hipMemcpyAsync(ctx->d_ou, &ctx->ou, sizeof(OUBox), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->afield = ctx->ou.afield;
     return GLM_OK;
 }
 
 int glm_ou_advance(glm_ctx* ctx)
 {
-    if (!ctx || !ctx->fre) return set_err(GLM_ERR_STATE, "glm_ou_advance: call glm_ou_setup first");
-    double dt = ctx->T / ctx->ndt;             // glmmhd_turb.jl:2148-2150
-    double alpha = 1.0 - dt / ctx->T;
-    double beta = std::sqrt(2.0 * dt / ctx->T);
-    HIPCHK(launch_ou_step(ctx->g, ctx->fre, ctx->fim, ctx->sqp, ctx->unitk, alpha, beta, ctx->comp, 1.0 - ctx->comp,
-                          ctx->seed, (unsigned long long)ctx->nstep, ctx->stream));
+    if (!ctx || !ctx->afield) return set_err(GLM_ERR_STATE, "glm_ou_advance: call glm_ou_setup first");
+    HIPCHK(launch_ou_advance(ctx->prec, ctx->d_ou, 1, ctx->ou.g, ctx->ou.nb, ctx->nstep, ctx->stream));
     ctx->nstep += 1;
-    if (ctx->prec == 8)
-        HIPCHK(launch_ou_afield<double>(ctx->g, ctx->fre, ctx->fim, ctx->work, ctx->cs, ctx->sn, ctx->amp,
-                                        (double*)ctx->afield, ctx->stream));
-    else
-        HIPCHK(launch_ou_afield<float>(ctx->g, ctx->fre, ctx->fim, ctx->work, ctx->cs, ctx->sn, ctx->amp,
-                                       (float*)ctx->afield, ctx->stream));
     return GLM_OK;
 }
 

@@ -12,6 +12,10 @@ template <typename T, int PRED, int RIEM>
 hipError_t launch_march_pr(const MarchArgs<T>& a, int recon, int slope, hipStream_t st);
 template <typename T, int PRED, int RIEM>
 const void* march_kernel_ptr_pr(int recon, int slope);
+template <typename T, int PRED, int RIEM>
+hipError_t launch_ens_pr(const EnsLaunch<T>& e, int recon, int slope, int grid, hipStream_t st);
+template <typename T, int PRED, int RIEM>
+const void* ens_kernel_ptr_pr(int recon, int slope);
 
 template <typename T>
 inline hipError_t launch_march(const MarchArgs<T>& a, int pred, int riem, int recon, int slope, hipStream_t st)
@@ -30,6 +34,28 @@ inline const void* march_kernel_ptr(int pred, int riem, int recon, int slope)
 {
 #define GLM_CASE(P, R) \
     if (pred == P && riem == R) return march_kernel_ptr_pr<T, P, R>(recon, slope);
+    GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
+    GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
+#undef GLM_CASE
+    return nullptr;
+}
+
+template <typename T>
+inline hipError_t launch_ens(const EnsLaunch<T>& e, int pred, int riem, int recon, int slope, int grid, hipStream_t st)
+{
+#define GLM_CASE(P, R) \
+    if (pred == P && riem == R) return launch_ens_pr<T, P, R>(e, recon, slope, grid, st);
+    GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
+    GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
+#undef GLM_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+inline const void* ens_kernel_ptr(int pred, int riem, int recon, int slope)
+{
+#define GLM_CASE(P, R) \
+    if (pred == P && riem == R) return ens_kernel_ptr_pr<T, P, R>(recon, slope);
     GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
     GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
 #undef GLM_CASE

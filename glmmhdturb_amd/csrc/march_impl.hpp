@@ -308,33 +308,47 @@ GLM_DEV double update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const P
     return cfl_speed(q, s);
 }
 
-template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) march_kernel(MarchArgs<T> a)
-{
-    __shared__ T P[9][NPL];
-    __shared__ T XB[9][NPL];   // +x face states, then x fluxes, at the LDS plane position of the cell
-    __shared__ T YB[9][NPL];
-    __shared__ ColdArgs<T> ca;
-    __shared__ T FCB[6][NT];
-    __shared__ T UR[3][9][NO];   // owned lanes' conserved state of the last three planes loaded
-                                 // (plane k in slot k mod 3): the update base, read once from HBM
+// LDS of one march workgroup.
+template <typename T>
+struct MarchLDS {
+    T P[9][NPL];    // primitives of the current plane over the tile grown by two cells
+    T XB[9][NPL];   // +x face states, then x fluxes, at the LDS plane position of the cell
+    T YB[9][NPL];
+    T FCB[6][NT];   // forcing interpolation cache (ForceCache)
+    T UR[3][9][NO]; // owned lanes' conserved state of the last three planes loaded (plane k in
+                    // slot k mod 3): the update base, read once from HBM
+    ColdArgs<T> ca;
+};
 
+template <typename T>
+GLM_DEV void stage_cold(const MarchArgs<T>& a, ColdArgs<T>& ca)
+{
+    for (int v = 0; v < 9; ++v) { ca.q[v] = a.q[v]; ca.o[v] = a.o[v]; }
+    ca.afield = a.afield;
+    ca.dtdx = a.dtdx; ca.glm_fac = a.glm_fac; ca.smallc = a.smallc; ca.turb_min_rho = a.turb_min_rho;
+    ca.ramp = a.ramp; ca.dt = a.dt; ca.dx = a.dx; ca.lx = a.lx; ca.ly = a.ly; ca.lz = a.lz;
+    ca.update_floor = a.update_floor; ca.do_turb = a.do_turb; ca.tgs = a.tgs;
+}
+
+// The march of one tile: the workgroup's lanes march the columns of tile (tile_x, tile_y) through
+// z-chunk zc, tile = tile_x + tiles_x * (tile_y + tiles_y * zc), and write the updated planes.
+// L.ca must hold `a`'s cold arguments, published by a barrier.  Returns the lane's max CFL speed
+// over the cells it updated.
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
+GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
+{
+    T (&P)[9][NPL] = L.P;
+    T (&XB)[9][NPL] = L.XB;
+    T (&YB)[9][NPL] = L.YB;
+    T (&UR)[3][9][NO] = L.UR;
+    const ColdArgs<T>& ca = L.ca;
     const Phys<T>& s = a.ph;
     const int tid = threadIdx.x;
     int tx, ty;
     lane_cell(tid, tx, ty);
-    if (tid == 0) {
-        for (int v = 0; v < 9; ++v) { ca.q[v] = a.q[v]; ca.o[v] = a.o[v]; }
-        ca.afield = a.afield;
-        ca.dtdx = a.dtdx; ca.glm_fac = a.glm_fac; ca.smallc = a.smallc; ca.turb_min_rho = a.turb_min_rho;
-        ca.ramp = a.ramp; ca.dt = a.dt; ca.dx = a.dx; ca.lx = a.lx; ca.ly = a.ly; ca.lz = a.lz;
-        ca.update_floor = a.update_floor; ca.do_turb = a.do_turb; ca.tgs = a.tgs;
-    }
-    __syncthreads();
-    int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const int tile_x = b % a.tiles_x; b /= a.tiles_x;
-    const int tile_y = b % a.tiles_y; b /= a.tiles_y;
-    const int zs = b * a.zchunk;
+    const int tile_x = tile % a.tiles_x; tile /= a.tiles_x;
+    const int tile_y = tile % a.tiles_y; tile /= a.tiles_y;
+    const int zs = tile * a.zchunk;
     const int ze = min(zs + a.zchunk, a.nz);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const size_t plane = (size_t)nx * ny;
@@ -351,7 +365,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     const size_t hcol = (size_t)wrapi(tile_x * OX - 2 + hx, nx) + (size_t)nx * wrapi(tile_y * OY - 2 + hy, ny);
     const int lo_ring = hy * PX + hx;
 
-    if (a.want_cfl && blockIdx.x == 0 && tid == 0) *a.cmax_next = 0ull;
 
     // column window: primitives of planes q-1, q, q+1
     T pm[9], p0[9], pp[9];
@@ -383,7 +396,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
     double cmax = 0.0;
     ForceCache<T> fc;
     fc.k0 = -0x7fffffff;
-    fc.b = &FCB[0][tid];
+    fc.b = &L.FCB[0][tid];
 
     for (int q = zs - 1; q <= ze; ++q) {
         const bool last = q == ze;
@@ -538,23 +551,140 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
         __syncthreads();
     }
 
-    if (a.want_cfl) {
-        // workgroup max through LDS (the last wavefront of a TX*TY group may be partial), then one
-        // atomic per workgroup on the bits of a non-negative double (NaN compares largest)
-        __syncthreads();
-        unsigned long long* red = reinterpret_cast<unsigned long long*>(&XB[0][0]);
-        red[tid] = (unsigned long long)__double_as_longlong(cmax);
-        __syncthreads();
-        if (tid < 64) {
-            unsigned long long bits = 0ull;
-            for (int t = tid; t < NT; t += 64) bits = red[t] > bits ? red[t] : bits;
+    return cmax;
+}
+
+// Workgroup max of the lanes' CFL speeds through LDS (the last wavefront may be partial), then one
+// atomic on the bits of a non-negative double (NaN compares largest).
+template <typename T>
+GLM_DEV void cfl_merge(MarchLDS<T>& L, double cmax, unsigned long long* target)
+{
+    const int tid = threadIdx.x;
+    __syncthreads();
+    unsigned long long* red = reinterpret_cast<unsigned long long*>(&L.XB[0][0]);
+    red[tid] = (unsigned long long)__double_as_longlong(cmax);
+    __syncthreads();
+    if (tid < 64) {
+        unsigned long long bits = 0ull;
+        for (int t = tid; t < NT; t += 64) bits = red[t] > bits ? red[t] : bits;
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                unsigned long long o = __shfl_xor(bits, off, 64);
-                bits = o > bits ? o : bits;
-            }
-            if (tid == 0 && bits != 0ull) atomicMax(a.cmax, bits);
+        for (int off = 32; off >= 1; off >>= 1) {
+            unsigned long long o = __shfl_xor(bits, off, 64);
+            bits = o > bits ? o : bits;
         }
+        if (tid == 0 && bits != 0ull) atomicMax(target, bits);
+    }
+}
+
+// One step of one grid: workgroup w marches tile xcd_block(w) (x fastest, then y, then z-chunk).
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) march_kernel(MarchArgs<T> a)
+{
+    __shared__ MarchLDS<T> L;
+    if (threadIdx.x == 0) stage_cold(a, L.ca);
+    __syncthreads();
+    const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE>(a, L, xcd_block((int)blockIdx.x, (int)gridDim.x));
+    if (a.want_cfl) {
+        cfl_merge(L, cmax, a.cmax);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *a.cmax_next = 0ull;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ensemble: many boxes of uneven size in one launch per sweep.  The boxes' tiles form one flat
+// list (box positions in a chosen order, each box's tiles contiguous in its own x, y, z-chunk
+// order); tile_off[pos] is the first flat tile of position pos.  Each box's MarchArgs live in
+// device memory and are refreshed every sweep by the step-control kernel (ens.hip).
+// ---------------------------------------------------------------------------------------------
+constexpr int ENS_LOG = 256;   // sweeps kept in the per-box log and timing records
+
+template <typename T>
+struct EnsLaunch {
+    const MarchArgs<T>* boxes;   // per box
+    const int* order;            // flat position -> box
+    const int* tile_off;         // [npos + 1]
+    int npos, total;             // positions and flat tiles
+    int persistent;              // 0: workgroup w marches flat tile xcd_block(w); 1: workgroups take
+                                 // tiles from per-XCD queues and steal from the others when theirs is empty
+    int nbox;
+    unsigned int* queue;         // [NXCD] ticket counters, zeroed by the step-control kernel
+    const long long* sweep;      // index of the current sweep
+    unsigned long long* t0;      // [ENS_LOG][nbox] wall-clock start of a box's first tile in the sweep
+    unsigned long long* t1;      // [ENS_LOG][nbox] wall-clock end of its last tile
+};
+
+// A uniform struct load through the constant address space, so that it lands in scalar registers.
+template <typename S>
+GLM_DEV S uniform_load(const S* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) S*)p;
+#else
+    return *p;   // host pass of the kernel templates: never executed
+#endif
+}
+
+// First flat tile of XCD x's contiguous range (the inverse view of xcd_block) and its length.
+GLM_DEV int xcd_start(int x, int total)
+{
+    const int q = total / NXCD, r = total % NXCD;
+    return x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+}
+GLM_DEV int xcd_count(int x, int total) { return total / NXCD + (x < total % NXCD ? 1 : 0); }
+
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) ens_march_kernel(EnsLaunch<T> e)
+{
+    __shared__ MarchLDS<T> L;
+    __shared__ int s_job[2];   // box (-1: no more work) and its local tile
+    const int tid = threadIdx.x;
+    const long long sw = *e.sweep;   // -1 before the first sweep (tuning launches)
+    const int slot = (int)((sw < 0 ? 0 : sw) % ENS_LOG);
+    int xq = (int)blockIdx.x % NXCD;
+    unsigned long long tb = 0ull;
+    for (int it = 0;; ++it) {
+        if (tid == 0) {
+            int tile = -1;
+            if (!e.persistent) {
+                if (it == 0) tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
+            } else {
+                for (int k = 0; k < NXCD && tile < 0; ++k) {
+                    const int x = (xq + k) % NXCD;
+                    const unsigned tk = atomicAdd(&e.queue[x], 1u);
+                    if (tk < (unsigned)xcd_count(x, e.total)) {
+                        tile = xcd_start(x, e.total) + (int)tk;
+                        xq = x;
+                    }
+                }
+            }
+            int box = -1, lt = 0;
+            if (tile >= 0) {
+                int lo = 0, hi = e.npos;   // tile_off[lo] <= tile < tile_off[lo + 1]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (e.tile_off[mid] <= tile) lo = mid; else hi = mid;
+                }
+                box = e.order[lo];
+                lt = tile - e.tile_off[lo];
+                stage_cold(e.boxes[box], L.ca);
+                tb = (unsigned long long)wall_clock64();
+            }
+            s_job[0] = box;
+            s_job[1] = lt;
+        }
+        __syncthreads();
+        const int box = __builtin_amdgcn_readfirstlane(s_job[0]);
+        const int lt = __builtin_amdgcn_readfirstlane(s_job[1]);
+        if (box < 0) break;
+        const MarchArgs<T> a = uniform_load(e.boxes + box);
+        const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE>(a, L, lt);
+        cfl_merge(L, cmax, a.cmax);
+        if (tid == 0) {
+            const size_t k = (size_t)slot * e.nbox + box;
+            atomicMin(&e.t0[k], tb);
+            atomicMax(&e.t1[k], (unsigned long long)wall_clock64());
+        }
+        __syncthreads();   // LDS (job, cold args, reduction) is free for the next tile
     }
 }
 

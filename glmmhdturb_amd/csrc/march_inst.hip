@@ -1,5 +1,6 @@
 // march_inst.hip -- one family of fused line-march kernels: precision GLM_T, predictor GLM_PRED,
-// Riemann solver GLM_RIEM (set by the build), all reconstructions x limiters.
+// Riemann solver GLM_RIEM (set by the build), all reconstructions x limiters, for one grid
+// (march_kernel) and for an ensemble of boxes (ens_march_kernel).
 #include "launch.hpp"
 
 #ifndef GLM_T
@@ -28,6 +29,32 @@ const void* march_kernel_ptr_pr<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
 {
 #define GLM_CASE(RC, SL) \
     if (recon == RC && slope == SL) return (const void*)&march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>;
+    GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
+    GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
+#undef GLM_CASE
+    return nullptr;
+}
+
+template <>
+hipError_t launch_ens_pr<GLM_T, GLM_PRED, GLM_RIEM>(const EnsLaunch<GLM_T>& e, int recon, int slope, int grid,
+                                                    hipStream_t st)
+{
+#define GLM_CASE(RC, SL)                                                                                 \
+    if (recon == RC && slope == SL) {                                                                    \
+        hipLaunchKernelGGL((ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>), dim3((unsigned)grid), dim3(NT), 0, st, e); \
+        return hipGetLastError();                                                                        \
+    }
+    GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
+    GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
+#undef GLM_CASE
+    return hipErrorInvalidValue;
+}
+
+template <>
+const void* ens_kernel_ptr_pr<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
+{
+#define GLM_CASE(RC, SL) \
+    if (recon == RC && slope == SL) return (const void*)&ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>;
     GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
     GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
 #undef GLM_CASE

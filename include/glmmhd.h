@@ -151,6 +151,65 @@ int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* c
 /* max |div B| of a CT state from the face fields (ct_mhd.jl:191-194); synchronous. */
 int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
 
+/* ===================================================================================== */
+/* 3. Ensemble of boxes (no reference counterpart: glmmhd_turb.jl runs one box per run).   */
+/*    Boxes of uneven size advance together.  A sweep is one step of every box, each with   */
+/*    its own dt from its own CFL max and its own OU forcing -- the run_turb loop           */
+/*    (glmmhd_turb.jl:2191-2241) evaluated on the device -- and one march launch over all   */
+/*    boxes' tiles.  A sweep holds no host round trip, so it can be replayed as a HIP graph. */
+/*    All boxes share one scheme (glm_params) and one precision.                            */
+/* ===================================================================================== */
+typedef struct glm_ens glm_ens;
+
+typedef struct glm_ens_box {
+    int nx, ny, nz;
+    int do_turb;                /* OU kick with this box's forcing (below) */
+    void* u0[9];                /* state planes holding the initial state */
+    void* u1[9];                /* second state buffer of the same size */
+    double boxlen;              /* box length along x: dx = boxlen / nx (dxof, :54) */
+    double courant;             /* dt = courant*dx/cmax, clipped to the next OU time (:2222-2226) */
+    double ch_scale, cp_coef;   /* ch = courant*dx/dt/3*ch_scale; psi *= exp(-(ch^2/(cp*boxlen*ch))*dt) (:884-885) */
+    int g, ndt;                 /* OU forcing as glm_ou_setup: Fourier grid g, ndt OU steps per turb_T */
+    double kmin, kmax, turb_T, comp_frac, amp;
+    unsigned long long seed;
+} glm_ens_box;
+
+/* Create an ensemble of nbox (1..1024) boxes; precision 8 or 4.  The library keeps pointers to
+ * the caller's state buffers.  Returns NULL on failure (see glm_last_error). */
+glm_ens* glm_ens_create(int nbox, const glm_ens_box* boxes, const glm_params* scheme, int precision);
+void glm_ens_destroy(glm_ens* e);
+/* The ensemble's own HIP stream (hipStream_t as void*); all ensemble work runs on it. */
+void* glm_ens_stream(glm_ens* e);
+/* Start (or restart) every box at t = 0 from its u0: OU state zero, CFL max of u0 (ctot). */
+int glm_ens_reset(glm_ens* e);
+/* Enqueue nsweeps sweeps (asynchronous). */
+int glm_ens_step(glm_ens* e, int nsweeps);
+/* on = 1: capture one sweep into a HIP graph and replay it in glm_ens_step; 0: eager launches. */
+int glm_ens_graph(glm_ens* e, int on);
+int glm_ens_sync(glm_ens* e);
+/* Sweeps done since the last reset (synchronous). */
+long long glm_ens_sweeps(glm_ens* e);
+/* The 9 device planes holding the box's current state (synchronous). */
+int glm_ens_current(glm_ens* e, int box, void** planes);
+#define GLM_ENS_NLOG 8
+#define GLM_ENS_LOG_LEN 256
+/* The box's step log of the last `count` sweeps (count <= min(sweeps, GLM_ENS_LOG_LEN)), oldest
+ * first, GLM_ENS_NLOG values per sweep: sweep index, t at the start of the step, dt, ch,
+ * glm_fac, ramp, the CFL max dt came from, OU advance in this step (0/1).  Synchronous. */
+int glm_ens_log(glm_ens* e, int box, int count, double* out);
+/* Per sweep and box, the time from the box's first tile start to its last tile end in the march
+ * launch, in ms (device wall clock): count x nbox values, oldest sweep first.  Synchronous. */
+int glm_ens_timing(glm_ens* e, int count, double* box_ms);
+/* Launch configuration: zchunk = z planes per tile (0: automatic), persistent = 0 (one workgroup
+ * per tile) or k >= 1 (k workgroups per CU take tiles from per-XCD queues), order = 0 (boxes
+ * as given) or 1 (largest box first). */
+int glm_ens_configure(glm_ens* e, int zchunk, int persistent, int order);
+/* Autotuner: times `reps` march launches over the current states for each candidate
+ * configuration, keeps the fastest and returns it in best[3] = {zchunk, persistent, order} with
+ * its median launch time.  The states and the step log do not change (the second buffers are
+ * used as scratch).  Synchronous. */
+int glm_ens_tune(glm_ens* e, int reps, int* best, double* best_ms);
+
 /* Error handling (the reference has none: spike_mhd.cu's AS_LIB path never checks errors). */
 int glm_last_error(void);
 const char* glm_error_string(int code);

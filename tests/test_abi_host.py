@@ -65,6 +65,57 @@ def test_params_struct_matches_header():
     assert p.riemann == 3 and p.predictor == 0 and p.face_guard == 1 and p.update_floor == 1
 
 
+def test_ens_box_struct_matches_header():
+    """glm_ens_box field order and types as declared in include/glmmhd.h."""
+    txt = open(HEADER).read()
+    body = txt[txt.index("typedef struct glm_ens_box {"):txt.index("} glm_ens_box;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    ctypes_of = {"int": C.c_int, "double": C.c_double, "unsigned long long": C.c_ulonglong}
+    fields = []
+    for ctype, names in re.findall(r"\b(unsigned long long|double|int|void\*)\s+([^;]+);", body):
+        for n in names.split(","):
+            n = n.strip()
+            if ctype == "void*":
+                name, cnt = re.match(r"(\w+)\[(\d+)\]", n).groups()
+                fields.append((name, C.c_void_p * int(cnt)))
+            else:
+                fields.append((n, ctypes_of[ctype]))
+    got = [(n, t) for n, t in _lib.GLMEnsBox._fields_]
+    assert [n for n, _ in got] == [n for n, _ in fields]
+    for (n, t), (_, u) in zip(got, fields):
+        assert C.sizeof(t) == C.sizeof(u) and getattr(t, "_type_", None) == getattr(u, "_type_", None), n
+    assert C.sizeof(_lib.GLMEnsBox) == 248
+
+
+def test_ens_create_rejects_bad_arguments_without_gpu_calls():
+    lib = _lib.load()
+    p = _lib.GLMParams()
+    lib.glm_params_default(C.byref(p))
+    boxes = (_lib.GLMEnsBox * 1)()
+    assert not lib.glm_ens_create(0, boxes, C.byref(p), 8)
+    assert lib.glm_last_error() == 1
+    assert not lib.glm_ens_create(1, boxes, C.byref(p), 2)
+    assert not lib.glm_ens_create(2000, boxes, C.byref(p), 8)
+    boxes[0].nx = boxes[0].ny = boxes[0].nz = 8   # no planes, no boxlen
+    assert not lib.glm_ens_create(1, boxes, C.byref(p), 8)
+    assert b"bad box" in lib.glm_error_string(lib.glm_last_error())
+
+
+def test_random_boxes_follow_the_workload_ranges():
+    from glmmhdturb_amd import random_boxes
+    bx = random_boxes(64, seed=7)
+    assert len(bx) == 64 and {b.N for b in bx} <= {32, 48, 64, 96, 128}
+    assert len({b.seed for b in bx}) == 64
+    for b in bx:
+        beta = 2.0 / b.b0[0] ** 2
+        assert 0.5 <= beta <= 100.0
+        g = min(64, b.N)
+        assert b.turb_gs == g
+        mach = (b.amp / (1e5 * (g / 64) ** 3)) ** (2 / 3)
+        assert 0.3 - 1e-12 <= mach <= 2.0 + 1e-12
+    assert random_boxes(64, seed=7) == bx
+
+
 def test_params_defaults_follow_reference():
     p = Params()
     assert (p.N, p.boxlen, p.gamma, p.courant, p.smallr, p.smallc) == (256, 1.0, 1.01, 0.7, 1e-6, 1e-4)

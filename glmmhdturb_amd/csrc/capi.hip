@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/glmmhd.h"
+#include "adj_launch.hpp"
 #include "aux_launch.hpp"
 #include "ct_launch.hpp"
 #include "host_common.hpp"
@@ -64,12 +65,15 @@ struct glm_ctx {
     void* afield;
     // CT scratch (face fluxes / EMFs of the multi-pass CT path)
     void* ct_scratch;
+    // reverse step: scratch and the forcing-adjoint accumulator
+    void* adj_scratch;
+    double* d_fdot;
 };
 
 static void ctx_free(glm_ctx* c)
 {
     if (!c) return;
-    void* ptrs[] = {c->d_cmax, c->d_parts, c->d_diag, c->d_ou, c->ct_scratch};
+    void* ptrs[] = {c->d_cmax, c->d_parts, c->d_diag, c->d_ou, c->ct_scratch, c->adj_scratch, c->d_fdot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     ou_box_free(c->ou);
@@ -306,6 +310,74 @@ int glm_ou_copy_afield(glm_ctx* ctx, void* dst)
     return GLM_OK;
 }
 long long glm_ou_steps(glm_ctx* ctx) { return ctx ? ctx->nstep : -1; }
+
+int glm_ou_set_afield(glm_ctx* ctx, const void* src)
+{
+    if (!ctx || !ctx->afield || !src) return set_err(GLM_ERR_STATE, "glm_ou_set_afield: no forcing field or null src");
+    size_t bytes = 3 * (size_t)ctx->g * ctx->g * ctx->g * (size_t)ctx->prec;
+    HIPCHK(hipMemcpyAsync(ctx->afield, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return GLM_OK;
+}
+
+int glm_step_adjoint(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
+                     const void* afield)
+{
+    if (!ctx || !u0 || !lam1 || !lam0) return set_err(GLM_ERR_ARG, "glm_step_adjoint: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    if (ctx->prec != 8) return set_err(GLM_ERR_ARG, "glm_step_adjoint: the reverse step is FP64 only");
+    if (p->recon == GLM_RECON_PPM_SPIKE) return set_err(GLM_ERR_ARG, "glm_step_adjoint: PPM_SPIKE has no adjoint");
+    if (p->do_turb && (!afield || ctx->g < 2))
+        return set_err(GLM_ERR_STATE, "glm_step_adjoint: do_turb needs the step's forcing field (and glm_ou_setup)");
+    for (int v = 0; v < 9; ++v)
+        if (!u0[v] || !lam1[v] || !lam0[v] || lam0[v] == lam1[v] || lam0[v] == u0[v])
+            return set_err(GLM_ERR_ARG, "glm_step_adjoint: bad plane pointers (lam0 must not alias lam1 or u0)");
+    if (!ctx->adj_scratch && hipMalloc(&ctx->adj_scratch, adj_scratch_bytes(ctx->n)) != hipSuccess) {
+        ctx->adj_scratch = nullptr;
+        return set_err(GLM_ERR_ALLOC, "glm_step_adjoint: scratch allocation failed");
+    }
+    if (!ctx->d_fdot) {
+        if (hipMalloc(&ctx->d_fdot, sizeof(double)) != hipSuccess) {
+            ctx->d_fdot = nullptr;
+            return set_err(GLM_ERR_ALLOC, "glm_step_adjoint: allocation failed");
+        }
+        HIPCHK(hipMemsetAsync(ctx->d_fdot, 0, sizeof(double), ctx->stream));
+    }
+    AdjArgs a;
+    memset(&a, 0, sizeof a);
+    for (int v = 0; v < 9; ++v) {
+        a.u0[v] = (const double*)u0[v];
+        a.lam1[v] = (const double*)lam1[v];
+        a.lam0[v] = (double*)lam0[v];
+    }
+    a.nx = ctx->nx; a.ny = ctx->ny; a.nz = ctx->nz;
+    a.tgs = ctx->g;
+    a.ph = make_phys<double>(p);
+    a.dtdx = p->dt / p->dx;
+    a.half_dtdx = 0.5 * a.dtdx;
+    a.glm_fac = p->glm_fac; a.smallc = p->smallc; a.turb_min_rho = p->turb_min_rho; a.ramp = p->ramp;
+    a.dt = p->dt; a.dx = p->dx;
+    a.lx = ctx->nx * p->dx; a.ly = ctx->ny * p->dx; a.lz = ctx->nz * p->dx;
+    a.mh_revert = p->mh_revert; a.face_guard = p->face_guard; a.update_floor = p->update_floor;
+    a.do_turb = p->do_turb;
+    a.afield = (const double*)afield;
+    hipError_t he = launch_step_adjoint(a, ctx->adj_scratch, p->predictor, p->riemann, p->recon, p->slope, ctx->d_fdot,
+                                        ctx->stream);
+    if (he == hipErrorInvalidValue) return set_err(GLM_ERR_ARG, "glm_step_adjoint: unsupported scheme");
+    HIPCHK(he);
+    return GLM_OK;
+}
+
+int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out)
+{
+    if (!ctx || !out) return set_err(GLM_ERR_ARG, "glm_adjoint_force_dot: null argument");
+    *out = 0.0;
+    if (!ctx->d_fdot) return GLM_OK;
+    HIPCHK(hipMemcpyAsync(out, ctx->d_fdot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (reset) HIPCHK(hipMemsetAsync(ctx->d_fdot, 0, sizeof(double), ctx->stream));
+    return GLM_OK;
+}
 
 int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
 {

@@ -11,6 +11,8 @@
 #include <stdint.h>
 
 #define GLM_DEV __device__ __forceinline__
+// Point physics is also compiled for the host (CPU checks of the adjoint against dual numbers).
+#define GLM_HD __host__ __device__ __forceinline__
 
 namespace glm {
 
@@ -38,45 +40,54 @@ struct Ptr9 {
 
 // fmax/fmin/fabs semantics (the C restatement's): one v_max/v_min instruction, |x| as a source
 // modifier, copysign as one bit-field insert.
-GLM_DEV double tmax(double a, double b) { return __builtin_fmax(a, b); }
-GLM_DEV float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
-GLM_DEV double tmin(double a, double b) { return __builtin_fmin(a, b); }
-GLM_DEV float tmin(float a, float b) { return __builtin_fminf(a, b); }
-GLM_DEV double tabs(double a) { return __builtin_fabs(a); }
-GLM_DEV float tabs(float a) { return __builtin_fabsf(a); }
-GLM_DEV double tcopysign(double a, double b) { return __builtin_copysign(a, b); }
-GLM_DEV float tcopysign(float a, float b) { return __builtin_copysignf(a, b); }
+GLM_HD double tmax(double a, double b) { return __builtin_fmax(a, b); }
+GLM_HD float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+GLM_HD double tmin(double a, double b) { return __builtin_fmin(a, b); }
+GLM_HD float tmin(float a, float b) { return __builtin_fminf(a, b); }
+GLM_HD double tabs(double a) { return __builtin_fabs(a); }
+GLM_HD float tabs(float a) { return __builtin_fabsf(a); }
+GLM_HD double tcopysign(double a, double b) { return __builtin_copysign(a, b); }
+GLM_HD float tcopysign(float a, float b) { return __builtin_copysignf(a, b); }
 
 // Reciprocal from the hardware estimate refined by Newton steps (two for double, one for float):
 // within about 1 ulp of 1/x for the normal-range operands of this solver, at half the cost of
-// an IEEE divide sequence.  Used where the quotient feeds further arithmetic.
-GLM_DEV double trcp(double x)
+// an IEEE divide sequence.  Used where the quotient feeds further arithmetic.  (The host build,
+// used only by CPU checks, divides.)
+GLM_HD double trcp(double x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rcp(x);
     double e = __builtin_fma(-x, r, 1.0);
     r = __builtin_fma(r, e, r);
     e = __builtin_fma(-x, r, 1.0);
     return __builtin_fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
 }
-GLM_DEV float trcp(float x)
+GLM_HD float trcp(float x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
     float r = __builtin_amdgcn_rcpf(x);
     float e = __builtin_fmaf(-x, r, 1.0f);
     return __builtin_fmaf(r, e, r);
+#else
+    return 1.0f / x;
+#endif
 }
-GLM_DEV double tsqrt(double a) { return __builtin_sqrt(a); }
-GLM_DEV float tsqrt(float a) { return __builtin_sqrtf(a); }
+GLM_HD double tsqrt(double a) { return __builtin_sqrt(a); }
+GLM_HD float tsqrt(float a) { return __builtin_sqrtf(a); }
 
 // Gas pressure (gamma-1)(E - |m|^2/(2 rho) - |B|^2/2), floored (glmmhd_turb.jl:75).  With FP32 state
 // the subtraction is carried out in FP64: in low-beta or high-Mach cells E is dominated by the
 // magnetic or kinetic part and the FP32 difference would lose the thermal part.
-GLM_DEV double gas_pressure(const double c[9], double ir, double gm1, double pfl)
+GLM_HD double gas_pressure(const double c[9], double ir, double gm1, double pfl)
 {
     double ekin = 0.5 * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
     double emag = 0.5 * (c[5] * c[5] + c[6] * c[6] + c[7] * c[7]);
     return __builtin_fmax(gm1 * (c[4] - ekin - emag), pfl);
 }
-GLM_DEV float gas_pressure(const float c[9], float ir, float gm1, float pfl)
+GLM_HD float gas_pressure(const float c[9], float ir, float gm1, float pfl)
 {
     double m1 = c[1], m2 = c[2], m3 = c[3], b1 = c[5], b2 = c[6], b3 = c[7];
     double ekin = 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * (double)ir;
@@ -86,7 +97,7 @@ GLM_DEV float gas_pressure(const float c[9], float ir, float gm1, float pfl)
 
 // glmmhd_turb.jl:68-77 cons2prim
 template <typename T>
-GLM_DEV void cons2prim(const T c[9], T q[9], const Phys<T>& s)
+GLM_HD void cons2prim(const T c[9], T q[9], const Phys<T>& s)
 {
     T rho = tmax(c[0], s.smallr);
     T ir = trcp(rho);
@@ -98,7 +109,7 @@ GLM_DEV void cons2prim(const T c[9], T q[9], const Phys<T>& s)
 
 // glmmhd_turb.jl:79-86 prim2cons
 template <typename T>
-GLM_DEV void prim2cons(const T q[9], T c[9], const Phys<T>& s)
+GLM_HD void prim2cons(const T q[9], T c[9], const Phys<T>& s)
 {
     T ekin = T(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     T emag = T(0.5) * (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]);
@@ -110,7 +121,7 @@ GLM_DEV void prim2cons(const T q[9], T c[9], const Phys<T>& s)
 
 // glmmhd_turb.jl:89-95 fast_speed; b2 = |B|^2 is frame independent.
 template <typename T>
-GLM_DEV T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
+GLM_HD T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
 {
     T ir = trcp(rho);
     T c2 = s.gamma * p * ir;
@@ -143,7 +154,7 @@ GLM_DEV double cfl_speed(const T qt[9], const Phys<T>& st)
 // Ideal-MHD flux of a primitive state in direction D (glmmhd_turb.jl:98-115 evaluated in the
 // rotated frame and rotated back, :161-162).  F[Bn] = F[psi] = 0.
 template <typename T, int D>
-GLM_DEV void flux_dir(const T q[9], T f[9], const Phys<T>& s)
+GLM_HD void flux_dir(const T q[9], T f[9], const Phys<T>& s)
 {
     constexpr int A = D % 3, B = (D + 1) % 3, C = (D + 2) % 3;
     const T rho = q[0], un = q[1 + A], u1 = q[1 + B], u2 = q[1 + C], p = q[4];
@@ -166,11 +177,11 @@ GLM_DEV void flux_dir(const T q[9], T f[9], const Phys<T>& s)
 
 // x-frame flux of a state already in the rotated (x-normal) order (phys_flux_x).
 template <typename T>
-GLM_DEV void flux_x(const T q[9], T f[9], const Phys<T>& s) { flux_dir<T, 0>(q, f, s); }
+GLM_HD void flux_x(const T q[9], T f[9], const Phys<T>& s) { flux_dir<T, 0>(q, f, s); }
 
 // glmmhd_turb.jl:145-150 moncen and :300-301 minmod (each halved by the caller's prim_slope).
 template <typename T>
-GLM_DEV T moncen(T dl, T dr)
+GLM_HD T moncen(T dl, T dr)
 {
     // sgn(dc) * m; at dc = -0 this gives -0 where the reference's sgn gives +0 (m = 0 there),
     // which no later sum can tell apart
@@ -179,13 +190,13 @@ GLM_DEV T moncen(T dl, T dr)
     return (dl * dr <= T(0)) ? T(0) : tcopysign(m, dc);
 }
 template <typename T>
-GLM_DEV T minmod(T a, T b)
+GLM_HD T minmod(T a, T b)
 {
     return (a * b <= T(0)) ? T(0) : (tabs(a) < tabs(b) ? a : b);
 }
 // Half slope (glmmhd_turb.jl:153-155 MonCen / :302-303 minmod).
 template <typename T, int SLOPE>
-GLM_DEV T half_slope(T l, T m, T r)
+GLM_HD T half_slope(T l, T m, T r)
 {
     T dl = m - l, dr = r - m;
     return T(0.5) * (SLOPE == SLOPE_MINMOD ? minmod(dl, dr) : moncen(dl, dr));
@@ -193,7 +204,7 @@ GLM_DEV T half_slope(T l, T m, T r)
 
 // glmmhd_turb.jl:917-923 ppm_mono / ppm_edges (Julia form).
 template <typename T>
-GLM_DEV void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
+GLM_HD void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
 {
     T slope = T(0.25) * (qp - qm);
     T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
@@ -210,7 +221,7 @@ GLM_DEV void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
 
 // spike_mhd.cu:164-172 ppm1 (both edges).
 template <typename T>
-GLM_DEV void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
+GLM_HD void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
 {
     T d = T(0.25) * (qp - qm), cu = (qm - T(2) * q0 + qp) * (T(1) / T(6));
     T qr = q0 + d - cu, ql = q0 - d - cu;
@@ -225,7 +236,7 @@ GLM_DEV void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
 
 // glmmhd_turb.jl:926 spj
 template <typename T>
-GLM_DEV bool spj(T a, T b, T c)
+GLM_HD bool spj(T a, T b, T c)
 {
     T hi = tmax(a, tmax(b, c));
     T lo = tmax(tmin(a, tmin(b, c)), T(1e-20));
@@ -239,7 +250,7 @@ GLM_DEV bool spj(T a, T b, T c)
 
 // glmmhd_turb.jl:182-190 llf_x (exact fast speed) and :311-324 llf_bound.
 template <typename T, bool BOUND>
-GLM_DEV void llf_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
+GLM_HD void llf_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
 {
     T b2L = L[5] * L[5] + L[6] * L[6] + L[7] * L[7];
     T b2R = R[5] * R[5] + R[6] * R[6] + R[7] * R[7];
@@ -261,7 +272,7 @@ GLM_DEV void llf_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
 
 // glmmhd_turb.jl:335-343 hll_x
 template <typename T>
-GLM_DEV void hll_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
+GLM_HD void hll_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
 {
     T bn = L[5];
     T cfL = fast_speed(L[0], L[4], L[5] * L[5] + L[6] * L[6] + L[7] * L[7], bn, s);
@@ -281,7 +292,7 @@ GLM_DEV void hll_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
 // selects (SM >= 0 -> left) has its 9-vectors formed; the scalar star/double-star quantities
 // of both sides are computed as in the reference.
 template <typename T>
-GLM_DEV void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
+GLM_HD void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
 {
     const T bn = L[5];
     const T dL = L[0], uL = L[1], vL = L[2], wL = L[3], pL = L[4], byL = L[6], bzL = L[7];
@@ -369,7 +380,7 @@ GLM_DEV void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
 // Riemann dispatch in direction D: floors (glmmhd_turb.jl:280-281), GLM pair (:175-179,
 // :282-286), solver, rotation back.  RIEM: LLF / LLF-bound / HLL / HLLD (with the LLF switch).
 template <typename T, int RIEM, int D>
-GLM_DEV void riemann(const T Lq[9], const T Rq[9], T out[9], const Phys<T>& s)
+GLM_HD void riemann(const T Lq[9], const T Rq[9], T out[9], const Phys<T>& s)
 {
     constexpr int A = D % 3, B = (D + 1) % 3, C = (D + 2) % 3;
     T L[9] = {tmax(Lq[0], s.smallr), Lq[1 + A], Lq[1 + B], Lq[1 + C], tmax(Lq[4], s.pfl),

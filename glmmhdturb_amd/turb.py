@@ -262,6 +262,28 @@ class Solver:
         torch.cuda.synchronize(self.device)
         return out
 
+    def set_afield(self, field):
+        """Replace the context's forcing field by `field` ((3, g, g, g) tensor, state dtype)."""
+        if self.forcing is None:
+            raise _lib.GLMError("set_afield requires setup_forcing()")
+        check(self.lib.glm_ou_set_afield(self.ctx, C.c_void_p(field.data_ptr())))
+
+    # ---- adjoint ----------------------------------------------------------------------------
+    def step_adjoint(self, u0, lam1, lam0, args: GLMParams, afield=None):
+        """lam0 = (dU1/dU0)^T lam1 for the step `args` taken from u0 (FP64); afield is the forcing
+        field the step used when args.do_turb."""
+        qa, k0 = self._planes(u0)
+        la, k1 = self._planes(lam1)
+        oa, k2 = self._planes(lam0)
+        af = None if afield is None else C.c_void_p(afield.data_ptr())
+        check(self.lib.glm_step_adjoint(self.ctx, qa, la, oa, C.byref(args), af))
+
+    def adjoint_force_dot(self, reset: bool = True) -> float:
+        """Sum over the reverse steps so far of (kick acceleration adjoint) . (kick acceleration)."""
+        out = C.c_double()
+        check(self.lib.glm_adjoint_force_dot(self.ctx, int(reset), C.byref(out)))
+        return out.value
+
     # ---- constrained transport ------------------------------------------------------------
     def ct_step(self, qold, qnew, args: GLMParams, face_slope: int = 0):
         qa, kq = self._planes(qold, 8)

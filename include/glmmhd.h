@@ -152,6 +152,28 @@ int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* c
 int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
 
 /* ===================================================================================== */
+/* 4. Adjoint (no reference counterpart: the reference has no derivatives).      */
+/*    The reverse of one glm_step for the scheme of its glm_params, FP64 contexts only:       */
+/*    lam0 = (dU1/dU0)^T lam1 with the step's dt, ch, psi decay, ramp and forcing field held  */
+/*    fixed.  Supported: full / normal-only predictor, PLM or PPM (Julia form), MonCen or     */
+/*    minmod, LLF / LLF-bound / HLL / HLLD (+ LLF switch), every face guard, update floor,    */
+/*    psi damping and the OU kick.  Checkpointing lives in the caller.                       */
+/* ===================================================================================== */
+/* u0: the 9 planes the forward step started from; lam1: adjoint of its result; lam0: output
+ * (must not alias u0 or lam1); afield: the step's forcing field (3 g^3 doubles, as
+ * glm_ou_copy_afield returns it) when p->do_turb, else NULL.  Also adds the step's
+ * sum over cells of (kick acceleration adjoint) . (kick acceleration) to the context's
+ * accumulator (glm_adjoint_force_dot): the derivative with respect to a factor on the forcing
+ * amplitude.  Asynchronous on the context stream. */
+int glm_step_adjoint(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
+                     const void* afield);
+/* Read (and with reset != 0 clear) the forcing-amplitude accumulator; synchronous. */
+int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out);
+/* Replace the context's forcing field by src (3 g^3 values in the state precision, device),
+ * e.g. to recompute forward steps from a checkpoint with the field they used. */
+int glm_ou_set_afield(glm_ctx* ctx, const void* src);
+
+/* ===================================================================================== */
 /* 3. Ensemble of boxes (no reference counterpart: glmmhd_turb.jl runs one box per run).   */
 /*    Boxes of uneven size advance together.  A sweep is one step of every box, each with   */
 /*    its own dt from its own CFL max and its own OU forcing -- the run_turb loop           */

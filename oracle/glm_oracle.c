@@ -678,12 +678,12 @@ void o_diagnostics(const real* u, int nx, int ny, int nz, double dx, double smal
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t c = cidx(i, j, k, nx, ny);
-                double r = rho[c], a = mx[c], b = my[c], d = mz[c];
+                double r = (double)rho[c], a = (double)mx[c], b = (double)my[c], d = (double)mz[c];
                 double m2 = a * a + b * b + d * d;
                 nadd(&mass, r);
                 nadd(&ke2, m2 / fmax(r, smallr_));
                 nadd(&ek, 0.5 * m2 / r);
-                double Bx = bx[c], By = by[c], Bz = bz[c];
+                double Bx = (double)bx[c], By = (double)by[c], Bz = (double)bz[c];
                 nadd(&em, 0.5 * (Bx * Bx + By * By + Bz * Bz));
                 nadd(&hc, (a * Bx + b * By + d * Bz) / r);
                 nadd(&et, (double)E[c]);

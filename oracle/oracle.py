@@ -1,4 +1,4 @@
-"""ctypes wrapper over the C oracle (oracle/_build/liboracle_{f64,f32}.so).
+"""ctypes wrapper over the C oracle (oracle/_build/liboracle_{f64,f32,dual}.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg, where it is the checker.  The product package never imports it.
@@ -350,6 +350,37 @@ class Oracle:
 def twiddles(g: int):
     m = np.arange(g, dtype=np.float64)
     return np.cos(2.0 * math.pi * m / g), np.sin(2.0 * math.pi * m / g)
+
+
+class DualOracle:
+    """The FP64 GLM step over forward-mode dual numbers (liboracle_dual.so): one step maps
+    (U, dU) to (L(U), J_L(U) dU) with the step's scalars (dt, ch, psi decay, ramp) fixed.  The
+    forcing field may carry a tangent (e.g. afield / amp for the derivative in the amplitude)."""
+
+    def __init__(self):
+        path = os.path.join(BUILD, "liboracle_dual.so")
+        if not os.path.exists(path):
+            build()
+        self.lib = C.CDLL(path)
+        vp, i = C.c_void_p, C.c_int
+        self.lib.od_step.argtypes = [vp, vp, i, i, i, vp, vp, vp]
+        self.lib.od_step.restype = i
+        self.lib.od_step_scratch_pairs.argtypes = [i, i, i]
+        self.lib.od_step_scratch_pairs.restype = C.c_size_t
+
+    def step(self, u, du, sc: "Scheme", afield=None, dafield=None):
+        nv, nz, ny, nx = u.shape
+        x = np.ascontiguousarray(np.stack([np.asarray(u, np.float64), np.asarray(du, np.float64)], axis=-1))
+        out = np.empty_like(x)
+        af = None
+        if afield is not None:
+            da = np.zeros_like(afield, dtype=np.float64) if dafield is None else dafield
+            af = np.ascontiguousarray(np.stack([np.asarray(afield, np.float64), np.asarray(da, np.float64)], axis=-1))
+        scratch = np.empty(2 * self.lib.od_step_scratch_pairs(nx, ny, nz), dtype=np.float64)
+        cs = sc.to_c()
+        self.lib.od_step(_ptr(x), _ptr(out), nx, ny, nz, C.cast(C.pointer(cs), C.c_void_p),
+                         None if af is None else _ptr(af), _ptr(scratch))
+        return out[..., 0].copy(), out[..., 1].copy()
 
 
 class OUForcing:

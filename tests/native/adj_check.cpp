@@ -1,0 +1,247 @@
+// adj_check.cpp -- CPU transpose-identity checks of the hand-written adjoint (adjoint.hpp):
+// for a point function y = f(x), <J v, w> from dual-number forward mode equals <v, J^T w> from
+// the adjoint, for random x, v, w.  Built by tests/native/Makefile; used by the CPU tests.
+#include "../../oracle/dual.hpp"
+#include "../../glmmhdturb_amd/csrc/adjoint.hpp"
+
+#include <cmath>
+#include <cstdint>
+#include <random>
+
+using namespace glm;
+
+namespace {
+
+template <typename T>
+Phys<T> phys(double gamma, double ch, double llf_d, double llf_p)
+{
+    Phys<T> s;
+    s.gamma = T(gamma);
+    s.igm1 = T(1.0 / (gamma - 1.0));
+    s.smallr = T(1e-6);
+    s.pfl = T(1e-6 * 1e-8 / gamma);
+    s.ch = T(ch);
+    s.ch2 = T(ch * ch);
+    s.half_ich = T(0.5 / ch);
+    s.llf_dmin = T(llf_d);
+    s.llf_pmin = T(llf_p);
+    return s;
+}
+
+struct Rng {
+    std::mt19937_64 g;
+    explicit Rng(uint64_t seed) : g(seed) {}
+    double u(double a, double b) { return std::uniform_real_distribution<double>(a, b)(g); }
+    double n() { return std::normal_distribution<double>(0.0, 1.0)(g); }
+};
+
+// a physical primitive state: rho, v, p, B, psi
+void prim(Rng& r, double q[9], double bscale)
+{
+    q[0] = r.u(0.3, 2.0);
+    for (int i = 1; i <= 3; ++i) q[i] = r.u(-1.5, 1.5);
+    q[4] = r.u(0.2, 2.0);
+    for (int i = 5; i <= 7; ++i) q[i] = bscale * r.u(-1.0, 1.0);
+    q[8] = 0.05 * r.u(-1.0, 1.0);
+}
+
+void prim2cons_d(const double q[9], double c[9], double gamma)
+{
+    Phys<double> s = phys<double>(gamma, 1.0, 0, 0);
+    prim2cons(q, c, s);
+}
+
+template <int N>
+void seedvec(Rng& r, double v[N])
+{
+    for (int i = 0; i < N; ++i) v[i] = r.n();
+}
+
+}  // namespace
+
+extern "C" {
+
+// which: 0 cons2prim, 1 prim2cons, 2..4 flux_dir<0..2>, 5 fast_speed, 6 half slope MonCen,
+// 7 half slope minmod, 8 ppm_edges, 9 llf_x, 10 llf_x bound, 11 hll_x, 12 hlld_x,
+// 13..15 riemann<HLLD, 0..2>, 16 riemann<HLL, 1>, 17 riemann<LLF, 2>, 18 riemann<HLLD> with the LLF switch,
+// 19..23 face states (full/normal-only predictor, PPM/PLM, guards), 24..25 full Hancock predictor
+// (MonCen/minmod), 26 update epilogue with the OU kick, 27 the same for a floored cell.
+// bscale sets |B| (0.05: weak field, 2: strong); out = {<Jv, w>, <v, J^T w>}.  Returns 0, or -1
+// for an unknown function.
+int adj_check(int which, uint64_t seed, double bscale, double* out)
+{
+    Rng r(seed);
+    const double gamma = 5.0 / 3.0, ch = 2.5;
+    Phys<double> s = phys<double>(gamma, ch, which == 18 ? 10.0 : 0.0, which == 18 ? 10.0 : 0.0);
+    Phys<Dual> sd = phys<Dual>(gamma, ch, which == 18 ? 10.0 : 0.0, which == 18 ? 10.0 : 0.0);
+    double jvw = 0.0, vjw = 0.0;
+    if (which <= 4) {
+        double q[9], x[9], v[9], w[9], xb[9] = {0};
+        prim(r, q, bscale);
+        if (which == 0) prim2cons_d(q, x, gamma);
+        else for (int i = 0; i < 9; ++i) x[i] = q[i];
+        seedvec<9>(r, v);
+        seedvec<9>(r, w);
+        Dual xd[9], yd[9];
+        for (int i = 0; i < 9; ++i) xd[i] = Dual(x[i], v[i]);
+        if (which == 0) { cons2prim(xd, yd, sd); cons2prim_adj(x, w, xb, s); }
+        else if (which == 1) { prim2cons(xd, yd, sd); prim2cons_adj(x, w, xb, s); }
+        else if (which == 2) { flux_dir<Dual, 0>(xd, yd, sd); flux_dir_adj<double, 0>(x, w, xb, s); }
+        else if (which == 3) { flux_dir<Dual, 1>(xd, yd, sd); flux_dir_adj<double, 1>(x, w, xb, s); }
+        else { flux_dir<Dual, 2>(xd, yd, sd); flux_dir_adj<double, 2>(x, w, xb, s); }
+        for (int i = 0; i < 9; ++i) { jvw += yd[i].d * w[i]; vjw += v[i] * xb[i]; }
+    } else if (which == 5) {
+        double q[9];
+        prim(r, q, bscale);
+        double b2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7], v[4], w = r.n();
+        seedvec<4>(r, v);
+        Dual cf = fast_speed(Dual(q[0], v[0]), Dual(q[4], v[1]), Dual(b2, v[2]), Dual(q[5], v[3]), sd);
+        double xb[4] = {0, 0, 0, 0};
+        fast_speed_adj(q[0], q[4], b2, q[5], w, xb[0], xb[1], xb[2], xb[3], s);
+        jvw = cf.d * w;
+        for (int i = 0; i < 4; ++i) vjw += v[i] * xb[i];
+    } else if (which <= 8) {
+        double x[3] = {r.u(-1, 1), r.u(-1, 1), r.u(-1, 1)}, v[3], w[2], xb[3] = {0, 0, 0};
+        seedvec<3>(r, v);
+        seedvec<2>(r, w);
+        Dual a(x[0], v[0]), b(x[1], v[1]), c(x[2], v[2]);
+        if (which == 6) {
+            Dual h = half_slope<Dual, SLOPE_MONCEN>(a, b, c);
+            half_slope_adj<double, SLOPE_MONCEN>(x[0], x[1], x[2], w[0], xb[0], xb[1], xb[2]);
+            jvw = h.d * w[0];
+        } else if (which == 7) {
+            Dual h = half_slope<Dual, SLOPE_MINMOD>(a, b, c);
+            half_slope_adj<double, SLOPE_MINMOD>(x[0], x[1], x[2], w[0], xb[0], xb[1], xb[2]);
+            jvw = h.d * w[0];
+        } else {
+            Dual lo, hi;
+            ppm_edges(a, b, c, lo, hi);
+            ppm_edges_adj(x[0], x[1], x[2], w[0], w[1], xb[0], xb[1], xb[2]);
+            jvw = lo.d * w[0] + hi.d * w[1];
+        }
+        for (int i = 0; i < 3; ++i) vjw += v[i] * xb[i];
+    } else if (which <= 18) {
+        double L[9], R[9], vL[9], vR[9], w[9], Lb[9] = {0}, Rb[9] = {0};
+        prim(r, L, bscale);
+        prim(r, R, bscale);
+        if (which <= 12) R[5] = L[5];   // solvers take a common normal field (the GLM bn*)
+        seedvec<9>(r, vL);
+        seedvec<9>(r, vR);
+        seedvec<9>(r, w);
+        if (which <= 12) { vR[5] = vL[5]; vL[8] = vR[8] = 0.0; w[5] = w[8] = 0.0; }
+        Dual Ld[9], Rd[9], fd[9];
+        for (int i = 0; i < 9; ++i) { Ld[i] = Dual(L[i], vL[i]); Rd[i] = Dual(R[i], vR[i]); }
+        switch (which) {
+        case 9: llf_x<Dual, false>(Ld, Rd, fd, sd); llf_x_adj<double, false>(L, R, w, Lb, Rb, s); break;
+        case 10: llf_x<Dual, true>(Ld, Rd, fd, sd); llf_x_adj<double, true>(L, R, w, Lb, Rb, s); break;
+        case 11: hll_x(Ld, Rd, fd, sd); hll_x_adj(L, R, w, Lb, Rb, s); break;
+        case 12: hlld_x(Ld, Rd, fd, sd); hlld_x_adj(L, R, w, Lb, Rb, s); break;
+        case 13: riemann<Dual, RIEM_HLLD, 0>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLLD, 0>(L, R, w, Lb, Rb, s); break;
+        case 14: riemann<Dual, RIEM_HLLD, 1>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLLD, 1>(L, R, w, Lb, Rb, s); break;
+        case 15: riemann<Dual, RIEM_HLLD, 2>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLLD, 2>(L, R, w, Lb, Rb, s); break;
+        case 16: riemann<Dual, RIEM_HLL, 1>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLL, 1>(L, R, w, Lb, Rb, s); break;
+        case 17: riemann<Dual, RIEM_LLF, 2>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_LLF, 2>(L, R, w, Lb, Rb, s); break;
+        default: riemann<Dual, RIEM_HLLD, 0>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLLD, 0>(L, R, w, Lb, Rb, s); break;
+        }
+        if (which <= 12) {
+            // the solver's slot-5 input is the shared bn*: fold R's slot-5 adjoint into it
+            Lb[5] += Rb[5];
+            Rb[5] = 0.0;
+            for (int i = 0; i < 9; ++i) jvw += fd[i].d * w[i];
+            for (int i = 0; i < 9; ++i) vjw += vL[i] * Lb[i] + vR[i] * Rb[i];
+        } else {
+            for (int i = 0; i < 9; ++i) jvw += fd[i].d * w[i];
+            for (int i = 0; i < 9; ++i) vjw += vL[i] * Lb[i] + vR[i] * Rb[i];
+        }
+    } else if (which <= 23) {
+        // face states of one cell: inputs l, m0, r, mhf; outputs lo, hi
+        double x[4][9], v[4][9], w[2][9], xb[4][9] = {{0}};
+        for (int k = 0; k < 4; ++k) prim(r, x[k], bscale);
+        for (int k = 0; k < 4; ++k) seedvec<9>(r, v[k]);
+        seedvec<9>(r, w[0]);
+        seedvec<9>(r, w[1]);
+        Dual xd[4][9], lo[9], hi[9];
+        for (int k = 0; k < 4; ++k) for (int i = 0; i < 9; ++i) xd[k][i] = Dual(x[k][i], v[k][i]);
+        FaceOpts o{1, GUARD_IFLO};
+        const double hd = 0.01;
+        switch (which) {
+        case 19:
+            faces_fwd<Dual, PRED_FULL, RECON_PPM, SLOPE_MONCEN, 0>(xd[0], xd[1], xd[2], xd[3], true, Dual(hd), o, sd, lo, hi);
+            faces_adj<double, PRED_FULL, RECON_PPM, SLOPE_MONCEN, 0>(x[0], x[1], x[2], x[3], true, hd, o, s, w[0], w[1], xb[0], xb[1], xb[2], xb[3]);
+            break;
+        case 20:
+            faces_fwd<Dual, PRED_FULL, RECON_PPM, SLOPE_MONCEN, 1>(xd[0], xd[1], xd[2], xd[3], false, Dual(hd), o, sd, lo, hi);
+            faces_adj<double, PRED_FULL, RECON_PPM, SLOPE_MONCEN, 1>(x[0], x[1], x[2], x[3], false, hd, o, s, w[0], w[1], xb[0], xb[1], xb[2], xb[3]);
+            break;
+        case 21:
+            o.face_guard = GUARD_CLAMP;
+            faces_fwd<Dual, PRED_FULL, RECON_PLM, SLOPE_MINMOD, 2>(xd[0], xd[1], xd[2], xd[3], true, Dual(hd), o, sd, lo, hi);
+            faces_adj<double, PRED_FULL, RECON_PLM, SLOPE_MINMOD, 2>(x[0], x[1], x[2], x[3], true, hd, o, s, w[0], w[1], xb[0], xb[1], xb[2], xb[3]);
+            break;
+        case 22:
+            faces_fwd<Dual, PRED_ONED, RECON_PPM, SLOPE_MONCEN, 1>(xd[0], xd[1], xd[2], xd[3], true, Dual(hd), o, sd, lo, hi);
+            faces_adj<double, PRED_ONED, RECON_PPM, SLOPE_MONCEN, 1>(x[0], x[1], x[2], x[3], true, hd, o, s, w[0], w[1], xb[0], xb[1], xb[2], xb[3]);
+            break;
+        default:
+            o.face_guard = GUARD_NONE;
+            faces_fwd<Dual, PRED_ONED, RECON_PLM, SLOPE_MONCEN, 2>(xd[0], xd[1], xd[2], xd[3], true, Dual(hd), o, sd, lo, hi);
+            faces_adj<double, PRED_ONED, RECON_PLM, SLOPE_MONCEN, 2>(x[0], x[1], x[2], x[3], true, hd, o, s, w[0], w[1], xb[0], xb[1], xb[2], xb[3]);
+            break;
+        }
+        for (int i = 0; i < 9; ++i) jvw += lo[i].d * w[0][i] + hi[i].d * w[1][i];
+        for (int k = 0; k < 4; ++k) for (int i = 0; i < 9; ++i) vjw += v[k][i] * xb[k][i];
+    } else if (which <= 25) {
+        // 24: full Hancock predictor from the cell and its six neighbours (MonCen; 25: minmod)
+        double p0[9], nb[6][9], v0[9], vn[6][9], w[9], p0b[9] = {0}, nbb[6][9] = {{0}};
+        prim(r, p0, bscale);
+        for (int k = 0; k < 6; ++k) {
+            for (int i = 0; i < 9; ++i) nb[k][i] = p0[i] * (1.0 + 0.2 * r.u(-1, 1)) + 0.05 * r.u(-1, 1);
+            nb[k][0] = std::fabs(nb[k][0]) + 0.1;
+            nb[k][4] = std::fabs(nb[k][4]) + 0.1;
+        }
+        seedvec<9>(r, v0);
+        for (int k = 0; k < 6; ++k) seedvec<9>(r, vn[k]);
+        seedvec<9>(r, w);
+        Dual pd[9], nd[6][9], mh[9];
+        for (int i = 0; i < 9; ++i) pd[i] = Dual(p0[i], v0[i]);
+        for (int k = 0; k < 6; ++k) for (int i = 0; i < 9; ++i) nd[k][i] = Dual(nb[k][i], vn[k][i]);
+        const double hd = 0.02;
+        if (which == 24) {
+            hancock_fwd<Dual, SLOPE_MONCEN>(pd, nd, mh, Dual(hd), 1, sd);
+            hancock_adj<double, SLOPE_MONCEN>(p0, nb, w, p0b, nbb, hd, 1, s);
+        } else {
+            hancock_fwd<Dual, SLOPE_MINMOD>(pd, nd, mh, Dual(hd), 1, sd);
+            hancock_adj<double, SLOPE_MINMOD>(p0, nb, w, p0b, nbb, hd, 1, s);
+        }
+        for (int i = 0; i < 9; ++i) jvw += mh[i].d * w[i];
+        for (int i = 0; i < 9; ++i) vjw += v0[i] * p0b[i];
+        for (int k = 0; k < 6; ++k) for (int i = 0; i < 9; ++i) vjw += vn[k][i] * nbb[k][i];
+    } else if (which <= 27) {
+        // 26: update epilogue with the OU kick; 27: a floored cell (rho below smallr)
+        double q[9], c[9], a3[3], v[9], va[3], w[9], cb[9] = {0}, a3b[3] = {0};
+        prim(r, q, bscale);
+        prim2cons_d(q, c, gamma);
+        if (which == 27) c[0] = 5e-7;
+        for (int k = 0; k < 3; ++k) a3[k] = r.u(-3, 3);
+        seedvec<9>(r, v);
+        seedvec<3>(r, va);
+        seedvec<9>(r, w);
+        Dual cd[9], ad[3];
+        for (int i = 0; i < 9; ++i) cd[i] = Dual(c[i], v[i]);
+        for (int k = 0; k < 3; ++k) ad[k] = Dual(a3[k], va[k]);
+        UpdOpts o{1, 1};
+        const double fac = 0.97, smallc = 1e-4, tmr = 1e-5, dt = 0.01;
+        epilogue_fwd<Dual>(cd, ad, Dual(fac), Dual(smallc), Dual(tmr), Dual(dt), o, sd);
+        epilogue_adj<double>(c, a3, fac, smallc, tmr, dt, o, s, w, cb, a3b);
+        for (int i = 0; i < 9; ++i) jvw += cd[i].d * w[i];
+        for (int i = 0; i < 9; ++i) vjw += v[i] * cb[i];
+        for (int k = 0; k < 3; ++k) vjw += va[k] * a3b[k];
+    } else {
+        return -1;
+    }
+    out[0] = jvw;
+    out[1] = vjw;
+    return 0;
+}
+
+}  // extern "C"

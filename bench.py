@@ -38,6 +38,9 @@ def parse():
                     help="also run 3 plain device copies of 256 MiB (HBM counter calibration)")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-N", type=int, default=128)
+    ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--ens-boxes", type=int, default=64)
     return ap.parse_args()
 
@@ -300,6 +303,69 @@ def run_ensemble(sweeps, warmup, nbox):
     return out
 
 
+def run_adjoint(N, nsteps, window=50, snapshots=16):
+    """The adjoint workload: N^3 FP64 PPM + full Hancock + HLLD + GLM with OU forcing (shells 1-3,
+    T = 0.5), initial state rho = p = 1, B0 = (0.5, 0, 0) plus a seeded solenoidal velocity
+    (shells 1-4, rms 0.1); J = magnetic energy averaged over the last `window` of `nsteps` steps;
+    gradients with respect to the initial velocity and the forcing amplitude through binomial
+    checkpointing with `snapshots` stored states."""
+    import torch
+    from glmmhdturb_amd import Params
+    from glmmhdturb_amd.adjoint import (Trajectory, initial_state, recompute_steps, solenoidal_velocity,
+                                        velocity_gradient)
+    p = Params(N=N, gamma=5.0 / 3.0, courant=0.3, turb_T=0.5, turb_Ndt=100, comp_frac=0.0, turb_gs=64,
+               turb_kmin=1.0, turb_kmax=3.0, turb_seed=1)
+    tr = Trajectory(p, nsteps, window, AMP, riemann="hlld", recon="ppm", snapshots=snapshots)
+    v0 = torch.from_numpy(solenoidal_velocity(N, seed=5, kmin=1.0, kmax=4.0, rms=0.1)).cuda()
+    u0 = initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma, v0)
+    tr.forward(u0)                      # warm-up (kernel loads, allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = tr.forward(u0)
+    fwd_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lam0, damp = tr.pullback()
+    torch.cuda.synchronize()
+    adj_s = time.perf_counter() - t0
+    g = velocity_gradient(lam0, 1.0, v0)
+    # per-step kernel times on the recorded map
+    st = torch.cuda.current_stream()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    r = tr.records[nsteps // 2]
+    a, b, lam = u0.clone(), tr.s.new_state(), tr.s.new_state()
+    if r.field >= 0:
+        tr.s.set_afield(tr.fields[r.field])
+    e0.record(st)
+    for _ in range(5):
+        tr.s.step(a, b, r.args)
+    e1.record(st)
+    for _ in range(5):
+        tr.s.step_adjoint(a, b, lam, r.args, tr.fields[r.field] if r.field >= 0 else None)
+    e2.record(st)
+    torch.cuda.synchronize()
+    step_ms, rstep_ms = e0.elapsed_time(e1) / 5, e1.elapsed_time(e2) / 5
+    # central-difference spot check on one velocity direction of the recorded map
+    dv = torch.from_numpy(solenoidal_velocity(N, seed=77, kmin=1.0, kmax=4.0, rms=0.1)).cuda()
+    eps = 1e-6
+    jp = tr.replay(initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma, v0 + eps * dv))
+    jm = tr.replay(initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma, v0 - eps * dv))
+    fd_v, ad_v = (jp - jm) / (2 * eps), float((g * dv).sum().item())
+    out = {
+        "workload": f"adjoint of {nsteps} forced steps at {N}^3, FP64 PPM + full Hancock + HLLD + GLM + OU; "
+                    f"J = <E_mag> over the last {window} steps; gradients wrt the initial solenoidal velocity "
+                    f"(shells 1-4, rms 0.1) and the forcing amplitude; binomial checkpointing, {snapshots} snapshots",
+        "adjoint_over_forward_cost": adj_s / fwd_s, "forward_s": fwd_s, "adjoint_s": adj_s, "J": J,
+        "recomputed_forward_steps": tr.forward_steps, "reverse_steps": tr.reverse_steps,
+        "schedule_forward_steps": recompute_steps(nsteps, snapshots),
+        "forward_step_ms": step_ms, "reverse_step_ms": rstep_ms, "reverse_over_forward_step": rstep_ms / step_ms,
+        "dJ_damp": damp,
+        "fd_check": {"velocity_direction": {"fd": fd_v, "adjoint": ad_v, "rel_err": abs(fd_v - ad_v) / abs(fd_v)},
+                     "eps": eps, "map": "recorded dt, ch, psi decay, ramp and forcing realisation"},
+    }
+    tr.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -345,6 +411,8 @@ def main():
                        "diag_rel_err_vs_fp64_from_rest": rel,
                        "diag_rel_err_vs_fp64_gradient_ic_100_steps": fp32_developed(N, 100),
                        "note": "FP32 state and fluxes; FP64 pressure recovery, CFL speed and diagnostics"}
+    if not args.no_adjoint:
+        rec["adjoint"] = run_adjoint(args.adj_N, args.adj_steps)
     if not args.no_ensemble:
         rec["ensemble"] = run_ensemble(args.steps, args.warmup, args.ens_boxes)
     if not args.no_gradient:

@@ -216,12 +216,17 @@ def solenoidal_velocity(N: int, seed: int, kmin: float = 1.0, kmax: float = 4.0,
     return v
 
 
-def initial_state(solver: Solver, rho0: float, p0: float, b0, gamma: float, v):
-    """Uniform state (glmmhd_turb.jl:2164-2172) with velocity field v ((3, nz, ny, nx) tensor)."""
+def initial_state(solver: Solver, rho0: float, p0: float, b0, gamma: float, v, b=None):
+    """Uniform state (glmmhd_turb.jl:2164-2172) with velocity field v and, optionally, a magnetic
+    perturbation b added to B0 ((3, nz, ny, nx) tensors)."""
     u = solver.new_state()
     solver.ic_uniform(u, rho0, p0, b0, gamma)
     u[1:4] = rho0 * v
     u[4] += 0.5 * rho0 * (v ** 2).sum(dim=0)
+    if b is not None:
+        bb = u[5:8].clone()
+        u[5:8] = bb + b
+        u[4] += 0.5 * ((bb + b) ** 2 - bb ** 2).sum(dim=0)
     return u
 
 

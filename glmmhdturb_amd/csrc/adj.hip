@@ -16,8 +16,14 @@ size_t adj_scratch_bytes(size_t n) { return (size_t)ADJ_NPLANES * n * sizeof(dou
 template <int PRED, int RECON, int SLOPE>
 static void launch_cellwise(const AdjArgs& a, dim3 g, hipStream_t st, bool faces)
 {
-    if (faces) hipLaunchKernelGGL((adj_faces_kernel<PRED, RECON, SLOPE>), g, dim3(ADJ_NT), 0, st, a);
-    else hipLaunchKernelGGL((adj_cell_kernel<PRED, RECON, SLOPE>), g, dim3(ADJ_NT), 0, st, a);
+    if (faces) {
+        hipLaunchKernelGGL((adj_faces_kernel<PRED, RECON, SLOPE>), g, dim3(ADJ_NT), 0, st, a);
+        return;
+    }
+    hipLaunchKernelGGL((adj_cell_dir_kernel<PRED, RECON, SLOPE, 0>), g, dim3(ADJ_NT), 0, st, a);
+    hipLaunchKernelGGL((adj_cell_dir_kernel<PRED, RECON, SLOPE, 1>), g, dim3(ADJ_NT), 0, st, a);
+    hipLaunchKernelGGL((adj_cell_dir_kernel<PRED, RECON, SLOPE, 2>), g, dim3(ADJ_NT), 0, st, a);
+    if (PRED == PRED_FULL) hipLaunchKernelGGL((adj_hancock_kernel<SLOPE>), g, dim3(ADJ_NT), 0, st, a);
 }
 
 static bool dispatch_cellwise(const AdjArgs& a, dim3 g, hipStream_t st, bool faces, int pred, int recon, int slope)

@@ -10,8 +10,9 @@
 //             (kick, psi decay, floor): LPRE = adjoint of the pre-epilogue state
 //   4 riemann per lower face: lambda_F = dtdx (LPRE(c) - LPRE(c - e_d)), Riemann adjoint ->
 //             adjoints of the two face states meeting there (each slot has one writer)
-//   5 cell    face-state and predictor adjoints of every cell -> contributions to the
-//             primitive adjoints of the cell and its six neighbours (7 x 9 values per cell)
+//   5 cell    face-state adjoints (one launch per direction) and the predictor adjoint of every
+//             cell -> contributions to the primitive adjoints of the cell and its six
+//             neighbours (7 x 9 values per cell)
 //   6 gather  each cell sums the contributions aimed at it, then c2p adjoint: lam0
 // Every pass writes only its own cell's slots (pass 4 writes the slots of the two face states of
 // one face), so there are no atomics and the result is deterministic.
@@ -233,35 +234,107 @@ __global__ void __launch_bounds__(ADJ_NT) adj_riemann_kernel(AdjArgs a)
 
 // ---- pass 5 ----------------------------------------------------------------------------------
 // CON slot k of a cell: 0 the cell itself, 1 + 2d its -d neighbour, 2 + 2d its +d neighbour.
-template <int PRED, int RECON, int SLOPE>
-__global__ void __launch_bounds__(ADJ_NT) adj_cell_kernel(AdjArgs a)
+// 5d (one launch per direction D): face-state adjoints of the cell's two faces in D -> slots
+// 1 + 2D and 2 + 2D, plus the cell's own slot 0 and the predictor adjoint MHB (the D = 0 launch
+// stores, the others add).  5h (full predictor): the predictor adjoint -> slots 0..6 (added).
+// One direction per launch keeps each thread's live set small.
+GLM_DEV void load_p(const AdjArgs& a, size_t n, size_t c, double* out9) { ld9(plane(a, ADJ_W, n), n, c, out9); }
+
+template <int PRED, int RECON, int SLOPE, int D>
+__global__ void __launch_bounds__(ADJ_NT) adj_cell_dir_kernel(AdjArgs a)
 {
     const size_t n = (size_t)a.nx * a.ny * a.nz;
     const FaceOpts o{a.mh_revert, a.face_guard};
+    double* C = plane(a, ADJ_CON, n);
+    double* MHB = plane(a, ADJ_MHB, n);
     for (size_t c = (size_t)blockIdx.x * ADJ_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * ADJ_NT) {
         const Cell x = cell_of(c, a.nx, a.ny);
-        double p0[9], nb[6][9], mh[9], con[7][9], mhb[9], lob[9], hib[9];
-        load_stencil(a, x, n, p0, nb);
-        const bool up = RECON == RECON_PPM ? ppm_up(p0, nb) : true;
+        double p0[9], l[9], r[9], mh[9], lob[9], hib[9], lb[9], m0b[9], rb[9], mhb[9];
+        load_p(a, n, c, p0);
+        load_p(a, n, nbr(x, D, -1, a.nx, a.ny, a.nz), l);
+        load_p(a, n, nbr(x, D, +1, a.nx, a.ny, a.nz), r);
+        bool up = true;
+        if (RECON == RECON_PPM) {
+            const double* P = plane(a, ADJ_W, n) + 4 * n;
+            double pr[6];
+            for (int d = 0; d < 3; ++d) {
+                pr[2 * d] = P[nbr(x, d, -1, a.nx, a.ny, a.nz)];
+                pr[2 * d + 1] = P[nbr(x, d, +1, a.nx, a.ny, a.nz)];
+            }
+            up = !(spj(pr[0], p0[4], pr[1]) || spj(pr[2], p0[4], pr[3]) || spj(pr[4], p0[4], pr[5]));
+        }
         if (PRED == PRED_FULL) ld9(plane(a, ADJ_MH, n), n, c, mh);
-        for (int k = 0; k < 7; ++k)
-            for (int v = 0; v < 9; ++v) con[k][v] = 0.0;
-        for (int v = 0; v < 9; ++v) mhb[v] = 0.0;
-        ld9(plane(a, ADJ_LFS + fs_slot(0, 0), n), n, c, lob);
-        ld9(plane(a, ADJ_LFS + fs_slot(0, 1), n), n, c, hib);
-        faces_adj<double, PRED, RECON, SLOPE, 0>(nb[0], p0, nb[1], mh, up, a.half_dtdx, o, a.ph, lob, hib, con[1], con[0],
-                                                 con[2], mhb);
-        ld9(plane(a, ADJ_LFS + fs_slot(1, 0), n), n, c, lob);
-        ld9(plane(a, ADJ_LFS + fs_slot(1, 1), n), n, c, hib);
-        faces_adj<double, PRED, RECON, SLOPE, 1>(nb[2], p0, nb[3], mh, up, a.half_dtdx, o, a.ph, lob, hib, con[3], con[0],
-                                                 con[4], mhb);
-        ld9(plane(a, ADJ_LFS + fs_slot(2, 0), n), n, c, lob);
-        ld9(plane(a, ADJ_LFS + fs_slot(2, 1), n), n, c, hib);
-        faces_adj<double, PRED, RECON, SLOPE, 2>(nb[4], p0, nb[5], mh, up, a.half_dtdx, o, a.ph, lob, hib, con[5], con[0],
-                                                 con[6], mhb);
-        if (PRED == PRED_FULL) hancock_adj<double, SLOPE>(p0, nb, mhb, con[0], &con[1], a.half_dtdx, a.mh_revert, a.ph);
-        double* C = plane(a, ADJ_CON, n);
-        for (int k = 0; k < 7; ++k) st9(C + (size_t)k * 9 * n, n, c, con[k]);
+        else for (int v = 0; v < 9; ++v) mh[v] = 0.0;
+        ld9(plane(a, ADJ_LFS + fs_slot(D, 0), n), n, c, lob);
+        ld9(plane(a, ADJ_LFS + fs_slot(D, 1), n), n, c, hib);
+        for (int v = 0; v < 9; ++v) { lb[v] = 0.0; m0b[v] = 0.0; rb[v] = 0.0; mhb[v] = 0.0; }
+        faces_adj<double, PRED, RECON, SLOPE, D>(l, p0, r, mh, up, a.half_dtdx, o, a.ph, lob, hib, lb, m0b, rb, mhb);
+        st9(C + (size_t)(1 + 2 * D) * 9 * n, n, c, lb);
+        st9(C + (size_t)(2 + 2 * D) * 9 * n, n, c, rb);
+        if (D > 0) {
+            double t[9];
+            ld9(C, n, c, t);
+            for (int v = 0; v < 9; ++v) m0b[v] += t[v];
+            if (PRED == PRED_FULL) {
+                ld9(MHB, n, c, t);
+                for (int v = 0; v < 9; ++v) mhb[v] += t[v];
+            }
+        }
+        st9(C, n, c, m0b);
+        if (PRED == PRED_FULL) st9(MHB, n, c, mhb);
+    }
+}
+
+template <int SLOPE, int D>
+GLM_DEV void hancock_dir_add(const AdjArgs& a, size_t n, const Cell& x, const double p0[9], const double accb[9],
+                             double p0b[9])
+{
+    double l[9], r[9], lb[9], rb[9], t[9];
+    const size_t cl = nbr(x, D, -1, a.nx, a.ny, a.nz), cr = nbr(x, D, +1, a.nx, a.ny, a.nz);
+    load_p(a, n, cl, l);
+    load_p(a, n, cr, r);
+    for (int v = 0; v < 9; ++v) { lb[v] = 0.0; rb[v] = 0.0; }
+    hancock_adj_dir<double, SLOPE, D>(p0, l, r, accb, p0b, lb, rb, a.ph);
+    double* C = plane(a, ADJ_CON, n);
+    ld9(C + (size_t)(1 + 2 * D) * 9 * n, n, x.c, t);
+    for (int v = 0; v < 9; ++v) t[v] += lb[v];
+    st9(C + (size_t)(1 + 2 * D) * 9 * n, n, x.c, t);
+    ld9(C + (size_t)(2 + 2 * D) * 9 * n, n, x.c, t);
+    for (int v = 0; v < 9; ++v) t[v] += rb[v];
+    st9(C + (size_t)(2 + 2 * D) * 9 * n, n, x.c, t);
+}
+template <int SLOPE, int D>
+GLM_DEV void hancock_acc_load(const AdjArgs& a, size_t n, const Cell& x, const double p0[9], double acc[9])
+{
+    double l[9], r[9];
+    load_p(a, n, nbr(x, D, -1, a.nx, a.ny, a.nz), l);
+    load_p(a, n, nbr(x, D, +1, a.nx, a.ny, a.nz), r);
+    hancock_acc_dir<double, SLOPE, D>(p0, l, r, acc, a.ph);
+}
+
+template <int SLOPE>
+__global__ void __launch_bounds__(ADJ_NT) adj_hancock_kernel(AdjArgs a)
+{
+    const size_t n = (size_t)a.nx * a.ny * a.nz;
+    double* C = plane(a, ADJ_CON, n);
+    for (size_t c = (size_t)blockIdx.x * ADJ_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * ADJ_NT) {
+        const Cell x = cell_of(c, a.nx, a.ny);
+        double p0[9], acc[9], mhb[9], p0b[9], accb[9];
+        load_p(a, n, c, p0);
+        ld9(plane(a, ADJ_MHB, n), n, c, mhb);
+        for (int v = 0; v < 9; ++v) { acc[v] = 0.0; p0b[v] = 0.0; }
+        hancock_acc_load<SLOPE, 0>(a, n, x, p0, acc);
+        hancock_acc_load<SLOPE, 1>(a, n, x, p0, acc);
+        hancock_acc_load<SLOPE, 2>(a, n, x, p0, acc);
+        if (hancock_core_from_acc(p0, acc, mhb, p0b, accb, a.half_dtdx, a.mh_revert, a.ph)) {
+            hancock_dir_add<SLOPE, 0>(a, n, x, p0, accb, p0b);
+            hancock_dir_add<SLOPE, 1>(a, n, x, p0, accb, p0b);
+            hancock_dir_add<SLOPE, 2>(a, n, x, p0, accb, p0b);
+        }
+        double t[9];
+        ld9(C, n, c, t);
+        for (int v = 0; v < 9; ++v) t[v] += p0b[v];
+        st9(C, n, c, t);
     }
 }
 

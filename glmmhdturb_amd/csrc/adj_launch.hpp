@@ -8,7 +8,8 @@ namespace glm {
 constexpr int ADJ_NT = 256;
 
 // Scratch planes of the reverse step (n doubles each).
-constexpr int ADJ_W = 0, ADJ_MH = 9, ADJ_FS = 18, ADJ_LPRE = 72, ADJ_LFS = 81, ADJ_CON = 135, ADJ_NPLANES = 198;
+constexpr int ADJ_W = 0, ADJ_MH = 9, ADJ_FS = 18, ADJ_LPRE = 72, ADJ_LFS = 81, ADJ_CON = 135, ADJ_MHB = 198,
+              ADJ_NPLANES = 207;
 
 struct AdjArgs {
     const double* u0[9];

@@ -678,54 +678,83 @@ GLM_HD bool hancock_fwd(const T p0[9], const T (*nb)[9], T mh[9], T half_dtdx, i
     }
     return false;
 }
-template <typename T, int SLOPE>
-GLM_HD void hancock_adj(const T p0[9], const T (*nb)[9], const T mhb[9], T p0b[9], T (*nbb)[9], T half_dtdx,
-                        int mh_revert, const Phys<T>& s)
+// The predictor's adjoint in two parts, so that a kernel can take the directions one at a time:
+// the core (conversions and the revert) gives the adjoint of the flux sum, accb; each direction
+// then adds its flux and slope adjoints.  Returns false if the predictor reverted (mh = p0).
+template <typename T, int SLOPE, int D>
+GLM_HD void hancock_acc_dir(const T p0[9], const T l[9], const T r[9], T acc[9], const Phys<T>& s)
 {
-    T acc[9], t[9], f[9], hs[3][9];
-    for (int v = 0; v < 9; ++v) acc[v] = T(0);
-    for (int d = 0; d < 3; ++d) {
-        for (int v = 0; v < 9; ++v) hs[d][v] = half_slope<T, SLOPE>(nb[2 * d][v], p0[v], nb[2 * d + 1][v]);
-        for (int sgn = 0; sgn < 2; ++sgn) {
-            for (int v = 0; v < 9; ++v) t[v] = sgn == 0 ? p0[v] + hs[d][v] : p0[v] - hs[d][v];
-            if (d == 0) flux_dir<T, 0>(t, f, s);
-            else if (d == 1) flux_dir<T, 1>(t, f, s);
-            else flux_dir<T, 2>(t, f, s);
-            for (int v = 0; v < 9; ++v) acc[v] = sgn == 0 ? acc[v] + f[v] : acc[v] - f[v];
+    T t[9], f[9];
+    for (int sgn = 0; sgn < 2; ++sgn) {
+        for (int v = 0; v < 9; ++v) {
+            const T h = half_slope<T, SLOPE>(l[v], p0[v], r[v]);
+            t[v] = sgn == 0 ? p0[v] + h : p0[v] - h;
         }
+        flux_dir<T, D>(t, f, s);
+        for (int v = 0; v < 9; ++v) acc[v] = sgn == 0 ? acc[v] + f[v] : acc[v] - f[v];
     }
+}
+// Core from the flux sum acc of the three directions.
+template <typename T>
+GLM_HD bool hancock_core_from_acc(const T p0[9], const T acc[9], const T mhb[9], T p0b[9], T accb[9], T half_dtdx,
+                                  int mh_revert, const Phys<T>& s)
+{
     T u[9], mh[9];
     prim2cons(p0, u, s);
     for (int v = 0; v < 9; ++v) u[v] = u[v] - half_dtdx * acc[v];
     cons2prim(u, mh, s);
     if (mh_revert && (mh[0] <= s.smallr || mh[4] <= s.pfl)) {
         for (int v = 0; v < 9; ++v) p0b[v] += mhb[v];
-        return;
+        return false;
     }
     T ub[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
     cons2prim_adj(u, mhb, ub, s);
     prim2cons_adj(p0, ub, p0b, s);
-    T accb[9];
     for (int v = 0; v < 9; ++v) accb[v] = -half_dtdx * ub[v];
-    for (int d = 0; d < 3; ++d) {
-        T hsb[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-        for (int sgn = 0; sgn < 2; ++sgn) {
-            T fb_[9], tb[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-            for (int v = 0; v < 9; ++v) {
-                t[v] = sgn == 0 ? p0[v] + hs[d][v] : p0[v] - hs[d][v];
-                fb_[v] = sgn == 0 ? accb[v] : -accb[v];
-            }
-            if (d == 0) flux_dir_adj<T, 0>(t, fb_, tb, s);
-            else if (d == 1) flux_dir_adj<T, 1>(t, fb_, tb, s);
-            else flux_dir_adj<T, 2>(t, fb_, tb, s);
-            for (int v = 0; v < 9; ++v) {
-                p0b[v] += tb[v];
-                hsb[v] += sgn == 0 ? tb[v] : -tb[v];
-            }
+    return true;
+}
+template <typename T, int SLOPE>
+GLM_HD bool hancock_adj_core(const T p0[9], const T (*nb)[9], const T mhb[9], T p0b[9], T accb[9], T half_dtdx,
+                             int mh_revert, const Phys<T>& s)
+{
+    T acc[9];
+    for (int v = 0; v < 9; ++v) acc[v] = T(0);
+    hancock_acc_dir<T, SLOPE, 0>(p0, nb[0], nb[1], acc, s);
+    hancock_acc_dir<T, SLOPE, 1>(p0, nb[2], nb[3], acc, s);
+    hancock_acc_dir<T, SLOPE, 2>(p0, nb[4], nb[5], acc, s);
+    return hancock_core_from_acc(p0, acc, mhb, p0b, accb, half_dtdx, mh_revert, s);
+}
+template <typename T, int SLOPE, int D>
+GLM_HD void hancock_adj_dir(const T p0[9], const T l[9], const T r[9], const T accb[9], T p0b[9], T lb[9], T rb[9],
+                            const Phys<T>& s)
+{
+    T hs[9], hsb[9];
+    for (int v = 0; v < 9; ++v) { hs[v] = half_slope<T, SLOPE>(l[v], p0[v], r[v]); hsb[v] = T(0); }
+    for (int sgn = 0; sgn < 2; ++sgn) {
+        T t[9], fb_[9], tb[9];
+        for (int v = 0; v < 9; ++v) {
+            t[v] = sgn == 0 ? p0[v] + hs[v] : p0[v] - hs[v];
+            fb_[v] = sgn == 0 ? accb[v] : -accb[v];
+            tb[v] = T(0);
         }
-        for (int v = 0; v < 9; ++v)
-            half_slope_adj<T, SLOPE>(nb[2 * d][v], p0[v], nb[2 * d + 1][v], hsb[v], nbb[2 * d][v], p0b[v], nbb[2 * d + 1][v]);
+        flux_dir_adj<T, D>(t, fb_, tb, s);
+        for (int v = 0; v < 9; ++v) {
+            p0b[v] += tb[v];
+            hsb[v] += sgn == 0 ? tb[v] : -tb[v];
+        }
     }
+    for (int v = 0; v < 9; ++v) half_slope_adj<T, SLOPE>(l[v], p0[v], r[v], hsb[v], lb[v], p0b[v], rb[v]);
+}
+
+template <typename T, int SLOPE>
+GLM_HD void hancock_adj(const T p0[9], const T (*nb)[9], const T mhb[9], T p0b[9], T (*nbb)[9], T half_dtdx,
+                        int mh_revert, const Phys<T>& s)
+{
+    T accb[9];
+    if (!hancock_adj_core<T, SLOPE>(p0, nb, mhb, p0b, accb, half_dtdx, mh_revert, s)) return;
+    hancock_adj_dir<T, SLOPE, 0>(p0, nb[0], nb[1], accb, p0b, nbb[0], nbb[1], s);
+    hancock_adj_dir<T, SLOPE, 1>(p0, nb[2], nb[3], accb, p0b, nbb[2], nbb[3], s);
+    hancock_adj_dir<T, SLOPE, 2>(p0, nb[4], nb[5], accb, p0b, nbb[4], nbb[5], s);
 }
 
 // ---- face states of one cell in direction D (march_impl.hpp faces_dir) ----------------------

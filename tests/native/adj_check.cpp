@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdint>
 #include <random>
+#include <algorithm>
+#include <vector>
 
 using namespace glm;
 
@@ -65,7 +67,8 @@ extern "C" {
 // 7 half slope minmod, 8 ppm_edges, 9 llf_x, 10 llf_x bound, 11 hll_x, 12 hlld_x,
 // 13..15 riemann<HLLD, 0..2>, 16 riemann<HLL, 1>, 17 riemann<LLF, 2>, 18 riemann<HLLD> with the LLF switch,
 // 19..23 face states (full/normal-only predictor, PPM/PLM, guards), 24..25 full Hancock predictor
-// (MonCen/minmod), 26 update epilogue with the OU kick, 27 the same for a floored cell.
+// (MonCen/minmod), 26 update epilogue with the OU kick, 27 the same for a floored cell,
+// 28 riemann<LLF bound, 1>.
 // bscale sets |B| (0.05: weak field, 2: strong); out = {<Jv, w>, <v, J^T w>}.  Returns 0, or -1
 // for an unknown function.
 int adj_check(int which, uint64_t seed, double bscale, double* out)
@@ -120,7 +123,7 @@ int adj_check(int which, uint64_t seed, double bscale, double* out)
             jvw = lo.d * w[0] + hi.d * w[1];
         }
         for (int i = 0; i < 3; ++i) vjw += v[i] * xb[i];
-    } else if (which <= 18) {
+    } else if (which <= 18 || which == 28) {
         double L[9], R[9], vL[9], vR[9], w[9], Lb[9] = {0}, Rb[9] = {0};
         prim(r, L, bscale);
         prim(r, R, bscale);
@@ -141,6 +144,7 @@ int adj_check(int which, uint64_t seed, double bscale, double* out)
         case 15: riemann<Dual, RIEM_HLLD, 2>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLLD, 2>(L, R, w, Lb, Rb, s); break;
         case 16: riemann<Dual, RIEM_HLL, 1>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLL, 1>(L, R, w, Lb, Rb, s); break;
         case 17: riemann<Dual, RIEM_LLF, 2>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_LLF, 2>(L, R, w, Lb, Rb, s); break;
+        case 28: riemann<Dual, RIEM_LLFB, 1>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_LLFB, 1>(L, R, w, Lb, Rb, s); break;
         default: riemann<Dual, RIEM_HLLD, 0>(Ld, Rd, fd, sd); riemann_adj<double, RIEM_HLLD, 0>(L, R, w, Lb, Rb, s); break;
         }
         if (which <= 12) {
@@ -245,3 +249,132 @@ int adj_check(int which, uint64_t seed, double bscale, double* out)
 }
 
 }  // extern "C"
+
+// Host restatement of the six reverse passes of adj_impl.hpp (no forcing), for CPU comparison
+// with the dual-number oracle.  Scheme: pred 0/1, recon 0/1 (PLM/PPM), slope 0/1, riem 0..3,
+// guard 0..2; mh_revert, update_floor on.
+template <int PRED, int RECON, int SLOPE, int RIEM>
+static void adj_step_host(const double* u0, const double* lam1, double* lam0, int nx, int ny, int nz, double gamma,
+                          double dt, double dx, double ch, double glm_fac, int guard, double* dump = nullptr)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    Phys<double> s = phys<double>(gamma, ch, 0.0, 0.0);
+    const double dtdx = dt / dx, hd = 0.5 * dtdx;
+    FaceOpts o{1, guard};
+    UpdOpts uo{1, 0};
+    std::vector<double> W(9 * n), MH(9 * n), FS(54 * n), LPRE(9 * n), LFS(54 * n), CON(63 * n);
+    auto idx = [&](int i, int j, int k) {
+        i = (i % nx + nx) % nx; j = (j % ny + ny) % ny; k = (k % nz + nz) % nz;
+        return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+    };
+    auto ld = [&](const double* b, size_t c, double x[9]) { for (int v = 0; v < 9; ++v) x[v] = b[(size_t)v * n + c]; };
+    auto st = [&](double* b, size_t c, const double x[9]) { for (int v = 0; v < 9; ++v) b[(size_t)v * n + c] = x[v]; };
+    for (size_t c = 0; c < n; ++c) { double u[9], w[9]; ld(u0, c, u); cons2prim(u, w, s); st(W.data(), c, w); }
+    auto stencil = [&](int i, int j, int k, double p0[9], double nb[6][9]) {
+        ld(W.data(), idx(i, j, k), p0);
+        ld(W.data(), idx(i - 1, j, k), nb[0]); ld(W.data(), idx(i + 1, j, k), nb[1]);
+        ld(W.data(), idx(i, j - 1, k), nb[2]); ld(W.data(), idx(i, j + 1, k), nb[3]);
+        ld(W.data(), idx(i, j, k - 1), nb[4]); ld(W.data(), idx(i, j, k + 1), nb[5]);
+    };
+    auto upflag = [&](const double p0[9], double nb[6][9]) {
+        return RECON != RECON_PPM || !(spj(nb[0][4], p0[4], nb[1][4]) || spj(nb[2][4], p0[4], nb[3][4]) || spj(nb[4][4], p0[4], nb[5][4]));
+    };
+    for (int k = 0; k < nz; ++k) for (int j = 0; j < ny; ++j) for (int i = 0; i < nx; ++i) {
+        size_t c = idx(i, j, k);
+        double p0[9], nb[6][9], mh[9], lo[9], hi[9];
+        stencil(i, j, k, p0, nb);
+        bool up = upflag(p0, nb);
+        if (PRED == PRED_FULL) { hancock_fwd<double, SLOPE>(p0, nb, mh, hd, 1, s); st(MH.data(), c, mh); }
+        faces_fwd<double, PRED, RECON, SLOPE, 0>(nb[0], p0, nb[1], mh, up, hd, o, s, lo, hi);
+        st(FS.data() + 0 * 9 * n, c, lo); st(FS.data() + 1 * 9 * n, c, hi);
+        faces_fwd<double, PRED, RECON, SLOPE, 1>(nb[2], p0, nb[3], mh, up, hd, o, s, lo, hi);
+        st(FS.data() + 2 * 9 * n, c, lo); st(FS.data() + 3 * 9 * n, c, hi);
+        faces_fwd<double, PRED, RECON, SLOPE, 2>(nb[4], p0, nb[5], mh, up, hd, o, s, lo, hi);
+        st(FS.data() + 4 * 9 * n, c, lo); st(FS.data() + 5 * 9 * n, c, hi);
+    }
+    auto flux = [&](int d, size_t cl, size_t cr, double f[9]) {
+        double L[9], R[9];
+        ld(FS.data() + (size_t)(2 * d + 1) * 9 * n, cl, L);
+        ld(FS.data() + (size_t)(2 * d) * 9 * n, cr, R);
+        if (d == 0) riemann<double, RIEM, 0>(L, R, f, s);
+        else if (d == 1) riemann<double, RIEM, 1>(L, R, f, s);
+        else riemann<double, RIEM, 2>(L, R, f, s);
+    };
+    for (int k = 0; k < nz; ++k) for (int j = 0; j < ny; ++j) for (int i = 0; i < nx; ++i) {
+        size_t c = idx(i, j, k);
+        double acc[9] = {0}, fl[9], fh[9], r[9], lam[9], rb[9] = {0}, a3[3] = {0, 0, 0}, a3b[3] = {0, 0, 0};
+        flux(0, idx(i - 1, j, k), c, fl); flux(0, c, idx(i + 1, j, k), fh); for (int v = 0; v < 9; ++v) acc[v] += fl[v] - fh[v];
+        flux(1, idx(i, j - 1, k), c, fl); flux(1, c, idx(i, j + 1, k), fh); for (int v = 0; v < 9; ++v) acc[v] += fl[v] - fh[v];
+        flux(2, idx(i, j, k - 1), c, fl); flux(2, c, idx(i, j, k + 1), fh); for (int v = 0; v < 9; ++v) acc[v] += fl[v] - fh[v];
+        double u[9]; ld(u0, c, u); ld(lam1, c, lam);
+        for (int v = 0; v < 9; ++v) r[v] = u[v] + dtdx * acc[v];
+        epilogue_adj<double>(r, a3, glm_fac, 1e-4, 1e-5, dt, uo, s, lam, rb, a3b);
+        st(LPRE.data(), c, rb);
+    }
+    for (int k = 0; k < nz; ++k) for (int j = 0; j < ny; ++j) for (int i = 0; i < nx; ++i) {
+        size_t c = idx(i, j, k);
+        size_t cls[3] = {idx(i - 1, j, k), idx(i, j - 1, k), idx(i, j, k - 1)};
+        for (int d = 0; d < 3; ++d) {
+            double L[9], R[9], lr[9], ll[9], ob[9], Lb[9] = {0}, Rb[9] = {0};
+            ld(FS.data() + (size_t)(2 * d + 1) * 9 * n, cls[d], L);
+            ld(FS.data() + (size_t)(2 * d) * 9 * n, c, R);
+            ld(LPRE.data(), c, lr); ld(LPRE.data(), cls[d], ll);
+            for (int v = 0; v < 9; ++v) ob[v] = dtdx * (lr[v] - ll[v]);
+            if (d == 0) riemann_adj<double, RIEM, 0>(L, R, ob, Lb, Rb, s);
+            else if (d == 1) riemann_adj<double, RIEM, 1>(L, R, ob, Lb, Rb, s);
+            else riemann_adj<double, RIEM, 2>(L, R, ob, Lb, Rb, s);
+            st(LFS.data() + (size_t)(2 * d + 1) * 9 * n, cls[d], Lb);
+            st(LFS.data() + (size_t)(2 * d) * 9 * n, c, Rb);
+        }
+    }
+    for (int k = 0; k < nz; ++k) for (int j = 0; j < ny; ++j) for (int i = 0; i < nx; ++i) {
+        size_t c = idx(i, j, k);
+        double p0[9], nb[6][9], mh[9] = {0}, con[7][9] = {{0}}, mhb[9] = {0}, lob[9], hib[9];
+        stencil(i, j, k, p0, nb);
+        bool up = upflag(p0, nb);
+        if (PRED == PRED_FULL) ld(MH.data(), c, mh);
+        ld(LFS.data() + 0 * 9 * n, c, lob); ld(LFS.data() + 1 * 9 * n, c, hib);
+        faces_adj<double, PRED, RECON, SLOPE, 0>(nb[0], p0, nb[1], mh, up, hd, o, s, lob, hib, con[1], con[0], con[2], mhb);
+        ld(LFS.data() + 2 * 9 * n, c, lob); ld(LFS.data() + 3 * 9 * n, c, hib);
+        faces_adj<double, PRED, RECON, SLOPE, 1>(nb[2], p0, nb[3], mh, up, hd, o, s, lob, hib, con[3], con[0], con[4], mhb);
+        ld(LFS.data() + 4 * 9 * n, c, lob); ld(LFS.data() + 5 * 9 * n, c, hib);
+        faces_adj<double, PRED, RECON, SLOPE, 2>(nb[4], p0, nb[5], mh, up, hd, o, s, lob, hib, con[5], con[0], con[6], mhb);
+        if (PRED == PRED_FULL) hancock_adj<double, SLOPE>(p0, nb, mhb, con[0], &con[1], hd, 1, s);
+        for (int q = 0; q < 7; ++q) st(CON.data() + (size_t)q * 9 * n, c, con[q]);
+    }
+    for (int k = 0; k < nz; ++k) for (int j = 0; j < ny; ++j) for (int i = 0; i < nx; ++i) {
+        size_t c = idx(i, j, k);
+        double wb[9], t[9], u[9], lb[9];
+        ld(CON.data(), c, wb);
+        size_t lo3[3] = {idx(i - 1, j, k), idx(i, j - 1, k), idx(i, j, k - 1)};
+        size_t hi3[3] = {idx(i + 1, j, k), idx(i, j + 1, k), idx(i, j, k + 1)};
+        for (int d = 0; d < 3; ++d) {
+            ld(CON.data() + (size_t)(2 + 2 * d) * 9 * n, lo3[d], t); for (int v = 0; v < 9; ++v) wb[v] += t[v];
+            ld(CON.data() + (size_t)(1 + 2 * d) * 9 * n, hi3[d], t); for (int v = 0; v < 9; ++v) wb[v] += t[v];
+        }
+        ld(u0, c, u); ld(LPRE.data(), c, lb);
+        cons2prim_adj(u, wb, lb, s);
+        st(lam0, c, lb);
+    }
+    if (dump) {   // the scratch planes in the layout of adj_impl.hpp (W, MH, FS, LPRE, LFS, CON)
+        double* o = dump;
+        for (const std::vector<double>* v : {&W, &MH, &FS, &LPRE, &LFS, &CON}) {
+            std::copy(v->begin(), v->end(), o);
+            o += v->size();
+        }
+    }
+}
+
+extern "C" int adj_step_cpu(int scheme, const double* u0, const double* lam1, double* lam0, int nx, int ny, int nz,
+                            double gamma, double dt, double dx, double ch, double glm_fac, double* dump)
+{
+    // 0: full PLM MonCen LLF-bound iflo; 1: full PLM MonCen HLL iflo; 2: full PPM MonCen HLLD iflo;
+    // 3: full PLM minmod HLL none
+    switch (scheme) {
+    case 0: adj_step_host<PRED_FULL, RECON_PLM, SLOPE_MONCEN, RIEM_LLFB>(u0, lam1, lam0, nx, ny, nz, gamma, dt, dx, ch, glm_fac, GUARD_IFLO, dump); return 0;
+    case 1: adj_step_host<PRED_FULL, RECON_PLM, SLOPE_MONCEN, RIEM_HLL>(u0, lam1, lam0, nx, ny, nz, gamma, dt, dx, ch, glm_fac, GUARD_IFLO, dump); return 0;
+    case 2: adj_step_host<PRED_FULL, RECON_PPM, SLOPE_MONCEN, RIEM_HLLD>(u0, lam1, lam0, nx, ny, nz, gamma, dt, dx, ch, glm_fac, GUARD_IFLO, dump); return 0;
+    case 3: adj_step_host<PRED_FULL, RECON_PLM, SLOPE_MINMOD, RIEM_HLL>(u0, lam1, lam0, nx, ny, nz, gamma, dt, dx, ch, glm_fac, GUARD_NONE, dump); return 0;
+    default: return -1;
+    }
+}

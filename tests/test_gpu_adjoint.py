@@ -5,7 +5,8 @@ Pins (no reference counterpart exists):
   numbers (liboracle_dual.so, an independent restatement of the step) and J^T w from the GPU
   reverse step, for every supported scheme family and for the forcing amplitude;
 * the whole checkpointed trajectory against central finite differences of the recorded map on
-  random directions of the initial velocity and on the forcing amplitude;
+  random directions of the initial velocity and of the whole initial state (1e-6), and the
+  forcing-amplitude derivative against the dual-number oracle over the same recorded steps;
 * checkpointing: the gradient is bit-identical for any number of snapshots;
 * the trajectory gradient at 32^3 against the dual-number oracle run over the same recorded steps.
 """
@@ -92,41 +93,89 @@ def test_step_forcing_amplitude_vs_dual_oracle():
     s.close()
 
 
-def make_traj(n=16, nsteps=12, window=6, amp=3e3, snapshots=3, seed=1):
-    p = Params(N=n, gamma=5.0 / 3.0, courant=0.3, b0=0.5, turb_T=0.5, turb_Ndt=100, comp_frac=0.0,
+def make_traj(n=16, nsteps=12, window=6, amp=3e3, snapshots=3, seed=1, bpert=0.0, turb_T=0.5, ndt=100):
+    p = Params(N=n, gamma=5.0 / 3.0, courant=0.3, b0=0.5, turb_T=turb_T, turb_Ndt=ndt, comp_frac=0.0,
                turb_gs=n, turb_kmin=1.0, turb_kmax=3.0, turb_seed=seed)
     tr = Trajectory(p, nsteps, window, amp, riemann="hlld", recon="ppm", snapshots=snapshots)
     v0 = dev(solenoidal_velocity(n, seed=seed + 10, kmax=4.0, rms=0.1))
-    u0 = initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma, v0)
-    return tr, v0, u0
+    b = dev(solenoidal_velocity(n, seed=seed + 20, kmax=4.0, rms=bpert)) if bpert > 0 else None
+    u0 = initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma, v0, b)
+    return tr, v0, u0, b
 
 
-def test_trajectory_gradient_vs_central_differences():
-    tr, v0, u0 = make_traj()
+def fd_report(checks):
+    for name, fd, ad in checks:
+        print(f"[adjoint FD] {name}: FD {fd:.12e} adjoint {ad:.12e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
+    for name, fd, ad in checks:
+        assert abs(fd - ad) <= 1e-6 * abs(fd), name
+
+
+def test_trajectory_velocity_gradient_vs_central_differences():
+    """Central differences of the recorded map along 3 velocity directions.  The initial field
+    carries a solenoidal perturbation (rms 0.2) so that J = <E_mag> depends on the velocity at
+    first order (with a uniform B0 the first-order change sum B0 . db vanishes and the derivative
+    drowns in the rounding of J).  Density and pressure stay uniform along these directions."""
+    tr, v0, u0, b = make_traj(bpert=0.2)
     J = tr.forward(u0)
     assert abs(tr.replay(u0) - J) <= 1e-14 * abs(J)
     lam0, damp = tr.pullback()
     g = velocity_gradient(lam0, 1.0, v0)
+    eps = 1e-6
+    checks = []
     for d in range(3):
         dv = dev(solenoidal_velocity(16, seed=40 + d, kmax=4.0, rms=0.1))
-        eps = 1e-5
-        jp = tr.replay(initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), tr.p.gamma, v0 + eps * dv))
-        jm = tr.replay(initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), tr.p.gamma, v0 - eps * dv))
-        fd = (jp - jm) / (2 * eps)
-        ad = float((g * dv).sum().item())
-        print(f"[adjoint FD] direction {d}: FD {fd:.12e} adjoint {ad:.12e}", flush=True)
-        assert abs(fd - ad) <= 1e-6 * abs(fd)
-    eps = 1e-5
-    fd = (tr.replay(u0, 1 + eps) - tr.replay(u0, 1 - eps)) / (2 * eps * tr.amp)
-    print(f"[adjoint FD] amplitude: FD {fd:.12e} adjoint {damp:.12e}", flush=True)
-    assert abs(fd - damp) <= 1e-6 * abs(fd)
+        jp = tr.replay(initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), tr.p.gamma, v0 + eps * dv, b))
+        jm = tr.replay(initial_state(tr.s, 1.0, 1.0, (0.5, 0.0, 0.0), tr.p.gamma, v0 - eps * dv, b))
+        checks.append((f"velocity {d}", (jp - jm) / (2 * eps), float((g * dv).sum().item())))
+    fd_report(checks)
+    tr.close()
+
+
+def test_trajectory_state_and_amplitude_gradient():
+    """Central differences along 2 smooth directions of the whole initial state (all nine
+    variables), and the forcing-amplitude derivative, from a developed state: a uniform density or
+    pressure is a switch point of every limiter (dl = dr = 0), so the trajectory starts from the
+    end of a first run, where no variable is uniform.  Short forcing time (T = 0.05) so that the
+    kick is ramped up within the run."""
+    pre, v0, u0, b = make_traj(bpert=0.2, turb_T=0.05, ndt=10, amp=2e4, nsteps=8, window=1)
+    pre.forward(u0)
+    start = pre.uN.clone()
+    pre.close()
+    tr, _, _, _ = make_traj(turb_T=0.05, ndt=10, amp=2e4, nsteps=12, window=6)
+    tr.forward(start)
+    lam0, damp = tr.pullback()
+    eps = 1e-6
+    checks = []
+    scale = torch.tensor([0.01, 0.01, 0.01, 0.01, 0.01, 0.01, 0.01, 0.01, 0.001], dtype=torch.float64).cuda()
+    for d in range(2):
+        du = torch.stack([dev(solenoidal_velocity(16, seed=60 + 3 * d + k % 3, kmax=3.0, rms=1.0)[k % 3])
+                          for k in range(9)]) * scale[:, None, None, None]
+        fd = (tr.replay(start + eps * du) - tr.replay(start - eps * du)) / (2 * eps)
+        checks.append((f"state {d}", fd, float((lam0 * du).sum().item())))
+    fd_report(checks)
+    # The amplitude derivative is ~1e-6 of J here: a central difference either crosses limiter
+    # switches (large step) or drowns in the rounding of J (small step).  It is checked against
+    # the dual-number oracle run over the same recorded steps with the forcing fields as their
+    # own tangents (d/d(scale) of the fields scaled by scale), which is exact.
+    dorc = DualOracle()
+    u = start.cpu().numpy()
+    du = np.zeros_like(u)
+    dJ = 0.0
+    for n, r in enumerate(tr.records):
+        af = tr.fields[r.field].cpu().numpy() if r.field >= 0 else None
+        u, du = dorc.step(u, du, oracle_scheme(r.args, turb_gs=16), afield=af, dafield=af)
+        if tr.in_window(n + 1):
+            dJ += float((u[5:8] * du[5:8]).sum()) * tr.dV() / tr.window
+    ad = damp * tr.amp
+    print(f"[adjoint amplitude] dual {dJ:.15e} adjoint {ad:.15e}", flush=True)
+    assert abs(dJ - ad) <= 1e-9 * abs(dJ)
     tr.close()
 
 
 def test_checkpointing_is_bit_identical():
     grads = []
     for snaps in (0, 2, 40):
-        tr, v0, u0 = make_traj(nsteps=9, window=4, snapshots=snaps)
+        tr, v0, u0, _ = make_traj(nsteps=9, window=4, snapshots=snaps)
         tr.forward(u0)
         lam0, damp = tr.pullback()
         grads.append((lam0.clone(), damp, tr.forward_steps, tr.reverse_steps))
@@ -140,7 +189,7 @@ def test_checkpointing_is_bit_identical():
 def test_trajectory_gradient_vs_dual_oracle_32():
     """dJ/dv0 . dv at 32^3 over 4 recorded steps: the GPU adjoint against the dual-number oracle
     run over the same recorded steps and forcing fields."""
-    tr, v0, u0 = make_traj(n=32, nsteps=4, window=2, snapshots=1, seed=2)
+    tr, v0, u0, _ = make_traj(n=32, nsteps=4, window=2, snapshots=1, seed=2)
     tr.forward(u0)
     lam0, _ = tr.pullback()
     dv = dev(solenoidal_velocity(32, seed=77, kmax=4.0, rms=0.1))

@@ -413,7 +413,7 @@ int glm_ens_reset(glm_ens* e)
         const glm_ens_box& x = e->box[b];
         EnsBox& h = eb[b];
         memset(&h, 0, sizeof h);
-        h.turb_T = x.do_turb ? x.turb_T : 1.0;
+        h.turb_T = x.turb_T > 0.0 ? x.turb_T : 1.0;   // the ramp min(t/turb_T, 1) is logged for every box
         h.dt_ou = x.do_turb ? x.turb_T / x.ndt : 0.0;   // glmmhd_turb.jl:2148
         h.courant = x.courant;
         h.dx = x.boxlen / x.nx;

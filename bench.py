@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-N", type=int, default=128)
     ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--no-ct", action="store_true")
+    ap.add_argument("--ct-N", type=int, default=512)
+    ap.add_argument("--ct-steps", type=int, default=200)
     ap.add_argument("--ens-boxes", type=int, default=64)
     return ap.parse_args()
 
@@ -303,6 +306,67 @@ def run_ensemble(sweeps, warmup, nbox):
     return out
 
 
+def ct_diag(q, dx):
+    """E_kin, E_mag (cell-centred B from the face fields), cross helicity sum m.B/rho, in FP64."""
+    import torch
+    x = q.double()
+    rho, m = x[0], x[1:4]
+    bcc = torch.stack([0.5 * (x[5] + torch.roll(x[5], -1, 2)), 0.5 * (x[6] + torch.roll(x[6], -1, 1)),
+                       0.5 * (x[7] + torch.roll(x[7], -1, 0))])
+    dV = dx ** 3
+    return {"ekin": float((0.5 * (m ** 2).sum(0) / rho).sum().item()) * dV,
+            "emag": float((0.5 * (bcc ** 2).sum(0)).sum().item()) * dV,
+            "cross_helicity": float(((m * bcc).sum(0) / rho).sum().item()) * dV}
+
+
+def run_ct_precision(N, nsteps):
+    """The precision workload on the constrained-transport path (ct_mhd.jl step!): N^3 periodic,
+    rho = 1, p = 1, face-centred B0 = (0, 0, 0.2) (beta = 50), OU forcing as the turbulence
+    workload, HLLD fluxes with PLM + normal-only Hancock faces; FP32 state against an FP64 run of
+    the same seed after nsteps steps, each precision with its own CFL dt."""
+    import torch
+    from glmmhdturb_amd import Params, Solver, dxof, step_args
+    p = Params(N=N, gamma=5.0 / 3.0, courant=0.3, turb_T=0.5, turb_Ndt=100, comp_frac=0.0, turb_gs=64,
+               turb_kmin=1.0, turb_kmax=3.0, turb_seed=1)
+    out = {}
+    for prec in ("f64", "f32"):
+        s = Solver(p, prec)
+        q1, q2 = s.new_state(8), s.new_state(8)
+        q1[0] = 1.0
+        q1[4] = 1.0 / (p.gamma - 1.0) + 0.5 * 0.2 ** 2
+        q1[7] = 0.2
+        f = s.setup_forcing(p, AMP)
+        t, cmax = 0.0, s.ct_ctot_max(q1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            while t >= f.next_time:
+                f.advance()
+            dt = min(p.courant * dxof(p) / cmax, f.next_time - t)
+            s.ct_step(q1, q2, step_args(p, dt, t, do_turb=True, riemann="hlld", recon="plm", predictor="oned",
+                                        want_cfl=True))
+            q1, q2 = q2, q1
+            t += dt
+            cmax = s.cfl_max()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        d = ct_diag(q1, dxof(p))
+        d["divbmax"] = s.ct_divb_max(q1)
+        d["value"] = N ** 3 * nsteps / wall
+        d["t"] = t
+        out[prec] = d
+        s.close()
+        del q1, q2
+        torch.cuda.empty_cache()
+    rel = {k: abs(out["f32"][k] - out["f64"][k]) / abs(out["f64"][k]) for k in ("ekin", "emag")}
+    rel["cross_helicity_vs_bound"] = abs(out["f32"]["cross_helicity"] - out["f64"]["cross_helicity"]) / math.sqrt(
+        2.0 * out["f64"]["ekin"] * 2.0 * out["f64"]["emag"])
+    return {"workload": f"constrained transport, {N}^3, OU-forced, B0 = (0, 0, 0.2), HLLD + PLM, {nsteps} steps",
+            "fp64": out["f64"], "fp32": out["f32"], "fp32_rel_err_vs_fp64": rel,
+            "divbmax_fp32": out["f32"]["divbmax"], "divbmax_fp64": out["f64"]["divbmax"],
+            "note": "multi-pass CT step (four kernels); FP32 state with FP64 pressure recovery and CFL"}
+
+
 def run_adjoint(N, nsteps, window=50, snapshots=16):
     """The adjoint workload: N^3 FP64 PPM + full Hancock + HLLD + GLM with OU forcing (shells 1-3,
     T = 0.5), initial state rho = p = 1, B0 = (0.5, 0, 0) plus a seeded solenoidal velocity
@@ -411,6 +475,8 @@ def main():
                        "diag_rel_err_vs_fp64_from_rest": rel,
                        "diag_rel_err_vs_fp64_gradient_ic_100_steps": fp32_developed(N, 100),
                        "note": "FP32 state and fluxes; FP64 pressure recovery, CFL speed and diagnostics"}
+    if not args.no_ct:
+        rec["ct_precision"] = run_ct_precision(args.ct_N, args.ct_steps)
     if not args.no_adjoint:
         rec["adjoint"] = run_adjoint(args.adj_N, args.adj_steps)
     if not args.no_ensemble:

@@ -65,7 +65,7 @@ hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int
     hipLaunchKernelGGL(adj_prim_kernel, g, dim3(ADJ_NT), 0, st, a);
     if (!dispatch_cellwise(a, g, st, true, pred, recon, slope)) return hipErrorInvalidValue;
     if (!dispatch_faceswise(a, g, st, true, riem)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adj_fdot_kernel, dim3(1), dim3(64), 0, st, (const double*)a.part, (int)g.x, fdot);
+    hipLaunchKernelGGL(adj_fdot_kernel, dim3(1), dim3(ADJ_NT), 0, st, (const double*)a.part, (int)g.x, fdot);
     dispatch_faceswise(a, g, st, false, riem);
     dispatch_cellwise(a, g, st, false, pred, recon, slope);
     hipLaunchKernelGGL(adj_gather_kernel, g, dim3(ADJ_NT), 0, st, a);

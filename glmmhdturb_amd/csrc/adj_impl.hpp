@@ -363,14 +363,20 @@ __global__ void __launch_bounds__(ADJ_NT) adj_gather_kernel(AdjArgs a)
     }
 }
 
-// Sum of the per-block partials of pass 3 in block order, added to *acc.
-__global__ void adj_fdot_kernel(const double* part, int nparts, double* acc)
+// Sum of the per-block partials of pass 3, added to *acc.  One block: thread t sums partials
+// t, t + 256, ... in order, then a fixed tree; deterministic for a given grid.
+__global__ void __launch_bounds__(ADJ_NT) adj_fdot_kernel(const double* part, int nparts, double* acc)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int b = 0; b < nparts; ++b) s += part[b];
-        *acc += s;
+    __shared__ double red[ADJ_NT];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += ADJ_NT) s += part[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = ADJ_NT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *acc += red[0];
 }
 
 }  // namespace glm

@@ -11,6 +11,17 @@
 
 namespace glm {
 
+// Gas pressure (gamma-1)(E - rho v^2/2 - B^2/2), floored (ct_mhd.jl:26-34).  With FP32 state the
+// subtraction is carried out in FP64, as in the GLM path: E can be dominated by the kinetic or
+// magnetic part.
+template <typename T>
+GLM_DEV T ct_pressure(T E, T rho, T vx, T vy, T vz, T bx, T by, T bz, T gamma, T smallp)
+{
+    const double r = rho, u = vx, v = vy, w = vz, b1 = bx, b2 = by, b3 = bz;
+    const double e = (double)E - 0.5 * r * (u * u + v * v + w * w) - 0.5 * (b1 * b1 + b2 * b2 + b3 * b3);
+    return tmax((T)(((double)gamma - 1.0) * e), smallp);
+}
+
 constexpr int CT_NT = 256;
 
 template <typename T>
@@ -92,9 +103,7 @@ GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const CTArgs<T>& a)
     for (int k = 0; k < 8; ++k) U[k] = U[k] - a.half_dtdx * (FR[k] - FL[k]);
     T rho = tmax(U[0], a.smallr), ir = T(1) / rho;
     T vx = U[1] * ir, vy = U[2] * ir, vz = U[3] * ir;
-    T p = tmax((a.gamma - T(1)) * (U[4] - T(0.5) * rho * (vx * vx + vy * vy + vz * vz) -
-                                   T(0.5) * (U[5] * U[5] + U[6] * U[6] + U[7] * U[7])),
-               a.smallp);
+    T p = ct_pressure(U[4], rho, vx, vy, vz, U[5], U[6], U[7], a.gamma, a.smallp);
     mh[0] = rho; mh[1] = vx; mh[2] = vy; mh[3] = vz; mh[4] = p; mh[5] = U[5]; mh[6] = U[6]; mh[7] = U[7];
 }
 
@@ -174,9 +183,7 @@ __global__ void __launch_bounds__(CT_NT) ct_prim_kernel(CTArgs<T> a)
     T bz = T(0.5) * (a.q[7][c] + a.q[7][cidx(a, i, j, cwrap(k + 1, a.nz))]);
     T rho = tmax(a.q[0][c], a.smallr), ir = T(1) / rho;
     T vx = a.q[1][c] * ir, vy = a.q[2][c] * ir, vz = a.q[3][c] * ir;
-    T p = tmax((a.gamma - T(1)) * (a.q[4][c] - T(0.5) * rho * (vx * vx + vy * vy + vz * vz) -
-                                   T(0.5) * (bx * bx + by * by + bz * bz)),
-               a.smallp);
+    T p = ct_pressure(a.q[4][c], rho, vx, vy, vz, bx, by, bz, a.gamma, a.smallp);
     T q8[8] = {rho, vx, vy, vz, p, bx, by, bz};
 #pragma unroll
     for (int t = 0; t < 8; ++t) a.prim[(size_t)t * n + c] = q8[t];

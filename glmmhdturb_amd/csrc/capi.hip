@@ -68,6 +68,9 @@ struct glm_ctx {
     // reverse step: scratch and the forcing-adjoint accumulator
     void* adj_scratch;
     double* d_fdot;
+    // concurrent march workgroups (CUs x resident workgroups per CU) of the last march variant
+    int slots_key;
+    int slots;
 };
 
 static void ctx_free(glm_ctx* c)
@@ -161,7 +164,18 @@ static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_param
         a.o[v] = (T*)o[v];
     }
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
-    march_tiling(c->nx, c->ny, c->nz, p->zchunk, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
+    const int key = 1 + (((p->predictor * 4 + p->riemann) * 4 + p->recon) * 4 + p->slope);
+    if (c->slots_key != key) {
+        int dev = 0, cus = 0, occ = 0;
+        const void* fn = march_kernel_ptr<T>(p->predictor, p->riemann, p->recon, p->slope);
+        if (!fn) return set_err(GLM_ERR_ARG, "glm_step: unknown scheme");
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, NT, 0));
+        c->slots = cus * (occ > 0 ? occ : 1);
+        c->slots_key = key;
+    }
+    march_tiling(c->nx, c->ny, c->nz, p->zchunk, c->slots, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
     a.ph = make_phys<T>(p);
     double dtdx = p->dt / p->dx;
     a.dtdx = (T)dtdx;

@@ -172,7 +172,7 @@ Layout make_layout(const glm_ens* e, int zreq, int ord)
     L.tile_off.resize(n + 1);
     for (int b = 0; b < n; ++b) {
         const glm_ens_box& x = e->box[b];
-        march_tiling(x.nx, x.ny, x.nz, box_zchunk(x.nz, zreq), L.tiles_x[b], L.tiles_y[b], L.nzc[b], L.zchunk[b]);
+        march_tiling(x.nx, x.ny, x.nz, box_zchunk(x.nz, zreq), 0, L.tiles_x[b], L.tiles_y[b], L.nzc[b], L.zchunk[b]);
     }
     int off = 0;
     for (int p = 0; p < n; ++p) {

@@ -62,20 +62,29 @@ inline const void* ens_kernel_ptr(int pred, int riem, int recon, int slope)
     return nullptr;
 }
 
-// Workgroup tiling of the march for an (nx, ny, nz) grid: tiles and z-chunks.
-inline void march_tiling(int nx, int ny, int nz, int zchunk_req, int& tiles_x, int& tiles_y, int& nzc, int& zchunk)
+// Workgroup tiling of the march for an (nx, ny, nz) grid: tiles and z-chunks.  With no requested
+// z-chunk the number of chunks minimises the modelled run time on `slots` concurrent workgroups
+// (CUs x resident workgroups per CU): rounds = ceil(workgroups / slots), each as long as one chunk
+// plus about one plane of lead-in (the first plane's predictor and the chunk's extra z face).
+// A grid whose workgroup count is a multiple of the slots leaves no partly filled last round.
+inline void march_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int& tiles_x, int& tiles_y, int& nzc,
+                         int& zchunk)
 {
     tiles_x = (nx + OX - 1) / OX;
     tiles_y = (ny + OY - 1) / OY;
     zchunk = zchunk_req > 0 ? zchunk_req : 0;
     if (zchunk == 0) {
-        // enough workgroups to fill 256 CUs a few times over, z-chunks no shorter than 16 planes
-        long cols = (long)tiles_x * tiles_y;
-        int want = (int)((2048 + cols - 1) / cols);
-        int maxc = nz / 16 > 1 ? nz / 16 : 1;
-        int c = want < maxc ? want : maxc;
-        if (c < 1) c = 1;
-        zchunk = (nz + c - 1) / c;
+        const long cols = (long)tiles_x * tiles_y;
+        const long sl = slots > 0 ? slots : 256;
+        const int maxc = nz / 8 > 1 ? nz / 8 : 1;   // chunks of at least 8 planes
+        long best = -1;
+        for (int c = 1; c <= maxc; ++c) {
+            const int zc = (nz + c - 1) / c;
+            const int n = (nz + zc - 1) / zc;      // chunks actually formed
+            if (n != c) continue;
+            const long cost = ((cols * n + sl - 1) / sl) * (long)(zc + 1);
+            if (best < 0 || cost < best) { best = cost; zchunk = zc; }
+        }
     }
     if (zchunk > nz) zchunk = nz;
     nzc = (nz + zchunk - 1) / zchunk;

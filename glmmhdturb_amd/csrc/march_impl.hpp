@@ -122,11 +122,34 @@ GLM_DEV void helper_cell(int h, int& px, int& py)
     else { px = PX - 1; py = 2 + r - 2 * OX - OY; }
 }
 
+// HBM access through the global address space.  Pointers read back from LDS (the cold arguments)
+// have lost their address space, and generic loads are counted together with LDS operations, so
+// every LDS wait in the march would also wait for the prefetch of the next planes.
+template <typename T>
+GLM_DEV T gload(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) T*)p;
+#else
+    return *p;   // host pass of the kernel templates: never executed
+#endif
+}
+
+template <typename T>
+GLM_DEV void gstore_nt(T v, T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)p);
+#else
+    *p = v;
+#endif
+}
+
 template <typename T>
 GLM_DEV void load9(const ColdArgs<T>& a, size_t off, T c[9])
 {
 #pragma unroll
-    for (int v = 0; v < 9; ++v) c[v] = a.q[v][off];
+    for (int v = 0; v < 9; ++v) c[v] = gload(a.q[v] + off);
 }
 
 // Face guard on one face state (glmmhd_turb.jl:618 iflo; spike_mhd.cu:145-147 clamp).
@@ -224,7 +247,7 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
 template <typename T>
 GLM_DEV T force_bilinear(const T* af, int g, int i0w, int i1w, int j0w, int j1w, int kw, T wx, T wy)
 {
-    auto at = [&](int i, int j) { return af[(size_t)i + (size_t)g * ((size_t)j + (size_t)g * kw)]; };
+    auto at = [&](int i, int j) { return gload(af + ((size_t)i + (size_t)g * ((size_t)j + (size_t)g * kw))); };
     return (at(i0w, j0w) * (T(1) - wx) + at(i1w, j0w) * wx) * (T(1) - wy) +
            (at(i0w, j1w) * (T(1) - wx) + at(i1w, j1w) * wx) * wy;
 }
@@ -247,25 +270,34 @@ GLM_DEV void force_accel(ForceCache<T>& fc, const ColdArgs<T>& a, int gx, int gy
 {
     const int g = a.tgs;
     const T gg = T(g);
-    T cx = ((T(gx) + T(0.5)) * a.dx) / a.lx * gg;
-    T cy = ((T(gy) + T(0.5)) * a.dx) / a.ly * gg;
     T cz = ((T(gz) + T(0.5)) * a.dx) / a.lz * gg;
     T fz = floor(cz);
     int k0 = (int)fz;
     T wz = cz - fz;
-    if (k0 != fc.k0) {
+    if (k0 != fc.k0) {   // the column's (x, y) position is needed only when a forcing plane is crossed
+        T cx = ((T(gx) + T(0.5)) * a.dx) / a.lx * gg;
+        T cy = ((T(gy) + T(0.5)) * a.dx) / a.ly * gg;
         T fx = floor(cx), fy = floor(cy);
         int i0 = (int)fx, j0 = (int)fy;
         T wx = cx - fx, wy = cy - fy;
         int i0w = wrapi(i0, g), i1w = wrapi(i0 + 1, g), j0w = wrapi(j0, g), j1w = wrapi(j0 + 1, g);
         const size_t g3 = (size_t)g * g * g;
         const bool shift = k0 == fc.k0 + 1;
+        // all corner loads are issued before the first result is stored
+        T n0[3], n1[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const T* af = a.afield + c * g3;
-            fc.b0(c) = shift ? fc.b1(c) : force_bilinear(af, g, i0w, i1w, j0w, j1w, wrapi(k0, g), wx, wy);
-            fc.b1(c) = force_bilinear(af, g, i0w, i1w, j0w, j1w, wrapi(k0 + 1, g), wx, wy);
+        for (int c = 0; c < 3; ++c)
+            n1[c] = force_bilinear(a.afield + c * g3, g, i0w, i1w, j0w, j1w, wrapi(k0 + 1, g), wx, wy);
+        if (shift) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) n0[c] = fc.b1(c);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                n0[c] = force_bilinear(a.afield + c * g3, g, i0w, i1w, j0w, j1w, wrapi(k0, g), wx, wy);
         }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { fc.b0(c) = n0[c]; fc.b1(c) = n1[c]; }
         fc.k0 = k0;
     }
 #pragma unroll
@@ -479,7 +511,7 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
             if (!(ct <= cmax)) cmax = ct;
             const size_t off = col + plane * zq;
 #pragma unroll
-            for (int v = 0; v < 9; ++v) __builtin_nontemporal_store(u0[v], ca.o[v] + off);
+            for (int v = 0; v < 9; ++v) gstore_nt(u0[v], ca.o[v] + off);
         }
         if (last) break;
 
@@ -532,7 +564,9 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
         }
 
         // ---- advance the column window and publish plane q+1 to LDS ------------------------------
-        __syncthreads();
+        // The last reads of plane q's LDS cells precede the face-state barrier when this plane's
+        // faces were solved; otherwise (the lead-in plane) they are the predictor's.
+        if (!inplane) __syncthreads();
 #pragma unroll
         for (int v = 0; v < 9; ++v) { pm[v] = p0[v]; p0[v] = pp[v]; }
         cons2prim(cn, pp, s);

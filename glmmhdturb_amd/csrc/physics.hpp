@@ -234,13 +234,16 @@ GLM_HD void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
     lo = ql; hi = qr;
 }
 
-// glmmhd_turb.jl:926 spj
+// glmmhd_turb.jl:926 spj: the correctly rounded quotient hi / lo exceeds 2, decided without the
+// division.  If hi > 2 lo, then hi >= 2 lo + ulp(2 lo), so hi / lo - 2 >= 2 ulp(lo) / lo > eps (the
+// half ulp of 2) and the quotient rounds above 2; if hi <= 2 lo it does not exceed 2.  2 lo is
+// exact (lo >= 1e-20) or, past half the largest finite value, infinite with the same answer.
 template <typename T>
 GLM_HD bool spj(T a, T b, T c)
 {
     T hi = tmax(a, tmax(b, c));
     T lo = tmax(tmin(a, tmin(b, c)), T(1e-20));
-    return hi / lo > T(2);
+    return hi > T(2) * lo;
 }
 
 // ----------------------------------------------------------------------------------------

@@ -248,6 +248,44 @@ int adj_check(int which, uint64_t seed, double bscale, double* out)
     return 0;
 }
 
+// The division-free spj against its quotient form max / max(min, 1e-20) > 2 on n triples whose
+// ratio sits within a few ulps of 2, plus random and tiny ones.  prec 8: double, 4: float.
+// Returns the number of disagreements.
+long spj_check(long n, uint64_t seed, int prec)
+{
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> ue(-60.0, 60.0), uu(0.0, 1.0);
+    long bad = 0;
+    for (long i = 0; i < n; ++i) {
+        const double lo = std::pow(10.0, ue(g) * 0.4);
+        const int kind = (int)(g() % 4);
+        auto run = [&](auto zero) {
+            using T = decltype(zero);
+            T l = (T)lo, h;
+            if (kind == 0) {            // 2 lo moved by -8..8 ulps
+                h = T(2) * l;
+                const int k = (int)(g() % 17) - 8;
+                for (int j = 0; j < (k < 0 ? -k : k); ++j) h = std::nextafter(h, k < 0 ? T(0) : T(1e30));
+            } else if (kind == 1) {
+                h = l * (T)(0.5 + 4.0 * uu(g));
+            } else if (kind == 2) {
+                h = (T)(lo * 1e-22);   // below the 1e-20 floor
+            } else {
+                h = T(2) * l * (T)(1.0 + (uu(g) - 0.5) * 1e-14);
+            }
+            T m = l + (h - l) * (T)uu(g);
+            T x[3] = {l, m, h};
+            std::shuffle(x, x + 3, g);
+            T hi = std::max(x[0], std::max(x[1], x[2]));
+            T lw = std::max(std::min(x[0], std::min(x[1], x[2])), T(1e-20));
+            volatile T q = hi / lw;
+            return (bool)(q > T(2)) != glm::spj(x[0], x[1], x[2]);
+        };
+        bad += prec == 4 ? run(0.0f) : run(0.0);
+    }
+    return bad;
+}
+
 }  // extern "C"
 
 // Host restatement of the six reverse passes of adj_impl.hpp (no forcing), for CPU comparison

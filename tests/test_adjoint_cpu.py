@@ -62,3 +62,13 @@ def test_checkpoint_schedule():
             assert recompute_steps(n, s) <= t * n
     assert 1 <= split_point(200, 16) < 200
     assert recompute_steps(200, 16) <= 3 * 200
+
+
+@pytest.mark.parametrize("prec", [8, 4], ids=["f64", "f32"])
+def test_spj_without_division(lib, prec):
+    """The march's strong-pressure-jump test (glmmhd_turb.jl:926) compares max > 2 max(min, 1e-20)
+    instead of dividing; it must agree with the correctly rounded quotient on every input,
+    including ratios within a few ulps of 2 and minima below the 1e-20 floor."""
+    lib.spj_check.argtypes = [C.c_long, C.c_uint64, C.c_int]
+    lib.spj_check.restype = C.c_long
+    assert lib.spj_check(400000, 7, prec) == 0

@@ -10,7 +10,7 @@
 //   * A workgroup of TX x TY = 18 x 14 threads owns a 16 x 12 column tile and a z-chunk (16
 //     doubles = one 128-byte line per row, so row stores are whole lines).  Each thread owns one
 //     (x,y) column of the tile grown by one halo cell and marches z.  Its column's primitives for
-//     planes q-1, q, q+1 live in registers.
+//     planes q-1 and q+1 live in registers; plane q is read back from LDS where it is needed.
 //   * LDS holds the primitives of the current plane q over the tile grown by two cells
 //     (20 x 16): the in-plane neighbours for slopes, PPM edges and the transverse predictor.
 //   * Workgroup ids are renumbered so that each XCD (the hardware deals ids round-robin over the
@@ -21,6 +21,13 @@
 //     exchanged through LDS, and the fluxes are exchanged back the same way.
 //   * The update of plane q-1 happens in the same pass, so face states and fluxes never touch
 //     HBM: one read and one write of the 9 conserved planes per cell and step (plus the halo).
+//     The update base U(q-1) is read again at the update; it was loaded two planes earlier by the
+//     same workgroup and is normally still in L2.
+//   * Budget: two workgroups per CU, i.e. two wavefronts per SIMD, so that one workgroup's LDS
+//     waits, barriers and dependent FP64 chains are covered by the other's issue.  That takes
+//     <= 80 KB of LDS (the three exchange planes and the forcing cache) and <= 256 registers per
+//     lane: the next planes are prefetched only after the plane's Riemann solves, and plane q and
+//     the predicted state are parked in LDS across the phases that do not use them.
 //   * The epilogue also folds the next step's CFL signal speed (glmmhd_turb.jl:1963-1971) into a
 //     per-thread max, reduced per wavefront and merged with one atomic per wavefront.
 #pragma once
@@ -51,7 +58,7 @@ static_assert(NT <= 256 && NT >= NHELP && NO % 64 == 0, "tile shape");
 
 // Minimum resident wavefronts per SIMD requested from the register allocator.
 #ifndef GLM_WAVES_F64
-#define GLM_WAVES_F64 1
+#define GLM_WAVES_F64 2
 #endif
 #ifndef GLM_WAVES_F32
 #define GLM_WAVES_F32 2
@@ -347,8 +354,6 @@ struct MarchLDS {
     T XB[9][NPL];   // +x face states, then x fluxes, at the LDS plane position of the cell
     T YB[9][NPL];
     T FCB[6][NT];   // forcing interpolation cache (ForceCache)
-    T UR[3][9][NO]; // owned lanes' conserved state of the last three planes loaded (plane k in
-                    // slot k mod 3): the update base, read once from HBM
     ColdArgs<T> ca;
 };
 
@@ -372,7 +377,6 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
     T (&P)[9][NPL] = L.P;
     T (&XB)[9][NPL] = L.XB;
     T (&YB)[9][NPL] = L.YB;
-    T (&UR)[3][9][NO] = L.UR;
     const ColdArgs<T>& ca = L.ca;
     const Phys<T>& s = a.ph;
     const int tid = threadIdx.x;
@@ -399,16 +403,12 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
 
 
     // column window: primitives of planes q-1, q, q+1
-    T pm[9], p0[9], pp[9];
+    T pm[9], pp[9];
     {
-        T c[9];
+        T c[9], p0[9];
         load9(ca, col + plane * wrapi(zs - 2, nz), c); cons2prim(c, pm, s);
         load9(ca, col + plane * wrapi(zs - 1, nz), c); cons2prim(c, p0, s);
         load9(ca, col + plane * wrapi(zs, nz), c); cons2prim(c, pp, s);
-        if (tid < NO) {
-#pragma unroll
-            for (int v = 0; v < 9; ++v) UR[zs % 3][v][tid] = c[v];
-        }
 #pragma unroll
         for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
         if (helper) {
@@ -436,6 +436,9 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
         const bool do_upd = owned && q >= zs + 1;
 
         T u0[9];
+        T p0[9];   // this column's cell of plane q, read back from the LDS plane (not carried in registers)
+#pragma unroll
+        for (int v = 0; v < 9; ++v) p0[v] = P[v][lo_own];
 
         // ---- reconstruction of this column's cell in plane q --------------------------------
         bool up = true;   // PPM strong-pressure-jump fallback flag (glmmhd_turb.jl:959)
@@ -500,9 +503,15 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
 #pragma unroll
             for (int v = 0; v < 9; ++v) zprev[v] = zhi[v];
         }
+        if (PRED == PRED_FULL && inplane) {
+            // park the predicted state in this lane's own +x slot (free until its x face state is
+            // stored) across the z solve and the update
+#pragma unroll
+            for (int v = 0; v < 9; ++v) XB[v][lo_own] = mh[v];
+        }
         if (do_upd) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) u0[v] = UR[(q - 1) % 3][v][tid];
+            for (int v = 0; v < 9; ++v) u0[v] = gload(ca.q[v] + col + plane * wrapi(q - 1, nz));
             T acc[9];
 #pragma unroll
             for (int v = 0; v < 9; ++v) acc[v] = pend[v] - fz[v];
@@ -515,14 +524,17 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
         }
         if (last) break;
 
-        // ---- prefetch the next planes ----------------------------------------------------------
-        T cn[9], hr[9];
-        load9(ca, col + plane * wrapi(q + 2, nz), cn);
-        if (helper) load9(ca, hcol + plane * wrapi(q + 1, nz), hr);
 
         // ---- x and y faces of plane q (only for planes this chunk updates) ---------------------
         if (inplane) {
             T xlo[9], ylo[9];
+            __asm__ volatile("" ::: "memory");   // re-read p0 here instead of keeping it live
+#pragma unroll
+            for (int v = 0; v < 9; ++v) p0[v] = P[v][lo_own];
+            if (PRED == PRED_FULL) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) mh[v] = XB[v][lo_own];
+            }
             {
                 T l[9], r[9], hi[9];
 #pragma unroll
@@ -563,19 +575,16 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
             }
         }
 
+        T cn[9], hr[9];
+        load9(ca, col + plane * wrapi(q + 2, nz), cn);
+        if (helper) load9(ca, hcol + plane * wrapi(q + 1, nz), hr);
         // ---- advance the column window and publish plane q+1 to LDS ------------------------------
         // The last reads of plane q's LDS cells precede the face-state barrier when this plane's
         // faces were solved; otherwise (the lead-in plane) they are the predictor's.
         if (!inplane) __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 9; ++v) { pm[v] = p0[v]; p0[v] = pp[v]; }
+        for (int v = 0; v < 9; ++v) { pm[v] = P[v][lo_own]; P[v][lo_own] = pp[v]; }
         cons2prim(cn, pp, s);
-        if (tid < NO) {
-#pragma unroll
-            for (int v = 0; v < 9; ++v) UR[(q + 2) % 3][v][tid] = cn[v];
-        }
-#pragma unroll
-        for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
         if (helper) {
             T h[9];
             cons2prim(hr, h, s);

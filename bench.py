@@ -354,6 +354,22 @@ def run_ct_precision(N, nsteps):
         d["divbmax"] = s.ct_divb_max(q1)
         d["value"] = N ** 3 * nsteps / wall
         d["t"] = t
+        # GPU time of one step at the last dt without the CFL pass: the fused march kernel
+        # against the multi-pass kernel sequence (ct_mhd.jl step!), 5 steps each, HIP events
+        a = step_args(p, dt, t, do_turb=True, riemann="hlld", recon="plm", predictor="oned", want_cfl=False)
+        st = torch.cuda.current_stream()
+        for key, passes in (("march_ms", False), ("passes_ms", True)):
+            s.ct_step(q1, q2, a, passes=passes)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(5):
+                s.ct_step(q1, q2, a, passes=passes)
+                q1, q2 = q2, q1
+            e1.record(st)
+            torch.cuda.synchronize()
+            d[key] = e0.elapsed_time(e1) / 5
+        bpc = 128 if prec == "f64" else 64
+        d["march_gbs_algorithmic"] = bpc * N ** 3 / (d["march_ms"] * 1e-3) / 1e9
         out[prec] = d
         s.close()
         del q1, q2
@@ -364,7 +380,9 @@ def run_ct_precision(N, nsteps):
     return {"workload": f"constrained transport, {N}^3, OU-forced, B0 = (0, 0, 0.2), HLLD + PLM, {nsteps} steps",
             "fp64": out["f64"], "fp32": out["f32"], "fp32_rel_err_vs_fp64": rel,
             "divbmax_fp32": out["f32"]["divbmax"], "divbmax_fp64": out["f64"]["divbmax"],
-            "note": "multi-pass CT step (four kernels); FP32 state with FP64 pressure recovery and CFL"}
+            "note": "fused CT march (one kernel per step, plus the CFL reduction in the run loop); FP32 state "
+                    "with FP64 pressure recovery and CFL; march_ms / passes_ms: GPU ms per step of the fused "
+                    "march and of the multi-pass kernels (ct_mhd.jl step!) at the final dt, without the CFL"}
 
 
 def run_adjoint(N, nsteps, window=50, snapshots=16):

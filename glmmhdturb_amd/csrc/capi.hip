@@ -71,6 +71,8 @@ struct glm_ctx {
     // concurrent march workgroups (CUs x resident workgroups per CU) of the last march variant
     int slots_key;
     int slots;
+    int ct_slots_key;
+    int ct_slots;
 };
 
 static void ctx_free(glm_ctx* c)
@@ -393,9 +395,10 @@ int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out)
     return GLM_OK;
 }
 
-int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
+static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope,
+                        bool fused, const char* who)
 {
-    if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, "glm_ct_step: null argument");
+    if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, who);
     int e = check_params(p);
     if (e) return e;
     if (p->riemann == GLM_RIEMANN_LLF_BOUND || p->recon == GLM_RECON_PPM)
@@ -403,7 +406,7 @@ int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* 
     for (int v = 0; v < 8; ++v)
         if (!q[v] || !o[v] || q[v] == o[v]) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers");
     if (p->do_turb && !ctx->afield) return set_err(GLM_ERR_STATE, "glm_ct_step: do_turb without glm_ou_setup");
-    if (!ctx->ct_scratch) {
+    if (!fused && !ctx->ct_scratch) {
         if (hipMalloc(&ctx->ct_scratch, ct_scratch_bytes(ctx->n, ctx->prec)) != hipSuccess)
             return set_err(GLM_ERR_ALLOC, "glm_ct_step: scratch allocation failed");
     }
@@ -416,10 +419,36 @@ int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* 
     h.update_floor = p->update_floor; h.do_turb = p->do_turb; h.tgs = ctx->g; h.afield = ctx->afield;
     h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2;
     if (p->want_cfl) HIPCHK(launch_clear_u64(ctx->d_cmax + 2, 1, ctx->stream));
-    if (ctx->prec == 8) HIPCHK(launch_ct_step<double>(q, o, h, ctx->ct_scratch, ctx->stream));
-    else HIPCHK(launch_ct_step<float>(q, o, h, ctx->ct_scratch, ctx->stream));
+    if (fused) {
+        const int key = 1000 + p->riemann;
+        if (ctx->ct_slots_key != key) {
+            int dev = 0, cus = 0, occ = 0;
+            const void* fn = ctx->prec == 8 ? ct_march_ptr<double>(p->riemann) : ct_march_ptr<float>(p->riemann);
+            if (!fn) return set_err(GLM_ERR_ARG, "glm_ct_step: unknown Riemann solver");
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, CT_MARCH_THREADS, 0));
+            ctx->ct_slots = cus * (occ > 0 ? occ : 1);
+            ctx->ct_slots_key = key;
+        }
+        if (ctx->prec == 8) HIPCHK(launch_ct_march<double>(q, o, h, ctx->ct_slots, ctx->stream));
+        else HIPCHK(launch_ct_march<float>(q, o, h, ctx->ct_slots, ctx->stream));
+    } else {
+        if (ctx->prec == 8) HIPCHK(launch_ct_step<double>(q, o, h, ctx->ct_scratch, ctx->stream));
+        else HIPCHK(launch_ct_step<float>(q, o, h, ctx->ct_scratch, ctx->stream));
+    }
     if (p->want_cfl) ctx->cfl_slot = 2;
     return GLM_OK;
+}
+
+int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
+{
+    return ct_step_impl(ctx, q, o, p, ct_face_slope, true, "glm_ct_step: null argument");
+}
+
+int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
+{
+    return ct_step_impl(ctx, q, o, p, ct_face_slope, false, "glm_ct_step_passes: null argument");
 }
 
 int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)

@@ -1,5 +1,6 @@
 // ct.hip -- launchers of the constrained-transport kernels (ct_impl.hpp).
 #include "ct_impl.hpp"
+#include "ct_march.hpp"
 #include "ct_launch.hpp"
 
 namespace glm {
@@ -51,6 +52,47 @@ hipError_t launch_ct_step(void* const* q, void* const* o, const CTArgsHost& h, v
 }
 
 template <typename T>
+const void* ct_march_ptr(int riemann)
+{
+    if (riemann == RIEM_HLLD) return (const void*)&ct_march_kernel<T, RIEM_HLLD>;
+    if (riemann == RIEM_HLL) return (const void*)&ct_march_kernel<T, RIEM_HLL>;
+    if (riemann == RIEM_LLF) return (const void*)&ct_march_kernel<T, RIEM_LLF>;
+    return nullptr;
+}
+
+template <typename T>
+hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, int slots, hipStream_t st)
+{
+    CTMArgs<T> a;
+    for (int v = 0; v < 8; ++v) {
+        a.q[v] = (const T*)q[v];
+        a.o[v] = (T*)o[v];
+    }
+    a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
+    ct_march_tiling(h.nx, h.ny, h.nz, slots, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
+    a.gamma = (T)h.gamma; a.igm1 = (T)(1.0 / (h.gamma - 1.0)); a.smallr = (T)h.smallr; a.smallp = (T)h.smallp;
+    double dtdx = h.dt / h.dx;
+    a.dtdx = (T)dtdx; a.half_dtdx = (T)(0.5 * dtdx); a.dt = (T)h.dt; a.dx = (T)h.dx;
+    a.lx = (T)(h.nx * h.dx); a.ly = (T)(h.ny * h.dx); a.lz = (T)(h.nz * h.dx);
+    a.smallc = (T)h.smallc; a.turb_min_rho = (T)h.turb_min_rho; a.ramp = (T)h.ramp;
+    a.face_slope = h.face_slope; a.recon = h.recon; a.update_floor = h.update_floor;
+    a.do_turb = h.do_turb; a.tgs = h.tgs;
+    a.afield = (const T*)h.afield;
+    dim3 g((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), b(CM_NT);
+    if (h.riemann == RIEM_HLLD) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLLD>), g, b, 0, st, a);
+    else if (h.riemann == RIEM_HLL) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLL>), g, b, 0, st, a);
+    else if (h.riemann == RIEM_LLF) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_LLF>), g, b, 0, st, a);
+    else return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !h.want_cfl) return e;
+    Ptr9<const T> o9;
+    for (int v = 0; v < 9; ++v) o9.p[v] = v < 8 ? (const T*)o[v] : nullptr;
+    hipLaunchKernelGGL((ct_ctot_kernel<T>), dim3(2048), dim3(CT_NT), 0, st, o9, h.nx, h.ny, h.nz, (T)h.gamma,
+                       (T)h.smallr, (T)h.smallp, h.cmax);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_ct_ctot(void* const* q, int nx, int ny, int nz, double gamma, double smallr, double smallp,
                           unsigned long long* out, hipStream_t st)
 {
@@ -73,7 +115,7 @@ hipError_t launch_ct_divb(void* const* q, int nx, int ny, int nz, double dx, uns
 template <typename T>
 const void* ct_kernel_ptr()
 {
-    return (const void*)&ct_flux_kernel<T>;
+    return (const void*)&ct_march_kernel<T, RIEM_HLL>;
 }
 
 #define GLM_CT_INST(T)                                                                                          \
@@ -81,7 +123,9 @@ const void* ct_kernel_ptr()
     template hipError_t launch_ct_ctot<T>(void* const*, int, int, int, double, double, double,                  \
                                           unsigned long long*, hipStream_t);                                    \
     template hipError_t launch_ct_divb<T>(void* const*, int, int, int, double, unsigned long long*, hipStream_t); \
-    template const void* ct_kernel_ptr<T>();
+    template const void* ct_kernel_ptr<T>();                                                                    \
+    template const void* ct_march_ptr<T>(int);                                                                  \
+    template hipError_t launch_ct_march<T>(void* const*, void* const*, const CTArgsHost&, int, hipStream_t);
 GLM_CT_INST(double)
 GLM_CT_INST(float)
 #undef GLM_CT_INST

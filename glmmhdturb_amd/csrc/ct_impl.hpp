@@ -87,9 +87,10 @@ GLM_DEV T ct_cf(const T q[8], T bn, T gamma)
     return tsqrt(dd + tsqrt(tmax(dd * dd - c2 * bn * bn * ir, T(0))));
 }
 
-// ct_mhd.jl:74-82 / spike_ctm.cu:170-184 hanc1d with offset hs (face = c -/+ hs)
-template <typename T, int D>
-GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const CTArgs<T>& a)
+// ct_mhd.jl:74-82 / spike_ctm.cu:170-184 hanc1d with offset hs (face = c -/+ hs).  A is any
+// argument block with igm1, half_dtdx, smallr, gamma and smallp (the multi-pass and the march).
+template <typename T, int D, typename A>
+GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const A& a)
 {
     T t[8], FL[8], FR[8], U[8];
 #pragma unroll
@@ -107,48 +108,57 @@ GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const CTArgs<T>& a)
     mh[0] = rho; mh[1] = vx; mh[2] = vy; mh[3] = vz; mh[4] = p; mh[5] = U[5]; mh[6] = U[6]; mh[7] = U[7];
 }
 
-// Flux at the lower-D face from the stencil m2, m1, c0, p1 (ct_mhd.jl:99-122 / spike_ctm.cu:223-245).
-template <typename T, int D>
-GLM_DEV void ct_face_flux(const T m2[8], const T m1[8], const T c0[8], const T p1[8], T bn, const CTArgs<T>& a, T f[8])
+// The two face states of cell c in direction D from its neighbours l, r (ct_mhd.jl:99-122 /
+// spike_ctm.cu:223-245, taken per cell): half slope fac * MonCen, the normal-only Hancock half
+// step, then PLM (mh -/+ hs) or the spike PPM edges (edge + mh - c), floored.  lo is the state on
+// the cell's lower D face, hi on its upper one; a face flux pairs hi of the lower cell with lo of
+// the upper cell.
+template <typename T, int D, typename A>
+GLM_DEV void ct_cell_states(const T l[8], const T c[8], const T r[8], const A& a, T lo[8], T hi[8])
 {
-    T hsL[8], hsR[8], mhL[8], mhR[8], L[8], R[8];
+    T hs[8], mh[8];
     const T fac = a.face_slope == 1 ? T(0.25) : T(0.5);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        hsL[k] = fac * moncen(m1[k] - m2[k], c0[k] - m1[k]);
-        hsR[k] = fac * moncen(c0[k] - m1[k], p1[k] - c0[k]);
-    }
-    ct_hanc1d<T, D>(m1, hsL, mhL, a);
-    ct_hanc1d<T, D>(c0, hsR, mhR, a);
+    for (int k = 0; k < 8; ++k) hs[k] = fac * moncen(c[k] - l[k], r[k] - c[k]);
+    ct_hanc1d<T, D>(c, hs, mh, a);
     if (a.recon == RECON_PPM_SPIKE) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            T lo, hi;
-            ppm_edges_spike(m2[k], m1[k], c0[k], lo, hi);
-            L[k] = hi + mhL[k] - m1[k];
-            ppm_edges_spike(m1[k], c0[k], p1[k], lo, hi);
-            R[k] = lo + mhR[k] - c0[k];
+            T el, eh;
+            ppm_edges_spike(l[k], c[k], r[k], el, eh);
+            hi[k] = eh + mh[k] - c[k];
+            lo[k] = el + mh[k] - c[k];
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) { L[k] = mhL[k] + hsL[k]; R[k] = mhR[k] - hsR[k]; }
+        for (int k = 0; k < 8; ++k) { hi[k] = mh[k] + hs[k]; lo[k] = mh[k] - hs[k]; }
     }
-    L[0] = tmax(L[0], a.smallr); L[4] = tmax(L[4], a.smallp);
-    R[0] = tmax(R[0], a.smallr); R[4] = tmax(R[4], a.smallp);
+    hi[0] = tmax(hi[0], a.smallr); hi[4] = tmax(hi[4], a.smallp);
+    lo[0] = tmax(lo[0], a.smallr); lo[4] = tmax(lo[4], a.smallp);
+}
+
+// Flux through a D face from the floored face states Lq (lower cell's hi) and Rq (upper cell's
+// lo) with the face-normal field bn (ct_mhd.jl:114-122, spike_ctm.cu:235-245): F[Bn] = 0.
+template <typename T, int RIEM, int D, typename A>
+GLM_DEV void ct_riemann(const T Lq[8], const T Rq[8], T bn, const A& a, T f[8])
+{
+    T L[8], R[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { L[k] = Lq[k]; R[k] = Rq[k]; }
     L[5 + D] = bn; R[5 + D] = bn;
-    if (a.riemann == RIEM_HLLD) {
+    if (RIEM == RIEM_HLLD) {
         // rotate D to x, Miyoshi-Kusano with the face-normal B and F[Bn] = 0, rotate back
-        constexpr int A = D % 3, B = (D + 1) % 3, C = (D + 2) % 3;
-        T Lr[9] = {L[0], L[1 + A], L[1 + B], L[1 + C], L[4], L[5 + A], L[5 + B], L[5 + C], T(0)};
-        T Rr[9] = {R[0], R[1 + A], R[1 + B], R[1 + C], R[4], R[5 + A], R[5 + B], R[5 + C], T(0)};
+        constexpr int A_ = D % 3, B_ = (D + 1) % 3, C_ = (D + 2) % 3;
+        T Lr[9] = {L[0], L[1 + A_], L[1 + B_], L[1 + C_], L[4], L[5 + A_], L[5 + B_], L[5 + C_], T(0)};
+        T Rr[9] = {R[0], R[1 + A_], R[1 + B_], R[1 + C_], R[4], R[5 + A_], R[5 + B_], R[5 + C_], T(0)};
         Phys<T> s;
         s.gamma = a.gamma; s.igm1 = a.igm1; s.smallr = a.smallr; s.pfl = a.smallp;
         s.ch = T(1); s.ch2 = T(1); s.half_ich = T(0.5); s.llf_dmin = T(0); s.llf_pmin = T(0);
         T fx[9];
         hlld_x(Lr, Rr, fx, s);
         f[0] = fx[0]; f[4] = fx[4];
-        f[1 + A] = fx[1]; f[1 + B] = fx[2]; f[1 + C] = fx[3];
-        f[5 + A] = T(0); f[5 + B] = fx[6]; f[5 + C] = fx[7];
+        f[1 + A_] = fx[1]; f[1 + B_] = fx[2]; f[1 + C_] = fx[3];
+        f[5 + A_] = T(0); f[5 + B_] = fx[6]; f[5 + C_] = fx[7];
         return;
     }
     T FL[8], FR[8], UL[8], UR[8];
@@ -158,7 +168,7 @@ GLM_DEV void ct_face_flux(const T m2[8], const T m1[8], const T c0[8], const T p
     ct_toC(R, UR, a.igm1);
     const T uL = L[1 + D], uR = R[1 + D];
     const T cL = ct_cf(L, L[5 + D], a.gamma), cR = ct_cf(R, R[5 + D], a.gamma);
-    if (a.riemann == RIEM_LLF) {
+    if (RIEM == RIEM_LLF) {
         T smax = tmax(tabs(uL) + cL, tabs(uR) + cR);
 #pragma unroll
         for (int k = 0; k < 8; ++k) f[k] = T(0.5) * (FL[k] + FR[k]) - T(0.5) * smax * (UR[k] - UL[k]);
@@ -168,6 +178,18 @@ GLM_DEV void ct_face_flux(const T m2[8], const T m1[8], const T c0[8], const T p
 #pragma unroll
         for (int k = 0; k < 8; ++k) f[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * ih;
     }
+}
+
+// Flux at the lower-D face from the stencil m2, m1, c0, p1 (ct_mhd.jl:99-122 / spike_ctm.cu:223-245).
+template <typename T, int D>
+GLM_DEV void ct_face_flux(const T m2[8], const T m1[8], const T c0[8], const T p1[8], T bn, const CTArgs<T>& a, T f[8])
+{
+    T lo[8], L[8], R[8], hi[8];
+    ct_cell_states<T, D>(m2, m1, c0, a, lo, L);
+    ct_cell_states<T, D>(m1, c0, p1, a, R, hi);
+    if (a.riemann == RIEM_HLLD) ct_riemann<T, RIEM_HLLD, D>(L, R, bn, a, f);
+    else if (a.riemann == RIEM_LLF) ct_riemann<T, RIEM_LLF, D>(L, R, bn, a, f);
+    else ct_riemann<T, RIEM_HLL, D>(L, R, bn, a, f);
 }
 
 // Pass 1: cell-centred B and primitives (ct_mhd.jl:85-96 bcc, :26-34 cellprim)

@@ -14,6 +14,7 @@ struct CTArgsHost {
 };
 
 size_t ct_scratch_bytes(size_t n, int prec);
+constexpr int CT_MARCH_THREADS = 252;   // lanes of a CT march workgroup (ct_march.hpp CM_NT)
 template <typename T>
 hipError_t launch_ct_step(void* const* q, void* const* o, const CTArgsHost& h, void* scratch, hipStream_t st);
 template <typename T>
@@ -23,5 +24,10 @@ template <typename T>
 hipError_t launch_ct_divb(void* const* q, int nx, int ny, int nz, double dx, unsigned long long* out, hipStream_t st);
 template <typename T>
 const void* ct_kernel_ptr();
+// Fused CT step (one z-march kernel) and its kernel for a Riemann solver (occupancy queries).
+template <typename T>
+hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, int slots, hipStream_t st);
+template <typename T>
+const void* ct_march_ptr(int riemann);
 
 }  // namespace glm

@@ -1,0 +1,347 @@
+// ct_march.hpp -- fused constrained-transport step as one z-march kernel (spike_ctm.cu:250-298
+// ct_march; physics of ct_mhd.jl:20-184).
+//
+// What it computes is the multi-pass CT step of ct_impl.hpp in one pass over HBM: cell-centred B
+// and primitives, the lower-face fluxes with the face-normal field (each face solved once), the
+// Balsara-Spicer edge EMFs, the hydro update and the face-B curl update, with the optional floor
+// and OU kick.  The point functions are the same ones (ct_cell_states, ct_riemann).
+//
+// Layout on the GPU (the GLM march's, adapted to CT):
+//   * A workgroup of 252 lanes owns a 16 x 12 column tile and a z-chunk.  Lanes 0..191 own the
+//     tile row by row; lanes 192..251 are the one-cell ring around it, corners included (the
+//     EMF at an edge of the tile needs fluxes of the ring's corner cells).  Every lane builds the
+//     x, y and z face states of its cell; the LDS plane holds the primitives of plane q over the
+//     tile grown by two cells (the in-plane neighbours of the ring).
+//   * Per plane q: (1) z face q-1/2 from the +z state carried from plane q-1; (2) +x/+y face
+//     states through LDS, x and y faces solved once; (3) fluxes back through LDS; (4) the edge
+//     EMFs of plane q (ex, ey need the y/x fluxes of plane q-1, carried per column); (5) the
+//     update of plane q-1, whose curl needs ex, ey of plane q and the EMFs of plane q-1 (kept in
+//     a two-plane LDS ring); (6) the prefetch and publish of plane q+1.
+//   * HBM traffic: one read and one write of the 8 planes per cell and step, plus the halo and
+//     the update base read again from L2.  The CFL of the result is a separate reduction.
+#pragma once
+#include "ct_impl.hpp"
+#include "ct_launch.hpp"
+#include "gpu_util.hpp"
+
+namespace glm {
+
+constexpr int CM_OX = 16, CM_OY = 12;
+constexpr int CM_NO = CM_OX * CM_OY;                           // owned lanes
+constexpr int CM_NH = 2 * (CM_OX + 2) + 2 * CM_OY;             // one-cell ring with corners
+constexpr int CM_NT = CM_NO + CM_NH;                           // 252
+constexpr int CM_PX = CM_OX + 4, CM_PY = CM_OY + 4;            // LDS plane (two-cell ring)
+constexpr int CM_NPL = CM_PX * CM_PY;
+constexpr int CM_NHELP = 2 * (CM_OX + 2) + 2 * (CM_OY + 2);    // two-cell ring without its corners
+static_assert(CM_NT <= 256 && CM_NT >= CM_NHELP && CM_NO % 64 == 0, "CT tile shape");
+static_assert(CM_NT == CT_MARCH_THREADS, "launch width of the CT march");
+
+#ifndef GLM_CT_WAVES_F64
+#define GLM_CT_WAVES_F64 2
+#endif
+#ifndef GLM_CT_WAVES_F32
+#define GLM_CT_WAVES_F32 2
+#endif
+#define GLM_CT_WAVES_PER_EU(T) (sizeof(T) == 8 ? GLM_CT_WAVES_F64 : GLM_CT_WAVES_F32)
+
+template <typename T>
+struct CTMArgs {
+    const T* q[8];
+    T* o[8];
+    int nx, ny, nz;
+    int tiles_x, tiles_y, nzc, zchunk;
+    T gamma, igm1, smallr, smallp, dtdx, half_dtdx, dt, dx, lx, ly, lz, smallc, turb_min_rho, ramp;
+    int face_slope, recon, update_floor, do_turb, tgs;
+    const T* afield;
+};
+
+// Lane t's cell (tx, ty) in the tile grown by one cell (tx in 0..OX+1, ty in 0..OY+1).
+GLM_DEV void cm_lane_cell(int t, int& tx, int& ty)
+{
+    if (t < CM_NO) { tx = 1 + t % CM_OX; ty = 1 + t / CM_OX; return; }
+    const int h = t - CM_NO;
+    if (h < CM_OX + 2) { tx = h; ty = 0; }
+    else if (h < 2 * (CM_OX + 2)) { tx = h - (CM_OX + 2); ty = CM_OY + 1; }
+    else if (h < 2 * (CM_OX + 2) + CM_OY) { tx = 0; ty = 1 + h - 2 * (CM_OX + 2); }
+    else { tx = CM_OX + 1; ty = 1 + h - 2 * (CM_OX + 2) - CM_OY; }
+}
+
+// LDS plane position (px, py) of helper cell h: the two-cell ring without its four corners.
+GLM_DEV void cm_helper_cell(int h, int& px, int& py)
+{
+    if (h < CM_OX + 2) { px = 1 + h; py = 0; }
+    else if (h < 2 * (CM_OX + 2)) { px = 1 + h - (CM_OX + 2); py = CM_PY - 1; }
+    else if (h < 2 * (CM_OX + 2) + CM_OY + 2) { px = 0; py = 1 + h - 2 * (CM_OX + 2); }
+    else { px = CM_PX - 1; py = 1 + h - 2 * (CM_OX + 2) - (CM_OY + 2); }
+}
+
+template <typename T>
+struct CTMarchLDS {
+    T P[8][CM_NPL];        // primitives of plane q (two-cell ring)
+    T XB[8][CM_NPL];       // +x face states, then x fluxes (0..4, By in 6) and the z flux's Bx (5), By (7)
+    T YB[8][CM_NPL];       // +y face states, then y fluxes (0..4, Bx in 5)
+    T E[2][3][CM_NPL];     // edge EMFs ex, ey, ez of the last two planes (plane k in slot k & 1)
+};
+
+// Primitives with cell-centred B of cell (x, y, z) (ct_mhd.jl:26-34, :85-96).
+template <typename T>
+GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[8])
+{
+    const size_t pl = (size_t)a.nx * a.ny;
+    const size_t c = (size_t)x + (size_t)a.nx * y + pl * z;
+    T u[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) u[v] = gload(a.q[v] + c);
+    const T bx1 = gload(a.q[5] + ((size_t)cwrap(x + 1, a.nx) + (size_t)a.nx * y + pl * z));
+    const T by1 = gload(a.q[6] + ((size_t)x + (size_t)a.nx * cwrap(y + 1, a.ny) + pl * z));
+    const T bz1 = gload(a.q[7] + ((size_t)x + (size_t)a.nx * y + pl * cwrap(z + 1, a.nz)));
+    const T bx = T(0.5) * (u[5] + bx1), by = T(0.5) * (u[6] + by1), bz = T(0.5) * (u[7] + bz1);
+    T rho = tmax(u[0], a.smallr), ir = T(1) / rho;
+    T vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+    T pr = ct_pressure(u[4], rho, vx, vy, vz, bx, by, bz, a.gamma, a.smallp);
+    p[0] = rho; p[1] = vx; p[2] = vy; p[3] = vz; p[4] = pr; p[5] = bx; p[6] = by; p[7] = bz;
+}
+
+// OU acceleration at cell (i, j, k) (ct_impl.hpp ct_update_kernel; glmmhd_turb.jl:483-505).
+template <typename T>
+GLM_DEV void cm_force(const CTMArgs<T>& a, int i, int j, int k, T acc[3])
+{
+    const int g = a.tgs;
+    const T gg = T(g);
+    T cx = ((T(i) + T(0.5)) * a.dx) / a.lx * gg, cy = ((T(j) + T(0.5)) * a.dx) / a.ly * gg;
+    T cz = ((T(k) + T(0.5)) * a.dx) / a.lz * gg;
+    T fx = floor(cx), fy = floor(cy), fz = floor(cz);
+    int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+    T wx = cx - fx, wy = cy - fy, wz = cz - fz;
+    int i0w = cwrap(i0, g), i1w = cwrap(i0 + 1, g), j0w = cwrap(j0, g), j1w = cwrap(j0 + 1, g);
+    int k0w = cwrap(k0, g), k1w = cwrap(k0 + 1, g);
+    size_t g3 = (size_t)g * g * g;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        const T* af = a.afield + cc * g3;
+        auto at = [&](int x, int y, int z) { return gload(af + ((size_t)x + (size_t)g * ((size_t)y + (size_t)g * z))); };
+        T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
+                (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
+               ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
+                (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
+        acc[cc] = a.ramp * t;
+    }
+}
+
+// Tiles and z-chunks of the CT march on `slots` concurrent workgroups: rounds x (chunk + 2 lead-in
+// planes), the same model as the GLM march's (launch.hpp march_tiling) with its longer lead-in.
+inline void ct_march_tiling(int nx, int ny, int nz, int slots, int& tiles_x, int& tiles_y, int& nzc, int& zchunk)
+{
+    tiles_x = (nx + CM_OX - 1) / CM_OX;
+    tiles_y = (ny + CM_OY - 1) / CM_OY;
+    const long cols = (long)tiles_x * tiles_y;
+    const long sl = slots > 0 ? slots : 256;
+    const int maxc = nz / 8 > 1 ? nz / 8 : 1;
+    long best = -1;
+    zchunk = nz;
+    for (int c = 1; c <= maxc; ++c) {
+        const int zc = (nz + c - 1) / c;
+        const int n = (nz + zc - 1) / zc;
+        if (n != c) continue;
+        const long cost = ((cols * n + sl - 1) / sl) * (long)(zc + 2);
+        if (best < 0 || cost < best) { best = cost; zchunk = zc; }
+    }
+    nzc = (nz + zchunk - 1) / zchunk;
+}
+
+// One step of the CT state: workgroup w marches tile xcd_block(w).
+template <typename T, int RIEM>
+__global__ void __launch_bounds__(CM_NT) __attribute__((amdgpu_waves_per_eu(GLM_CT_WAVES_PER_EU(T))))
+ct_march_kernel(CTMArgs<T> a)
+{
+    __shared__ CTMarchLDS<T> L;
+    T (&P)[8][CM_NPL] = L.P;
+    T (&XB)[8][CM_NPL] = L.XB;
+    T (&YB)[8][CM_NPL] = L.YB;
+    const int tid = threadIdx.x;
+    int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    int tx, ty;
+    cm_lane_cell(tid, tx, ty);
+    const int tile_x = tile % a.tiles_x; tile /= a.tiles_x;
+    const int tile_y = tile % a.tiles_y; tile /= a.tiles_y;
+    const int zs = tile * a.zchunk;
+    const int ze = min(zs + a.zchunk, a.nz);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const size_t plane = (size_t)nx * ny;
+    const int gxu = tile_x * CM_OX - 1 + tx, gyu = tile_y * CM_OY - 1 + ty;
+    const int gx = cwrap(gxu, nx), gy = cwrap(gyu, ny);
+    const bool owned = tid < CM_NO && gxu < nx && gyu < ny;
+    const bool corner = (tx == 0 || tx == CM_OX + 1) && (ty == 0 || ty == CM_OY + 1);
+    const bool need_x = tx >= 1;                      // this lane solves the x face at tx - 1/2
+    const bool need_y = ty >= 1;                      // and the y face at ty - 1/2
+    const bool need_z = !corner;                      // and the z face below its cell
+    const bool emf_z = tx >= 1 && ty >= 1;            // ez at the lane's lower x-y edge
+    const bool emf_x = tx >= 1 && tx <= CM_OX && ty >= 1;
+    const bool emf_y = tx >= 1 && ty >= 1 && ty <= CM_OY;
+    const size_t col = (size_t)gx + (size_t)nx * gy;
+    const int lo = (ty + 1) * CM_PX + (tx + 1);
+    const bool helper = tid < CM_NHELP;
+    int hx = 0, hy = 0;
+    cm_helper_cell(helper ? tid : 0, hx, hy);
+    const int hgx = cwrap(tile_x * CM_OX - 2 + hx, nx), hgy = cwrap(tile_y * CM_OY - 2 + hy, ny);
+    const int lo_ring = hy * CM_PX + hx;
+
+    // column window: primitives of planes q-1 and q+1 (plane q is read back from LDS)
+    T pm[8], pp[8];
+    {
+        T p0[8];
+        cm_prim(a, gx, gy, cwrap(zs - 2, nz), pm);
+        cm_prim(a, gx, gy, cwrap(zs - 1, nz), p0);
+        cm_prim(a, gx, gy, cwrap(zs, nz), pp);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) P[v][lo] = p0[v];
+        if (helper) {
+            T h[8];
+            cm_prim(a, hgx, hgy, cwrap(zs - 1, nz), h);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) P[v][lo_ring] = h[v];
+        }
+    }
+    __syncthreads();
+
+    T zprev[8];                 // +z face state of plane q-1
+    T fzp[5];                   // hydro part of the z flux below plane q-1
+    T pend[5];                  // (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) of plane q-1
+    T fxbz = T(0), fybz = T(0); // Fx[Bz], Fy[Bz] of this cell in plane q-1
+#pragma unroll
+    for (int v = 0; v < 8; ++v) zprev[v] = T(0);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { fzp[v] = T(0); pend[v] = T(0); }
+
+    for (int q = zs - 1; q <= ze; ++q) {
+        const int zq = cwrap(q, nz);
+        const bool lead = q == zs - 1;
+        const size_t off = col + plane * zq;
+
+        // ---- face states of this lane's cell in plane q, z face q-1/2 ----------------------------
+        T fz[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) fz[v] = T(0);
+        {
+            T c0[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) c0[v] = P[v][lo];
+            T zlo[8], zhi[8];
+            ct_cell_states<T, 2>(pm, c0, pp, a, zlo, zhi);
+            if (need_z && !lead) ct_riemann<T, RIEM, 2>(zprev, zlo, gload(a.q[7] + off), a, fz);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) zprev[v] = zhi[v];
+        }
+        T xlo[8], ylo[8];
+        {
+            T l[8], c0[8], r[8], hi[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) { l[v] = P[v][lo - 1]; c0[v] = P[v][lo]; r[v] = P[v][lo + 1]; }
+            ct_cell_states<T, 0>(l, c0, r, a, xlo, hi);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) XB[v][lo] = hi[v];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) { l[v] = P[v][lo - CM_PX]; r[v] = P[v][lo + CM_PX]; }
+            ct_cell_states<T, 1>(l, c0, r, a, ylo, hi);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) YB[v][lo] = hi[v];
+        }
+        __syncthreads();
+
+        // ---- x and y faces ------------------------------------------------------------------------
+        T fx[8], fy[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) { fx[v] = T(0); fy[v] = T(0); }
+        if (need_x) {
+            T Lx[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) Lx[v] = XB[v][lo - 1];
+            ct_riemann<T, RIEM, 0>(Lx, xlo, gload(a.q[5] + off), a, fx);
+        }
+        if (need_y) {
+            T Ly[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) Ly[v] = YB[v][lo - CM_PX];
+            ct_riemann<T, RIEM, 1>(Ly, ylo, gload(a.q[6] + off), a, fy);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { XB[v][lo] = fx[v]; YB[v][lo] = fy[v]; }
+        XB[6][lo] = fx[6];
+        YB[5][lo] = fy[5];
+        if (!lead) { XB[5][lo] = fz[5]; XB[7][lo] = fz[6]; }
+        __syncthreads();
+
+        if (!lead) {
+            // ---- edge EMFs of plane q (ct_mhd.jl:139-154) ----------------------------------------
+            // ez at (i-1/2, j-1/2), ex at (j-1/2, k-1/2), ey at (i-1/2, k-1/2) of this lane's cell
+            T ex = T(0), ey = T(0), ez = T(0);
+            if (emf_z) ez = T(0.25) * (fy[5] + YB[5][lo - 1] - fx[6] - XB[6][lo - CM_PX]);
+            if (emf_x) ex = T(0.25) * (fz[6] + XB[7][lo - CM_PX] - fy[7] - fybz);
+            if (emf_y) ey = T(0.25) * (fx[7] + fxbz - fz[5] - XB[5][lo - 1]);
+            T (&Ec)[3][CM_NPL] = L.E[q & 1];
+            T (&Ep)[3][CM_NPL] = L.E[(q - 1) & 1];
+            Ec[0][lo] = ex; Ec[1][lo] = ey; Ec[2][lo] = ez;
+
+            // ---- update of plane q-1 (ct_mhd.jl:124-136, :156-171) --------------------------------
+            if (owned && q >= zs + 1) {
+                const int zu = cwrap(q - 1, nz);
+                const size_t ou = col + plane * zu;
+                T r[5];
+#pragma unroll
+                for (int t = 0; t < 5; ++t) r[t] = gload(a.q[t] + ou) + a.dtdx * (pend[t] + (fzp[t] - fz[t]));
+                const T ex0 = Ep[0][lo], ey0 = Ep[1][lo], ez0 = Ep[2][lo];
+                const T bxn = gload(a.q[5] + ou) - a.dtdx * ((Ep[2][lo + CM_PX] - ez0) - (ey - ey0));
+                const T byn = gload(a.q[6] + ou) - a.dtdx * ((ex - ex0) - (Ep[2][lo + 1] - ez0));
+                const T bzn = gload(a.q[7] + ou) - a.dtdx * ((Ep[1][lo + 1] - ey0) - (Ep[0][lo + CM_PX] - ex0));
+                if (a.update_floor && r[0] < a.smallr) {
+                    T emag = T(0.5) * (pm[5] * pm[5] + pm[6] * pm[6] + pm[7] * pm[7]);
+                    r[0] = a.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0); r[4] = a.smallp * a.igm1 + emag;
+                }
+                if (a.do_turb && r[0] >= a.turb_min_rho) {
+                    T acc[3];
+                    cm_force(a, gx, gy, zu, acc);
+                    T rho = r[0], rhom = tmax(rho, a.smallr), irm = T(1) / rhom, fl = rho * a.smallc * a.smallc, e = r[4];
+                    e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
+                    e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
+                    e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
+                    T m2 = r[1] + rhom * acc[0] * a.dt, m3 = r[2] + rhom * acc[1] * a.dt, m4 = r[3] + rhom * acc[2] * a.dt;
+                    e = tmax(e + T(0.5) * m2 * m2 * irm, fl);
+                    e = tmax(e + T(0.5) * m3 * m3 * irm, fl);
+                    e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
+                    r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
+                }
+#pragma unroll
+                for (int t = 0; t < 5; ++t) gstore_nt(r[t], a.o[t] + ou);
+                gstore_nt(bxn, a.o[5] + ou);
+                gstore_nt(byn, a.o[6] + ou);
+                gstore_nt(bzn, a.o[7] + ou);
+            }
+            // in-plane flux differences of plane q for its update in the next pass
+            if (owned) {
+#pragma unroll
+                for (int t = 0; t < 5; ++t) pend[t] = (fx[t] - XB[t][lo + 1]) + (fy[t] - YB[t][lo + CM_PX]);
+            }
+#pragma unroll
+            for (int t = 0; t < 5; ++t) fzp[t] = fz[t];
+        }
+        fxbz = fx[7];
+        fybz = fy[7];
+        if (q == ze) break;
+
+        // ---- prefetch and publish plane q+1 -------------------------------------------------------
+        T pn[8], hn[8];
+        cm_prim(a, gx, gy, cwrap(q + 2, nz), pn);
+        if (helper) cm_prim(a, hgx, hgy, cwrap(q + 1, nz), hn);
+        __syncthreads();   // every read of plane q's LDS cells and of the flux exchange is done
+#pragma unroll
+        for (int v = 0; v < 8; ++v) { pm[v] = P[v][lo]; P[v][lo] = pp[v]; pp[v] = pn[v]; }
+        if (helper) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) P[v][lo_ring] = hn[v];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace glm

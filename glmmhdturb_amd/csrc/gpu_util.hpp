@@ -1,0 +1,41 @@
+// gpu_util.hpp -- small device helpers shared by the march kernels (GLM and CT).
+#pragma once
+#include "physics.hpp"
+
+namespace glm {
+
+constexpr int NXCD = 8;   // XCDs of an MI355X
+
+// Logical tile of hardware workgroup id w when ids are dealt round-robin over NXCD XCDs: XCD x
+// gets the contiguous logical range [start_x, start_x + count_x).  A bijection on [0, total).
+GLM_DEV int xcd_block(int w, int total)
+{
+    const int q = total / NXCD, r = total % NXCD;
+    const int xcd = w % NXCD, slot = w / NXCD;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+}
+
+// HBM access through the global address space.  Pointers read back from LDS (the cold arguments)
+// have lost their address space, and generic loads are counted together with LDS operations, so
+// every LDS wait in the march would also wait for the prefetch of the next planes.
+template <typename T>
+GLM_DEV T gload(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) T*)p;
+#else
+    return *p;   // host pass of the kernel templates: never executed
+#endif
+}
+
+template <typename T>
+GLM_DEV void gstore_nt(T v, T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)p);
+#else
+    *p = v;
+#endif
+}
+
+}  // namespace glm

@@ -31,6 +31,7 @@
 //   * The epilogue also folds the next step's CFL signal speed (glmmhd_turb.jl:1963-1971) into a
 //     per-thread max, reduced per wavefront and merged with one atomic per wavefront.
 #pragma once
+#include "gpu_util.hpp"
 #include "physics.hpp"
 
 namespace glm {
@@ -53,7 +54,6 @@ constexpr int PX = OX + 4, PY = OY + 4;     // LDS primitive plane (two-cell rin
 constexpr int NPL = PX * PY;
 constexpr int NHELP = 4 + NH;               // plane cells no lane owns: the 4 one-cell corners and the
                                             // two-cell ring without corners (in-plane neighbours of halo lanes)
-constexpr int NXCD = 8;                     // XCDs of an MI355X
 static_assert(NT <= 256 && NT >= NHELP && NO % 64 == 0, "tile shape");
 
 // Minimum resident wavefronts per SIMD requested from the register allocator.
@@ -92,15 +92,6 @@ struct ColdArgs {
     int update_floor, do_turb, tgs, pad_;
 };
 
-// Logical tile of hardware workgroup id w when ids are dealt round-robin over NXCD XCDs: XCD x
-// gets the contiguous logical range [start_x, start_x + count_x).  A bijection on [0, total).
-GLM_DEV int xcd_block(int w, int total)
-{
-    const int q = total / NXCD, r = total % NXCD;
-    const int xcd = w % NXCD, slot = w / NXCD;
-    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
-}
-
 GLM_DEV int wrapi(int i, int n)
 {
     int m = i % n;
@@ -127,29 +118,6 @@ GLM_DEV void helper_cell(int h, int& px, int& py)
     else if (r < 2 * OX) { px = 2 + r - OX; py = PY - 1; }
     else if (r < 2 * OX + OY) { px = 0; py = 2 + r - 2 * OX; }
     else { px = PX - 1; py = 2 + r - 2 * OX - OY; }
-}
-
-// HBM access through the global address space.  Pointers read back from LDS (the cold arguments)
-// have lost their address space, and generic loads are counted together with LDS operations, so
-// every LDS wait in the march would also wait for the prefetch of the next planes.
-template <typename T>
-GLM_DEV T gload(const T* p)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(1))) T*)p;
-#else
-    return *p;   // host pass of the kernel templates: never executed
-#endif
-}
-
-template <typename T>
-GLM_DEV void gstore_nt(T v, T* p)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)p);
-#else
-    *p = v;
-#endif
 }
 
 template <typename T>

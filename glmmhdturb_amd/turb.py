@@ -285,10 +285,13 @@ class Solver:
         return out.value
 
     # ---- constrained transport ------------------------------------------------------------
-    def ct_step(self, qold, qnew, args: GLMParams, face_slope: int = 0):
+    def ct_step(self, qold, qnew, args: GLMParams, face_slope: int = 0, passes: bool = False):
+        """One CT step (ct_mhd.jl step!, :174-184): the fused z-march kernel, or with passes=True
+        the multi-pass kernel sequence (same results to rounding)."""
         qa, kq = self._planes(qold, 8)
         oa, ko = self._planes(qnew, 8)
-        check(self.lib.glm_ct_step(self.ctx, qa, oa, C.byref(args), int(face_slope)))
+        fn = self.lib.glm_ct_step_passes if passes else self.lib.glm_ct_step
+        check(fn(self.ctx, qa, oa, C.byref(args), int(face_slope)))
 
     def ct_ctot_max(self, q) -> float:
         pa, keep = self._planes(q, 8)

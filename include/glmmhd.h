@@ -141,11 +141,15 @@ int glm_ou_copy_afield(glm_ctx* ctx, void* dst);
 /* Number of OU steps taken so far. */
 long long glm_ou_steps(glm_ctx* ctx);
 
-/* CT step (ct_mhd.jl step!, :174-184; spike_ctm.cu ct_march): q, o = 8 planes.  Uses gamma,
- * smallr, pfl (as SMALLP), dt, dx, riemann (HLL or HLLD or LLF), recon (PLM or PPM_SPIKE),
- * update_floor, do_turb, turb_min_rho, smallc, ramp, want_cfl from p; ct_face_slope selects the
- * spike_ctm.cu (0) or the ct_mhd.jl (1) face offset. */
+/* CT step as one fused z-march kernel (spike_ctm.cu:250-298 ct_march; physics of ct_mhd.jl
+ * step!, :174-184): q, o = 8 planes.  Uses gamma, smallr, pfl (as SMALLP), dt, dx, riemann (HLL
+ * or HLLD or LLF), recon (PLM or PPM_SPIKE), update_floor, do_turb, turb_min_rho, smallc, ramp,
+ * want_cfl from p; ct_face_slope selects the spike_ctm.cu (0) or the ct_mhd.jl (1) face offset. */
 int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope);
+/* The same CT step as the kernel sequence of ct_mhd.jl step! (:174-184): cell-centred B and
+ * primitives, face fluxes, edge EMFs, update, each its own pass over a 35-plane scratch.  Same
+ * arguments and results (to rounding) as glm_ct_step. */
+int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope);
 /* CFL max of a CT state (cell-centred B); synchronous. */
 int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
 /* max |div B| of a CT state from the face fields (ct_mhd.jl:191-194); synchronous. */

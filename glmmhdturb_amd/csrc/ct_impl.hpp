@@ -82,7 +82,7 @@ GLM_DEV void ct_toC(const T q[8], T u[8], T igm1)
 template <typename T>
 GLM_DEV T ct_cf(const T q[8], T bn, T gamma)
 {
-    T ir = T(1) / q[0];
+    T ir = trcp(q[0]);
     T c2 = gamma * q[4] * ir, b2 = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir, dd = T(0.5) * (b2 + c2);
     return tsqrt(dd + tsqrt(tmax(dd * dd - c2 * bn * bn * ir, T(0))));
 }
@@ -102,7 +102,7 @@ GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const A& a)
     ct_toC(q, U, a.igm1);
 #pragma unroll
     for (int k = 0; k < 8; ++k) U[k] = U[k] - a.half_dtdx * (FR[k] - FL[k]);
-    T rho = tmax(U[0], a.smallr), ir = T(1) / rho;
+    T rho = tmax(U[0], a.smallr), ir = trcp(rho);
     T vx = U[1] * ir, vy = U[2] * ir, vz = U[3] * ir;
     T p = ct_pressure(U[4], rho, vx, vy, vz, U[5], U[6], U[7], a.gamma, a.smallp);
     mh[0] = rho; mh[1] = vx; mh[2] = vy; mh[3] = vz; mh[4] = p; mh[5] = U[5]; mh[6] = U[6]; mh[7] = U[7];
@@ -174,7 +174,7 @@ GLM_DEV void ct_riemann(const T Lq[8], const T Rq[8], T bn, const A& a, T f[8])
         for (int k = 0; k < 8; ++k) f[k] = T(0.5) * (FL[k] + FR[k]) - T(0.5) * smax * (UR[k] - UL[k]);
     } else {
         T SL = tmin(tmin(uL - cL, uR - cR), T(0)), SR = tmax(tmax(uL + cL, uR + cR), T(0));
-        T ih = T(1) / (SR - SL);
+        T ih = trcp(SR - SL);
 #pragma unroll
         for (int k = 0; k < 8; ++k) f[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * ih;
     }
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(CT_NT) ct_prim_kernel(CTArgs<T> a)
     T bx = T(0.5) * (a.q[5][c] + a.q[5][cidx(a, cwrap(i + 1, a.nx), j, k)]);
     T by = T(0.5) * (a.q[6][c] + a.q[6][cidx(a, i, cwrap(j + 1, a.ny), k)]);
     T bz = T(0.5) * (a.q[7][c] + a.q[7][cidx(a, i, j, cwrap(k + 1, a.nz))]);
-    T rho = tmax(a.q[0][c], a.smallr), ir = T(1) / rho;
+    T rho = tmax(a.q[0][c], a.smallr), ir = trcp(rho);
     T vx = a.q[1][c] * ir, vy = a.q[2][c] * ir, vz = a.q[3][c] * ir;
     T p = ct_pressure(a.q[4][c], rho, vx, vy, vz, bx, by, bz, a.gamma, a.smallp);
     T q8[8] = {rho, vx, vy, vz, p, bx, by, bz};
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(CT_NT) ct_update_kernel(CTArgs<T> a)
                         (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
                 acc[cc] = a.ramp * t;
             }
-            T rho = r[0], rhom = tmax(rho, a.smallr), irm = T(1) / rhom, fl = rho * a.smallc * a.smallc, e = r[4];
+            T rho = r[0], rhom = tmax(rho, a.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc, e = r[4];
             e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
             e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
             e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(CT_NT) ct_ctot_kernel(Ptr9<const T> q, int nx,
         T bx = T(0.5) * (q[5][c] + q[5][(size_t)cwrap(i + 1, nx) + (size_t)nx * j + pl * k]);
         T by = T(0.5) * (q[6][c] + q[6][(size_t)i + (size_t)nx * cwrap(j + 1, ny) + pl * k]);
         T bz = T(0.5) * (q[7][c] + q[7][(size_t)i + (size_t)nx * j + pl * cwrap(k + 1, nz)]);
-        T rho = tmax(q[0][c], smallr), ir = T(1) / rho;
+        T rho = tmax(q[0][c], smallr), ir = trcp(rho);
         T vx = q[1][c] * ir, vy = q[2][c] * ir, vz = q[3][c] * ir;
         T p = tmax((gamma - T(1)) * (q[4][c] - T(0.5) * rho * (vx * vx + vy * vy + vz * vz) -
                                      T(0.5) * (bx * bx + by * by + bz * bz)),

@@ -12,8 +12,9 @@
 //     EMF at an edge of the tile needs fluxes of the ring's corner cells).  Every lane builds the
 //     x, y and z face states of its cell; the LDS plane holds the primitives of plane q over the
 //     tile grown by two cells (the in-plane neighbours of the ring).
-//   * Per plane q: (1) z face q-1/2 from the +z state carried from plane q-1; (2) +x/+y face
-//     states through LDS, x and y faces solved once; (3) fluxes back through LDS; (4) the edge
+//   * Per plane q: (1) +x/+y face states through LDS, x and y faces solved once; (2) fluxes back
+//     through LDS, then the z face q-1/2 from the +z state carried from plane q-1 (solved after
+//     the x/y fluxes have left the registers); (3) the z flux's field part through LDS; (4) the edge
 //     EMFs of plane q (ex, ey need the y/x fluxes of plane q-1, carried per column); (5) the
 //     update of plane q-1, whose curl needs ex, ey of plane q and the EMFs of plane q-1 (kept in
 //     a two-plane LDS ring); (6) the prefetch and publish of plane q+1.
@@ -96,7 +97,7 @@ GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[8])
     const T by1 = gload(a.q[6] + ((size_t)x + (size_t)a.nx * cwrap(y + 1, a.ny) + pl * z));
     const T bz1 = gload(a.q[7] + ((size_t)x + (size_t)a.nx * y + pl * cwrap(z + 1, a.nz)));
     const T bx = T(0.5) * (u[5] + bx1), by = T(0.5) * (u[6] + by1), bz = T(0.5) * (u[7] + bz1);
-    T rho = tmax(u[0], a.smallr), ir = T(1) / rho;
+    T rho = tmax(u[0], a.smallr), ir = trcp(rho);
     T vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
     T pr = ct_pressure(u[4], rho, vx, vy, vz, bx, by, bz, a.gamma, a.smallp);
     p[0] = rho; p[1] = vx; p[2] = vy; p[3] = vz; p[4] = pr; p[5] = bx; p[6] = by; p[7] = bz;
@@ -218,20 +219,10 @@ ct_march_kernel(CTMArgs<T> a)
         const bool lead = q == zs - 1;
         const size_t off = col + plane * zq;
 
-        // ---- face states of this lane's cell in plane q, z face q-1/2 ----------------------------
-        T fz[8];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) fz[v] = T(0);
-        {
-            T c0[8];
-#pragma unroll
-            for (int v = 0; v < 8; ++v) c0[v] = P[v][lo];
-            T zlo[8], zhi[8];
-            ct_cell_states<T, 2>(pm, c0, pp, a, zlo, zhi);
-            if (need_z && !lead) ct_riemann<T, RIEM, 2>(zprev, zlo, gload(a.q[7] + off), a, fz);
-#pragma unroll
-            for (int v = 0; v < 8; ++v) zprev[v] = zhi[v];
-        }
+        // face-normal fields of this cell's lower faces, loaded ahead of their Riemann solves
+        const T bnx = gload(a.q[5] + off), bny = gload(a.q[6] + off), bnz = gload(a.q[7] + off);
+
+        // ---- x and y face states of this lane's cell in plane q -----------------------------------
         T xlo[8], ylo[8];
         {
             T l[8], c0[8], r[8], hi[8];
@@ -256,19 +247,36 @@ ct_march_kernel(CTMArgs<T> a)
             T Lx[8];
 #pragma unroll
             for (int v = 0; v < 8; ++v) Lx[v] = XB[v][lo - 1];
-            ct_riemann<T, RIEM, 0>(Lx, xlo, gload(a.q[5] + off), a, fx);
+            ct_riemann<T, RIEM, 0>(Lx, xlo, bnx, a, fx);
         }
         if (need_y) {
             T Ly[8];
 #pragma unroll
             for (int v = 0; v < 8; ++v) Ly[v] = YB[v][lo - CM_PX];
-            ct_riemann<T, RIEM, 1>(Ly, ylo, gload(a.q[6] + off), a, fy);
+            ct_riemann<T, RIEM, 1>(Ly, ylo, bny, a, fy);
         }
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < 5; ++v) { XB[v][lo] = fx[v]; YB[v][lo] = fy[v]; }
         XB[6][lo] = fx[6];
         YB[5][lo] = fy[5];
+        const T fx7 = fx[7], fy7 = fy[7];
+        __asm__ volatile("" ::: "memory");   // the other flux components are read back from LDS below
+
+        // ---- z face q-1/2 (the plane's LDS cells are untouched until the publish) -----------------
+        T fz[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) fz[v] = T(0);
+        {
+            T c0[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) c0[v] = P[v][lo];
+            T zlo[8], zhi[8];
+            ct_cell_states<T, 2>(pm, c0, pp, a, zlo, zhi);
+            if (need_z && !lead) ct_riemann<T, RIEM, 2>(zprev, zlo, bnz, a, fz);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) zprev[v] = zhi[v];
+        }
         if (!lead) { XB[5][lo] = fz[5]; XB[7][lo] = fz[6]; }
         __syncthreads();
 
@@ -276,9 +284,9 @@ ct_march_kernel(CTMArgs<T> a)
             // ---- edge EMFs of plane q (ct_mhd.jl:139-154) ----------------------------------------
             // ez at (i-1/2, j-1/2), ex at (j-1/2, k-1/2), ey at (i-1/2, k-1/2) of this lane's cell
             T ex = T(0), ey = T(0), ez = T(0);
-            if (emf_z) ez = T(0.25) * (fy[5] + YB[5][lo - 1] - fx[6] - XB[6][lo - CM_PX]);
-            if (emf_x) ex = T(0.25) * (fz[6] + XB[7][lo - CM_PX] - fy[7] - fybz);
-            if (emf_y) ey = T(0.25) * (fx[7] + fxbz - fz[5] - XB[5][lo - 1]);
+            if (emf_z) ez = T(0.25) * (YB[5][lo] + YB[5][lo - 1] - XB[6][lo] - XB[6][lo - CM_PX]);
+            if (emf_x) ex = T(0.25) * (fz[6] + XB[7][lo - CM_PX] - fy7 - fybz);
+            if (emf_y) ey = T(0.25) * (fx7 + fxbz - fz[5] - XB[5][lo - 1]);
             T (&Ec)[3][CM_NPL] = L.E[q & 1];
             T (&Ep)[3][CM_NPL] = L.E[(q - 1) & 1];
             Ec[0][lo] = ex; Ec[1][lo] = ey; Ec[2][lo] = ez;
@@ -301,7 +309,7 @@ ct_march_kernel(CTMArgs<T> a)
                 if (a.do_turb && r[0] >= a.turb_min_rho) {
                     T acc[3];
                     cm_force(a, gx, gy, zu, acc);
-                    T rho = r[0], rhom = tmax(rho, a.smallr), irm = T(1) / rhom, fl = rho * a.smallc * a.smallc, e = r[4];
+                    T rho = r[0], rhom = tmax(rho, a.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc, e = r[4];
                     e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
                     e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
                     e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
@@ -320,13 +328,13 @@ ct_march_kernel(CTMArgs<T> a)
             // in-plane flux differences of plane q for its update in the next pass
             if (owned) {
 #pragma unroll
-                for (int t = 0; t < 5; ++t) pend[t] = (fx[t] - XB[t][lo + 1]) + (fy[t] - YB[t][lo + CM_PX]);
+                for (int t = 0; t < 5; ++t) pend[t] = (XB[t][lo] - XB[t][lo + 1]) + (YB[t][lo] - YB[t][lo + CM_PX]);
             }
 #pragma unroll
             for (int t = 0; t < 5; ++t) fzp[t] = fz[t];
         }
-        fxbz = fx[7];
-        fybz = fy[7];
+        fxbz = fx7;
+        fybz = fy7;
         if (q == ze) break;
 
         // ---- prefetch and publish plane q+1 -------------------------------------------------------

@@ -159,24 +159,22 @@ GLM_HD T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
 }
 
 // CFL signal speed sum_d (|v_d| + c_f(B_d)) of a primitive state (glmmhd_turb.jl:1963-1971),
-// sharing 1/rho, a^2 and the half sum between the three directions.  Evaluated in FP64 for either
-// state precision: the step size is a global FP64 reduction.
+// sharing 1/rho, a^2 and the half sum between the three directions.  Evaluated in the state
+// precision (the fast speed has no cancellation that FP32 would lose: the inner root is of a sum
+// of squares, the outer of a positive sum; the FP32 pressure it uses was recovered in FP64) and
+// returned as FP64: the max over cells and the step size are FP64.
 template <typename T>
-GLM_DEV double cfl_speed(const T qt[9], const Phys<T>& st)
+GLM_DEV double cfl_speed(const T q[9], const Phys<T>& s)
 {
-    double q[9];
-#pragma unroll
-    for (int v = 0; v < 9; ++v) q[v] = (double)qt[v];
-    struct { double gamma; } s = {(double)st.gamma};
-    double ir = trcp(q[0]);
-    double c2 = s.gamma * q[4] * ir;
-    double bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
-    double d2 = 0.5 * (bb + c2);
-    double dd = d2 * d2, k = c2 * ir;
-    double fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], 0.0)));
-    double fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], 0.0)));
-    double fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], 0.0)));
-    return (tabs(q[1]) + fx) + (tabs(q[2]) + fy) + (tabs(q[3]) + fz);
+    T ir = trcp(q[0]);
+    T c2 = s.gamma * q[4] * ir;
+    T bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
+    T d2 = T(0.5) * (bb + c2);
+    T dd = d2 * d2, k = c2 * ir;
+    T fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], T(0))));
+    T fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], T(0))));
+    T fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], T(0))));
+    return (double)((tabs(q[1]) + fx) + (tabs(q[2]) + fy) + (tabs(q[3]) + fz));
 }
 
 // Ideal-MHD flux of a primitive state in direction D (glmmhd_turb.jl:98-115 evaluated in the
@@ -244,7 +242,9 @@ GLM_HD void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
     bool hi_os = diff > lim;
     bool lo_os = diff < -lim;
     lo = flat ? q0 : (hi_os ? T(3) * q0 - T(2) * qr : ql);
-    hi = flat ? q0 : (hi_os ? qr : (lo_os ? T(3) * q0 - T(2) * ql : qr));
+    // the reference's hi = flat ? q0 : (hi_os ? qr : (lo_os ? 3 q0 - 2 ql : qr)); hi_os and lo_os
+    // exclude each other (lim >= 0, and NaN fails both), so the middle select is redundant
+    hi = flat ? q0 : (lo_os ? T(3) * q0 - T(2) * ql : qr);
 }
 
 // spike_mhd.cu:164-172 ppm1 (both edges).

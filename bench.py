@@ -445,6 +445,19 @@ def run_adjoint(N, nsteps, window=50, snapshots=16):
                      "eps": eps, "map": "recorded dt, ch, psi decay, ramp and forcing realisation"},
     }
     tr.close()
+    # the same reverse sweep over states kept in HBM by the forward run (nothing recomputed)
+    tra = Trajectory(p, nsteps, window, AMP, riemann="hlld", recon="ppm", snapshots="all")
+    tra.forward(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lam0a, dampa = tra.pullback()
+    torch.cuda.synchronize()
+    adj_a = time.perf_counter() - t0
+    out["all_states_stored"] = {
+        "adjoint_over_forward_cost": adj_a / fwd_s, "adjoint_s": adj_a, "recomputed_forward_steps": tra.forward_steps,
+        "stored_state_GB": len(tra.states) * 9 * 8 * N ** 3 / 1e9,
+        "same_gradient_bits": bool(torch.equal(lam0a, lam0)) and dampa == damp}
+    tra.close()
     return out
 
 

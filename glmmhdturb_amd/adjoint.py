@@ -7,7 +7,9 @@ derivatives).  The counterpart of a ChainRulesCore ``rrule`` / ``torch.autograd.
   magnetic energy sum 0.5 |B|^2 dV.
 * ``Trajectory.pullback(jbar)`` returns (jbar dJ/dU0, jbar dJ/damp): the reverse sweep runs one
   ``glm_step_adjoint`` per step over states recomputed from checkpoints with binomial
-  checkpointing (``snapshots`` stored states).
+  checkpointing (``snapshots`` stored states), or, with ``snapshots="all"``, over the states the
+  forward run kept in device memory (one state per step: 151 MB at 128^3 FP64, so 200 steps fit
+  many times over in an MI355X's 288 GB) with nothing recomputed.
 
 The derivative is that of the recorded map: dt and the forcing realisation are held at their
 recorded values (the CFL dt is a max over cells, not a smooth function of the state).
@@ -58,12 +60,14 @@ class StepRecord:
 
 class Trajectory:
     def __init__(self, p: Params, nsteps: int, window: int, amp: float, *, riemann="hlld", recon="ppm",
-                 predictor=None, slope="moncen", scheme="cube", snapshots: int = 16, do_turb: bool = True):
+                 predictor=None, slope="moncen", scheme="cube", snapshots=16, do_turb: bool = True):
         import torch
         self.torch = torch
         self.p, self.nsteps, self.window, self.amp = p, int(nsteps), int(window), float(amp)
         self.kw = dict(riemann=riemann, recon=recon, predictor=predictor, slope=slope, scheme=scheme)
-        self.snapshots = int(snapshots)
+        self.store_all = snapshots == "all"
+        self.snapshots = 0 if self.store_all else int(snapshots)
+        self.states = []
         self.do_turb = do_turb
         self.s = Solver(p, "f64")
         if do_turb:
@@ -92,6 +96,7 @@ class Trajectory:
         s, p = self.s, self.p
         self.u0 = u0.clone()
         self.records, self.fields = [], []
+        self.states = []
         u, w = u0.clone(), s.new_state()
         f = s.forcing
         if f is not None:
@@ -111,6 +116,8 @@ class Trajectory:
                 dt = min(dt, f.next_time - t)
             a = step_args(p, dt, t, do_turb=f is not None, want_cfl=True, **self.kw)
             self.records.append(StepRecord(a, fi))
+            if self.store_all and n > 0:
+                self.states.append(u.clone())      # U_n (U_0 is self.u0)
             s.step(u, w, a)
             u, w = w, u
             t += dt
@@ -185,13 +192,19 @@ class Trajectory:
         if self.in_window(self.nsteps):
             lam[5:8] = self.uN[5:8] * (self.dV() / self.window)
         s.adjoint_force_dot(reset=True)
-        lam0 = self._reverse(0, self.nsteps, self.u0, self.snapshots, lam)
+        if self.store_all:
+            for n in range(self.nsteps - 1, -1, -1):
+                lam = self._reverse_one(n, self.states[n - 1] if n > 0 else self.u0, lam)
+            lam0 = lam
+        else:
+            lam0 = self._reverse(0, self.nsteps, self.u0, self.snapshots, lam)
         fdot = s.adjoint_force_dot(reset=True)
         self.torch.cuda.synchronize()
         damp = fdot / self.amp if self.amp != 0 else 0.0
         return lam0 * jbar, damp * jbar
 
     def close(self):
+        self.states = []
         self.s.close()
 
 

@@ -173,8 +173,9 @@ def test_trajectory_state_and_amplitude_gradient():
 
 
 def test_checkpointing_is_bit_identical():
+    """Any number of snapshots, and all states kept from the forward run, give the same bits."""
     grads = []
-    for snaps in (0, 2, 40):
+    for snaps in (0, 2, 40, "all"):
         tr, v0, u0, _ = make_traj(nsteps=9, window=4, snapshots=snaps)
         tr.forward(u0)
         lam0, damp = tr.pullback()
@@ -182,8 +183,8 @@ def test_checkpointing_is_bit_identical():
         tr.close()
     for g in grads[1:]:
         assert torch.equal(g[0], grads[0][0]) and g[1] == grads[0][1]
-    assert grads[0][3] == grads[1][3] == grads[2][3] == 9
-    assert grads[0][2] > grads[1][2] >= grads[2][2]
+    assert grads[0][3] == grads[1][3] == grads[2][3] == grads[3][3] == 9
+    assert grads[0][2] > grads[1][2] >= grads[2][2] > grads[3][2] == 0
 
 
 def test_trajectory_gradient_vs_dual_oracle_32():

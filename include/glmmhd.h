@@ -49,7 +49,7 @@ double march_run_dev(float* const* qv, float* const* ov, int nsteps);
  * set the runtime grid used by march_run_dev / march_run_dev_f64 / ct_run.  Returns 0 or an
  * error code. */
 int march_set_grid(int nx, int ny, int nz);
-/* FP64 twin of march_run_dev (BASELINE FP64 path; no reference counterpart). */
+/* FP64 twin of march_run_dev (the FP64 state path; no reference counterpart). */
 double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps);
 
 /* spike_ctm.cu:305-325 CT march: qv/ov = 8 planes [U0..U4, bxf, byf, bzf]; cc = 3 Bcc scratch
@@ -156,7 +156,7 @@ int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* c
 int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
 
 /* ===================================================================================== */
-/* 4. Adjoint (no reference counterpart: the reference has no derivatives).      */
+/* 4. Adjoint (no reference counterpart: the reference computes no derivatives).             */
 /*    The reverse of one glm_step for the scheme of its glm_params, FP64 contexts only:       */
 /*    lam0 = (dU1/dU0)^T lam1 with the step's dt, ch, psi decay, ramp and forcing field held  */
 /*    fixed.  Supported: full / normal-only predictor, PLM or PPM (Julia form), MonCen or     */

@@ -41,7 +41,7 @@ static_assert(CM_NT == CT_MARCH_THREADS, "launch width of the CT march");
 #define GLM_CT_WAVES_F64 2
 #endif
 #ifndef GLM_CT_WAVES_F32
-#define GLM_CT_WAVES_F32 2
+#define GLM_CT_WAVES_F32 3
 #endif
 #define GLM_CT_WAVES_PER_EU(T) (sizeof(T) == 8 ? GLM_CT_WAVES_F64 : GLM_CT_WAVES_F32)
 
@@ -126,6 +126,7 @@ GLM_DEV void cm_force(const CTMArgs<T>& a, int i, int j, int k, T acc[3])
                ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
                 (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
         acc[cc] = a.ramp * t;
+        __asm__ volatile("" ::: "memory");   // one component's eight corner loads in flight at a time
     }
 }
 

@@ -32,7 +32,7 @@ EXPORTS = (
     "glm_ens_create", "glm_ens_destroy", "glm_ens_stream", "glm_ens_reset", "glm_ens_step", "glm_ens_graph",
     "glm_ens_sync", "glm_ens_sweeps", "glm_ens_current", "glm_ens_log", "glm_ens_timing", "glm_ens_configure",
     "glm_ens_tune",
-    "glm_step_adjoint", "glm_adjoint_force_dot", "glm_ou_set_afield",
+    "glm_step_adjoint", "glm_step_adjoint_passes", "glm_adjoint_force_dot", "glm_ou_set_afield",
 )
 ENS_NLOG = 8
 ENS_LOG_LEN = 256
@@ -102,6 +102,7 @@ def load(path: str | None = None):
         "glm_ens_log": (i, [vp, i, i, C.POINTER(d)]), "glm_ens_timing": (i, [vp, i, C.POINTER(d)]),
         "glm_ens_configure": (i, [vp, i, i, i]), "glm_ens_tune": (i, [vp, i, C.POINTER(i), C.POINTER(d)]),
         "glm_step_adjoint": (i, [vp, pp, pp, pp, C.POINTER(GLMParams), vp]),
+        "glm_step_adjoint_passes": (i, [vp, pp, pp, pp, C.POINTER(GLMParams), vp]),
         "glm_adjoint_force_dot": (i, [vp, i, C.POINTER(d)]), "glm_ou_set_afield": (i, [vp, vp]),
     }
     for name, (res, args) in sig.items():

@@ -1,5 +1,7 @@
 // adj.hip -- launches of the reverse step (adj_impl.hpp).
 #include "adj_impl.hpp"
+#include "adj_march.hpp"
+#include "launch.hpp"
 
 namespace glm {
 
@@ -54,8 +56,26 @@ static bool dispatch_faceswise(const AdjArgs& a, dim3 g, hipStream_t st, bool up
     return false;
 }
 
+static bool dispatch_back(const AdjArgs& a, hipStream_t st, int pred, int recon, int slope)
+{
+    int tx, ty, nzc, zc;
+    march_tiling(a.nx, a.ny, a.nz, 0, 256, tx, ty, nzc, zc);   // one workgroup per CU
+    const dim3 g((unsigned)(tx * ty * nzc));
+#define GLM_CASE(P, RC, SL)                                                                          \
+    if (pred == P && recon == RC && slope == SL) {                                                   \
+        hipLaunchKernelGGL((adj_back_kernel<P, RC, SL>), g, dim3(NT), 0, st, a, tx, ty, zc);         \
+        return true;                                                                                 \
+    }
+    GLM_CASE(PRED_FULL, RECON_PLM, SLOPE_MONCEN) GLM_CASE(PRED_FULL, RECON_PLM, SLOPE_MINMOD)
+    GLM_CASE(PRED_FULL, RECON_PPM, SLOPE_MONCEN) GLM_CASE(PRED_FULL, RECON_PPM, SLOPE_MINMOD)
+    GLM_CASE(PRED_ONED, RECON_PLM, SLOPE_MONCEN) GLM_CASE(PRED_ONED, RECON_PLM, SLOPE_MINMOD)
+    GLM_CASE(PRED_ONED, RECON_PPM, SLOPE_MONCEN) GLM_CASE(PRED_ONED, RECON_PPM, SLOPE_MINMOD)
+#undef GLM_CASE
+    return false;
+}
+
 hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int recon, int slope, double* fdot,
-                               hipStream_t st)
+                               bool fused, hipStream_t st)
 {
     if (recon == RECON_PPM_SPIKE) return hipErrorInvalidValue;
     const size_t n = (size_t)a.nx * a.ny * a.nz;
@@ -67,8 +87,12 @@ hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int
     if (!dispatch_faceswise(a, g, st, true, riem)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(adj_fdot_kernel, dim3(1), dim3(ADJ_NT), 0, st, (const double*)a.part, (int)g.x, fdot);
     dispatch_faceswise(a, g, st, false, riem);
-    dispatch_cellwise(a, g, st, false, pred, recon, slope);
-    hipLaunchKernelGGL(adj_gather_kernel, g, dim3(ADJ_NT), 0, st, a);
+    if (fused) {
+        dispatch_back(a, st, pred, recon, slope);
+    } else {
+        dispatch_cellwise(a, g, st, false, pred, recon, slope);
+        hipLaunchKernelGGL(adj_gather_kernel, g, dim3(ADJ_NT), 0, st, a);
+    }
     return hipGetLastError();
 }
 

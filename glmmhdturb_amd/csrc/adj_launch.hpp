@@ -28,10 +28,12 @@ struct AdjArgs {
 // Device scratch of one reverse step over n cells, in bytes (ADJ_NPLANES planes plus the
 // per-block partial sums).
 size_t adj_scratch_bytes(size_t n);
-// Launch the six passes of the reverse step and add the step's sum of (kick acceleration
-// adjoint . acceleration) to *fdot (device).  a.scr and a.part must come from a scratch of
-// adj_scratch_bytes(n).  Returns hipErrorInvalidValue for an unsupported scheme.
+// Launch the reverse step and add the step's sum of (kick acceleration adjoint . acceleration)
+// to *fdot (device).  With fused, the face-state and predictor adjoints and the gather run as one
+// z-march kernel (adj_march.hpp); otherwise as the separate passes 5 and 6.  a.scr and a.part must
+// come from a scratch of adj_scratch_bytes(n).  Returns hipErrorInvalidValue for an unsupported
+// scheme.
 hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int recon, int slope, double* fdot,
-                               hipStream_t st);
+                               bool fused, hipStream_t st);
 
 }  // namespace glm

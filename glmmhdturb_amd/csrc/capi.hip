@@ -335,8 +335,8 @@ int glm_ou_set_afield(glm_ctx* ctx, const void* src)
     return GLM_OK;
 }
 
-int glm_step_adjoint(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
-                     const void* afield)
+static int step_adjoint_impl(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
+                             const void* afield, bool fused)
 {
     if (!ctx || !u0 || !lam1 || !lam0) return set_err(GLM_ERR_ARG, "glm_step_adjoint: null argument");
     int e = check_params(p);
@@ -378,10 +378,22 @@ int glm_step_adjoint(glm_ctx* ctx, void* const* u0, void* const* lam1, void* con
     a.do_turb = p->do_turb;
     a.afield = (const double*)afield;
     hipError_t he = launch_step_adjoint(a, ctx->adj_scratch, p->predictor, p->riemann, p->recon, p->slope, ctx->d_fdot,
-                                        ctx->stream);
+                                        fused, ctx->stream);
     if (he == hipErrorInvalidValue) return set_err(GLM_ERR_ARG, "glm_step_adjoint: unsupported scheme");
     HIPCHK(he);
     return GLM_OK;
+}
+
+int glm_step_adjoint(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
+                     const void* afield)
+{
+    return step_adjoint_impl(ctx, u0, lam1, lam0, p, afield, true);
+}
+
+int glm_step_adjoint_passes(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
+                            const void* afield)
+{
+    return step_adjoint_impl(ctx, u0, lam1, lam0, p, afield, false);
 }
 
 int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out)

@@ -269,14 +269,16 @@ class Solver:
         check(self.lib.glm_ou_set_afield(self.ctx, C.c_void_p(field.data_ptr())))
 
     # ---- adjoint ----------------------------------------------------------------------------
-    def step_adjoint(self, u0, lam1, lam0, args: GLMParams, afield=None):
+    def step_adjoint(self, u0, lam1, lam0, args: GLMParams, afield=None, passes: bool = False):
         """lam0 = (dU1/dU0)^T lam1 for the step `args` taken from u0 (FP64); afield is the forcing
-        field the step used when args.do_turb."""
+        field the step used when args.do_turb.  passes=True runs the back half of the reverse step
+        as separate passes instead of the fused z-march (same results)."""
         qa, k0 = self._planes(u0)
         la, k1 = self._planes(lam1)
         oa, k2 = self._planes(lam0)
         af = None if afield is None else C.c_void_p(afield.data_ptr())
-        check(self.lib.glm_step_adjoint(self.ctx, qa, la, oa, C.byref(args), af))
+        fn = self.lib.glm_step_adjoint_passes if passes else self.lib.glm_step_adjoint
+        check(fn(self.ctx, qa, la, oa, C.byref(args), af))
 
     def adjoint_force_dot(self, reset: bool = True) -> float:
         """Sum over the reverse steps so far of (kick acceleration adjoint) . (kick acceleration)."""

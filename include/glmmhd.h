@@ -171,6 +171,10 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
  * amplitude.  Asynchronous on the context stream. */
 int glm_step_adjoint(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0, const glm_params* p,
                      const void* afield);
+/* The same reverse step with its face-state and predictor adjoints and its gather as separate
+ * passes over contribution planes instead of one z-march kernel: same arguments, same results. */
+int glm_step_adjoint_passes(glm_ctx* ctx, void* const* u0, void* const* lam1, void* const* lam0,
+                            const glm_params* p, const void* afield);
 /* Read (and with reset != 0 clear) the forcing-amplitude accumulator; synchronous. */
 int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out);
 /* Replace the context's forcing field by src (3 g^3 values in the state precision, device),

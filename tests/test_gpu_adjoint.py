@@ -209,3 +209,32 @@ def test_trajectory_gradient_vs_dual_oracle_32():
     print(f"[adjoint dual 32^3] dual {dJ:.15e} adjoint {ad:.15e}", flush=True)
     assert abs(dJ - ad) <= 1e-9 * abs(dJ)
     tr.close()
+
+
+@pytest.mark.parametrize("k", range(len(SCHEMES)))
+def test_fused_back_half_matches_passes(k):
+    """The reverse step's back half as one z-march kernel (glm_step_adjoint) against the same
+    sums as separate passes (glm_step_adjoint_passes), on a grid with partial tiles: the sums run in
+    the same order, so the results agree to the last bits the compiler's FMA choices leave."""
+    cfg = dict(SCHEMES[k])
+    scheme = cfg.pop("scheme")
+    orc = Oracle("f64")
+    nx, ny, nz = 20, 14, 11
+    p = Params(N=nx, nx=nx, ny=ny, nz=nz, gamma=5.0 / 3.0, courant=0.4)
+    s = Solver(p, "f64")
+    u = smooth_state(orc, nx, ny, nz, seed=31 + k)
+    dt = 0.4 * (1.0 / nx) / orc.ctot_max(u, p.gamma, p.smallr, p.smallr * p.smallc ** 2 / p.gamma)
+    a = step_args(p, dt, 0.0, want_cfl=False, scheme=scheme, **cfg)
+    if scheme == "spike":
+        a.riemann = 3
+    w = np.random.default_rng(200 + k).standard_normal(u.shape)
+    out = []
+    for passes in (False, True):
+        lam0 = s.new_state()
+        s.step_adjoint(dev(u), dev(w), lam0, a, passes=passes)
+        out.append(lam0.cpu().numpy())
+    d = np.abs(out[0] - out[1]).max()
+    print(f"[adjoint back half] {SCHEMES[k]}: max |fused - passes| {d:.3e} (max |lam0| {np.abs(out[1]).max():.3e})",
+          flush=True)
+    assert d <= 1e-13 * np.abs(out[1]).max()
+    s.close()

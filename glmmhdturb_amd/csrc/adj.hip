@@ -47,8 +47,12 @@ static bool dispatch_faceswise(const AdjArgs& a, dim3 g, hipStream_t st, bool up
 {
 #define GLM_CASE(R)                                                                                       \
     if (riem == R) {                                                                                      \
-        if (upre) hipLaunchKernelGGL((adj_upre_kernel<R>), g, dim3(ADJ_NT), 0, st, a);                    \
-        else hipLaunchKernelGGL((adj_riemann_kernel<R>), g, dim3(ADJ_NT), 0, st, a);                      \
+        if (upre) {                                                                                       \
+            hipLaunchKernelGGL((adj_flux_kernel<R>), g, dim3(ADJ_NT), 0, st, a);                          \
+            hipLaunchKernelGGL(adj_upre_kernel, g, dim3(ADJ_NT), 0, st, a);                               \
+        } else {                                                                                          \
+            hipLaunchKernelGGL((adj_riemann_kernel<R>), g, dim3(ADJ_NT), 0, st, a);                       \
+        }                                                                                                 \
         return true;                                                                                      \
     }
     GLM_CASE(RIEM_LLF) GLM_CASE(RIEM_LLFB) GLM_CASE(RIEM_HLL) GLM_CASE(RIEM_HLLD)

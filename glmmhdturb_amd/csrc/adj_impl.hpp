@@ -6,8 +6,9 @@
 // (every pass is a stencil over the periodic grid):
 //   1 prim    W = c2p(u0)
 //   2 faces   predictor (full Hancock) and the six face states of every cell
-//   3 upre    the step's flux divergence -> the pre-epilogue state, then the epilogue adjoint
-//             (kick, psi decay, floor): LPRE = adjoint of the pre-epilogue state
+//   3 upre    the flux through every face, each solved once (FLX), then the step's flux
+//             divergence -> the pre-epilogue state and the epilogue adjoint (kick, psi decay,
+//             floor): LPRE = adjoint of the pre-epilogue state
 //   4 riemann per lower face: lambda_F = dtdx (LPRE(c) - LPRE(c - e_d)), Riemann adjoint ->
 //             adjoints of the two face states meeting there (each slot has one writer)
 //   5 cell    face-state adjoints (one launch per direction) and the predictor adjoint of every
@@ -156,7 +157,24 @@ GLM_DEV void face_flux(const AdjArgs& a, size_t n, size_t cl, size_t cr, double 
 }
 
 // ---- pass 3 ----------------------------------------------------------------------------------
+// 3a: the flux through each cell's three lower faces, each face solved once -> FLX (27 planes)
 template <int RIEM>
+__global__ void __launch_bounds__(ADJ_NT) adj_flux_kernel(AdjArgs a)
+{
+    const size_t n = (size_t)a.nx * a.ny * a.nz;
+    for (size_t c = (size_t)blockIdx.x * ADJ_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * ADJ_NT) {
+        const Cell x = cell_of(c, a.nx, a.ny);
+        double f[9];
+        face_flux<RIEM, 0>(a, n, nbr(x, 0, -1, a.nx, a.ny, a.nz), c, f);
+        st9(plane(a, ADJ_FLX, n), n, c, f);
+        face_flux<RIEM, 1>(a, n, nbr(x, 1, -1, a.nx, a.ny, a.nz), c, f);
+        st9(plane(a, ADJ_FLX + 9, n), n, c, f);
+        face_flux<RIEM, 2>(a, n, nbr(x, 2, -1, a.nx, a.ny, a.nz), c, f);
+        st9(plane(a, ADJ_FLX + 18, n), n, c, f);
+    }
+}
+
+// 3b: flux divergence from FLX, then the epilogue adjoint
 __global__ void __launch_bounds__(ADJ_NT) adj_upre_kernel(AdjArgs a)
 {
     __shared__ double red[ADJ_NT];
@@ -166,16 +184,16 @@ __global__ void __launch_bounds__(ADJ_NT) adj_upre_kernel(AdjArgs a)
     for (size_t c = (size_t)blockIdx.x * ADJ_NT + threadIdx.x; c < n; c += (size_t)gridDim.x * ADJ_NT) {
         const Cell x = cell_of(c, a.nx, a.ny);
         double acc[9], fl[9], fh[9];
-        face_flux<RIEM, 0>(a, n, nbr(x, 0, -1, a.nx, a.ny, a.nz), c, fl);
-        face_flux<RIEM, 0>(a, n, c, nbr(x, 0, +1, a.nx, a.ny, a.nz), fh);
+        ld9(plane(a, ADJ_FLX, n), n, c, fl);
+        ld9(plane(a, ADJ_FLX, n), n, nbr(x, 0, +1, a.nx, a.ny, a.nz), fh);
 #pragma unroll
         for (int v = 0; v < 9; ++v) acc[v] = fl[v] - fh[v];
-        face_flux<RIEM, 1>(a, n, nbr(x, 1, -1, a.nx, a.ny, a.nz), c, fl);
-        face_flux<RIEM, 1>(a, n, c, nbr(x, 1, +1, a.nx, a.ny, a.nz), fh);
+        ld9(plane(a, ADJ_FLX + 9, n), n, c, fl);
+        ld9(plane(a, ADJ_FLX + 9, n), n, nbr(x, 1, +1, a.nx, a.ny, a.nz), fh);
 #pragma unroll
         for (int v = 0; v < 9; ++v) acc[v] += fl[v] - fh[v];
-        face_flux<RIEM, 2>(a, n, nbr(x, 2, -1, a.nx, a.ny, a.nz), c, fl);
-        face_flux<RIEM, 2>(a, n, c, nbr(x, 2, +1, a.nx, a.ny, a.nz), fh);
+        ld9(plane(a, ADJ_FLX + 18, n), n, c, fl);
+        ld9(plane(a, ADJ_FLX + 18, n), n, nbr(x, 2, +1, a.nx, a.ny, a.nz), fh);
         double r[9], lam[9], rb[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) {

@@ -9,7 +9,7 @@ constexpr int ADJ_NT = 256;
 
 // Scratch planes of the reverse step (n doubles each).
 constexpr int ADJ_W = 0, ADJ_MH = 9, ADJ_FS = 18, ADJ_LPRE = 72, ADJ_LFS = 81, ADJ_CON = 135, ADJ_MHB = 198,
-              ADJ_NPLANES = 207;
+              ADJ_FLX = 207, ADJ_NPLANES = 234;   // ADJ_FLX: the flux through each cell's three lower faces
 
 struct AdjArgs {
     const double* u0[9];

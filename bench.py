@@ -193,18 +193,43 @@ def measured_traffic(N):
     return None
 
 
-def cpu_baseline(seconds):
+def host_layout():
+    """CPU count and NUMA nodes of the host the CPU legs run on."""
+    nodes = []
+    base = "/sys/devices/system/node"
+    if os.path.isdir(base):
+        nodes = sorted(d for d in os.listdir(base) if d.startswith("node") and d[4:].isdigit())
+    return {"cpus": os.cpu_count(), "numa_nodes": len(nodes) or None,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def cpu_baseline(seconds, extra_seconds=5.0):
     """The oracle (C restatement, OpenMP) on a bounded sample of the same workload: 128^3, same
-    IC, forcing and scheme, as many steps as fit in about `seconds`."""
+    IC, forcing and scheme, as many steps as fit in about `seconds`.  Two shorter legs beside it:
+    the same scheme with an FP32 state, and the literal cpu_simd.jl scheme (full Hancock, MonCen,
+    LLF; cpu_simd.jl:62-101) in FP64."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    main = _cpu_leg(seconds, "f64", "ppm", "hlld")
+    main["cores"] = threads
+    main["host"] = host_layout()
+    main["other_legs"] = {
+        "fp32_same_scheme": _cpu_leg(extra_seconds, "f32", "ppm", "hlld"),
+        "fp64_cpu_simd_scheme": _cpu_leg(extra_seconds, "f64", "plm", "llf"),
+    }
+    return main
+
+
+def _cpu_leg(seconds, prec, recon, riemann):
     from oracle import OUForcing, Oracle, Scheme
     from glmmhdturb_amd import glm_speed, pfloor, ramp_of
 
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
     M = 128
     p = make_params(M)
-    orc = Oracle("f64")
+    orc = Oracle(prec)
     u = orc.ic_uniform(M, M, M, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma)
     fo = OUForcing(orc, g=64, kmin=1.0, kmax=3.0, T=0.5, ndt=100, comp_frac=0.0, amp=AMP, seed=1)
     fo.advance()
@@ -217,15 +242,17 @@ def cpu_baseline(seconds):
         dt = min(p.courant * (1.0 / M) / cm, fo.next_time - t)
         ch, fac = glm_speed(p, dt)
         sc = Scheme(gamma=p.gamma, smallr=p.smallr, smallc=p.smallc, dt=dt, dx=1.0 / M, ch=ch, glm_fac=fac,
-                    turb_min_rho=p.turb_min_rho, ramp=ramp_of(p, t), predictor="full", recon="ppm",
-                    riemann="hlld", mh_revert=True, face_guard="iflo", update_floor=True, do_turb=True, turb_gs=64)
+                    turb_min_rho=p.turb_min_rho, ramp=ramp_of(p, t), predictor="full", recon=recon,
+                    riemann=riemann, slope="moncen", mh_revert=True, face_guard="iflo", update_floor=True,
+                    do_turb=True, turb_gs=64)
         u = orc.step(u, sc, afield=fo.afield)
         busy += time.perf_counter() - t0
         t += dt
         steps += 1
+    scheme = "PPM + full Hancock + HLLD" if recon == "ppm" else "cpu_simd.jl scheme: PLM MonCen + full Hancock + LLF"
     return dict(value=M ** 3 * steps / busy, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"oracle C restatement (OpenMP, {threads} threads), 128^3 of the same workload, "
-                       f"{steps} steps incl. the CFL reduction, {busy:.1f} s")
+                sample=f"oracle C restatement ({prec.upper()}, OpenMP, {threads} threads), 128^3 of the same "
+                       f"workload ({scheme}), {steps} steps incl. the CFL reduction, {busy:.1f} s")
 
 
 def pct(xs, q):

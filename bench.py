@@ -21,6 +21,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s + % HBM peak, 256^3 HLLD FP64 vs FP32; adjoint/forward cost"
 HBM_PEAK = 8.0e12          # MI355X HBM3E, B/s
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, TFLOP/s
 BYTES_PER_CELL = {"f64": 144, "f32": 72}   # 9 conserved variables, one read + one write
 AMP = 1.0e5                # OU amplitude (frozen)
 
@@ -191,6 +192,26 @@ def measured_traffic(N):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1])
     return None
+
+
+def profiled_alu(N):
+    """FP64 operations per cell-update and the VALU busy fraction of the FP64 march kernel from the
+    committed PMC summaries (FLOP counts: every lane counted, FMA as 2)."""
+    if N != 256:
+        return None, None
+    fpc = busy = None
+    path = os.path.join(ROOT, "profiles", "march_256_ou_flops.txt")
+    if os.path.exists(path):
+        for line in open(path):
+            if "FP64 FLOPs per launch" in line:
+                fpc = float(line.split("=")[1].split("per cell")[0])
+                break
+    path = os.path.join(ROOT, "profiles", "march_fp64_256_ou_valu.txt")
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("VALU busy"):
+                busy = float(line.split("=")[-1].strip().rstrip("%")) / 100.0
+    return fpc, busy
 
 
 def host_layout():
@@ -522,6 +543,13 @@ def main():
                                          "x1.0 calibrated on a 1 GiB elementwise copy)",
                        "kernel": "glm::march_kernel<double, PRED_FULL, RIEM_HLLD>",
                        "kernel_ms": r64["kernel_ms"], "bytes_per_cell": BYTES_PER_CELL["f64"]}
+    fpc, busy = profiled_alu(N)
+    if fpc is not None:
+        tf = fpc * N ** 3 / (r64["kernel_ms"] * 1e-3) / 1e12
+        rec["alu"] = {"flops_per_cell": fpc, "achieved_tflops": tf, "peak_tflops": FP64_PEAK_TFLOPS,
+                      "frac": tf / FP64_PEAK_TFLOPS, "valu_busy": busy,
+                      "source": "profiles/march_256_ou_flops.txt and profiles/march_fp64_256_ou_valu.txt "
+                                "(PMC counts of the same kernel; rate from this run's kernel time)"}
     rec["state"] = {"t": r64["t"], "mach": r64["mach"], "divbmax": r64["divbmax"], "finite": r64["finite"]}
     if not args.no_fp32:
         r32 = run_gpu(N, args.steps, args.warmup, "f32")
@@ -532,7 +560,7 @@ def main():
                        "steps_compared": args.steps + args.warmup,
                        "diag_rel_err_vs_fp64_from_rest": rel,
                        "diag_rel_err_vs_fp64_gradient_ic_100_steps": fp32_developed(N, 100),
-                       "note": "FP32 state and fluxes; FP64 pressure recovery, CFL speed and diagnostics"}
+                       "note": "FP32 state and fluxes; FP64 pressure recovery, CFL max and diagnostics"}
     if not args.no_ct:
         rec["ct_precision"] = run_ct_precision(args.ct_N, args.ct_steps)
     if not args.no_adjoint:

@@ -193,6 +193,32 @@ def test_orszag_tang_multistep(orc):
     assert np.all(err <= TOL64), err
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("n,recon,riem,vrms,divb,rhomin", [
+    (64, "plm", "hlld", 0.7887, 2.9, None), (128, "plm", "hlld", 0.7906, 4.5, None),
+    (120, "ppm", "hll", 0.7910, 1.16, 0.0835)])
+def test_orszag_tang_reference_figures(orc, prec, n, recon, riem, vrms, divb, rhomin):
+    """The reference's own validation runs (validate_tiled_ot glmmhd_turb.jl:2278-2295: OT on an n^3
+    grid, courant 0.7, t_end 0.2, a CFL dt each step) through the GPU step and its diagnostics.  Its
+    printed figures (REPORT.md:24-26, BENCHMARKS.md:178; fp32 runs) hold to their digits in both
+    precisions: vrms within half a printed unit plus fp32 noise, max|div B| within 1%, rho_min to its
+    digits.  In FP64 the 3-D run also equals the oracle's n x n x 1 run (the state is z-invariant)."""
+    from glmmhdturb_amd.turb import run_ot
+    from test_oracle_physics import ot_oracle_run
+    d, u, s = run_ot(n, 0.2, riemann=riem, recon=recon, scheme="cube", courant=0.7, precision=prec, verbose=False)
+    s.close()
+    assert abs(d["vrms"] - vrms) <= 6e-5, d["vrms"]
+    assert d["divbmax"] == pytest.approx(divb, rel=0.01)
+    if rhomin is not None:
+        assert abs(d["rhomin"] - rhomin) <= 6e-5, d["rhomin"]
+    assert d["finite"]
+    if prec == "f64":
+        do = ot_oracle_run(orc, n, recon, riem)
+        assert d["vrms"] == pytest.approx(do["vrms"], rel=1e-10)
+        assert d["rhomin"] == pytest.approx(do["rhomin"], rel=1e-10)
+        assert d["divbmax"] == pytest.approx(do["divbmax"], rel=1e-6)
+
+
 def test_forced_turbulence_parity(orc):
     """OU-forced turbulence 24^3: forcing field and 12 steps (PPM + HLLD, kick on)."""
     N = 24

@@ -478,6 +478,44 @@ def test_ctot_and_diagnostics_brute_force(orc):
     assert d["divbmax"] == pytest.approx(np.abs(div).max(), rel=1e-14)
 
 
+# Orszag-Tang figures the reference prints from its own runs (fp32, cube integrators, gamma 5/3,
+# the Params defaults glmmhd_turb.jl:28-52 with courant 0.7, t_end 0.2 as in validate_tiled_ot
+# :2278-2295): vrms and max|div B| of PLM + HLLD at 64^3 and 128^3 (REPORT.md:24-26), and vrms, max|div B|
+# and rho_min of PPM + HLL at N = 120 (BENCHMARKS.md:178).  (N, recon, riemann, vrms, divbmax, rhomin)
+OT_ANCHORS = [(64, "plm", "hlld", 0.7887, 2.9, None),
+              (128, "plm", "hlld", 0.7906, 4.5, None),
+              (120, "ppm", "hll", 0.7910, 1.16, 0.0835)]
+
+
+def ot_oracle_run(orc, n, recon, riem, courant=0.7, t_end=0.2):
+    """The reference's validation run on the oracle: the z-invariant OT state as n x n x 1, a CFL dt
+    each step (compute_dt :1973-1978), ch and the psi decay from the dt (:884-885)."""
+    u = orc.ic_ot(n, n, 1, G)
+    dx, t = 1.0 / n, 0.0
+    while t < t_end:
+        dt = min(courant * dx / orc.ctot_max(u, G, SMALLR, PFL), t_end - t)
+        ch = courant * dx / dt / 3 * 0.25
+        sc = Scheme(gamma=G, dt=dt, dx=dx, ch=ch, glm_fac=math.exp(-ch * dt / 0.18), recon=recon, riemann=riem)
+        u = orc.step(u, sc)
+        t += dt
+    d = orc.diagnostics(u, dx, SMALLR)
+    d["vrms"] = math.sqrt(d["ke2"] / d["mass"])
+    return d
+
+
+@pytest.mark.parametrize("n,recon,riem,vrms,divb,rhomin", OT_ANCHORS)
+def test_orszag_tang_reference_figures(orc, n, recon, riem, vrms, divb, rhomin):
+    """The whole FP64 step reproduces the reference's printed validation figures: vrms to its 4
+    printed digits (half a unit plus fp32 noise), the max of the central-difference div B (a max of
+    difference quotients of an fp32 run) within 1%, rho_min to its digits."""
+    d = ot_oracle_run(orc, n, recon, riem)
+    assert abs(d["vrms"] - vrms) <= 6e-5, d["vrms"]
+    assert d["divbmax"] == pytest.approx(divb, rel=0.01)
+    if rhomin is not None:
+        assert abs(d["rhomin"] - rhomin) <= 6e-5, d["rhomin"]
+    assert d["finite"] == 1.0
+
+
 def test_orszag_tang_ic_closed_form(orc):
     n = 16
     u = orc.ic_ot(n, n, 2, G)

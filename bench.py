@@ -156,11 +156,28 @@ def diag_errors(d64, d32):
     return rel
 
 
+def _forced_steps(p, s, f, u1, u2, t, cmax, *, nsteps=None, t_end=None):
+    """The benchmark's forced step loop (OU advance cadence, CFL dt folded into the previous step)
+    for `nsteps` steps or up to `t_end`; returns the state pair, the time and the next CFL speed."""
+    from glmmhdturb_amd import dxof, step_args
+    n = 0
+    while (nsteps is not None and n < nsteps) or (t_end is not None and t < t_end):
+        while t >= f.next_time:
+            f.advance()
+        dt = min(p.courant * dxof(p) / cmax, f.next_time - t)
+        s.step(u1, u2, step_args(p, dt, t, do_turb=True, riemann="hlld", recon="ppm", want_cfl=True))
+        u1, u2 = u2, u1
+        t += dt
+        n += 1
+        cmax = s.cfl_max()
+    return u1, u2, t, cmax
+
+
 def fp32_developed(N, steps):
     """FP32 against FP64 on a developed state: the gradient IC (spike_mhd.cu:304-317) with the OU kick
     on, `steps` steps, each precision with its own CFL dt."""
     import torch
-    from glmmhdturb_amd import Solver, dxof, step_args
+    from glmmhdturb_amd import Solver
     p = make_params(N)
     out = {}
     for prec in ("f64", "f32"):
@@ -168,19 +185,42 @@ def fp32_developed(N, steps):
         u1, u2 = s.new_state(), s.new_state()
         s.ic_gradient(u1, p.gamma)
         f = s.setup_forcing(p, AMP)
-        t, cmax = 0.0, s.ctot_max(u1)
-        for _ in range(steps):
-            while t >= f.next_time:
-                f.advance()
-            dt = min(p.courant * dxof(p) / cmax, f.next_time - t)
-            s.step(u1, u2, step_args(p, dt, t, do_turb=True, riemann="hlld", recon="ppm", want_cfl=True))
-            u1, u2 = u2, u1
-            t += dt
-            cmax = s.cfl_max()
+        u1, u2, _, _ = _forced_steps(p, s, f, u1, u2, 0.0, s.ctot_max(u1), nsteps=steps)
         torch.cuda.synchronize()
         out[prec] = s.diagnostics(u1)
         s.close()
     return diag_errors(out["f64"], out["f32"])
+
+
+def fp32_developed_turbulence(N, t_dev, steps):
+    """FP32 against FP64 on the benchmark's own turbulence once it is developed: the FP64 run from
+    rest to t_dev (the rms Mach number has saturated there), then `steps` steps in each precision
+    from that state (rounded to FP32 for the FP32 run), the forcing advanced alike."""
+    import torch
+    from glmmhdturb_amd import Solver
+    p = make_params(N)
+    s64 = Solver(p, "f64")
+    u1, u2 = s64.new_state(), s64.new_state()
+    s64.ic_uniform(u1, p.rho0, 1.0, (p.b0, 0.0, 0.0), p.gamma)
+    f64 = s64.setup_forcing(p, AMP)
+    f64.advance()
+    u1, u2, t, cmax = _forced_steps(p, s64, f64, u1, u2, 0.0, s64.ctot_max(u1), t_end=t_dev)
+    d0 = s64.diagnostics(u1)
+    s32 = Solver(p, "f32")
+    v1, v2 = s32.new_state(), s32.new_state()
+    v1.copy_(u1.float())
+    f32 = s32.setup_forcing(p, AMP)
+    for _ in range(f64.nsteps):
+        f32.advance()
+    t0 = t
+    u1, u2, t, _ = _forced_steps(p, s64, f64, u1, u2, t0, cmax, nsteps=steps)
+    v1, v2, _, _ = _forced_steps(p, s32, f32, v1, v2, t0, s32.ctot_max(v1), nsteps=steps)
+    torch.cuda.synchronize()
+    rel = diag_errors(s64.diagnostics(u1), s32.diagnostics(v1))
+    rel.update(t_start=t0, mach_start=d0["mach"], steps=steps)
+    s64.close()
+    s32.close()
+    return rel
 
 
 def measured_traffic(N):
@@ -509,6 +549,11 @@ def run_adjoint(N, nsteps, window=50, snapshots=16):
     return out
 
 
+def note(msg):
+    """A progress line on stderr (the JSON record is the only line on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -519,6 +564,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     N = args.N
     r64 = run_gpu(N, args.steps, args.warmup, "f64")
+    note(f"fp64 {N}^3: {r64['kernel_ms']:.3f} ms per march launch")
     rec = {
         "metric": METRIC,
         "value": r64["value"],
@@ -553,6 +599,7 @@ def main():
     rec["state"] = {"t": r64["t"], "mach": r64["mach"], "divbmax": r64["divbmax"], "finite": r64["finite"]}
     if not args.no_fp32:
         r32 = run_gpu(N, args.steps, args.warmup, "f32")
+        note(f"fp32 {N}^3: {r32['kernel_ms']:.3f} ms per march launch")
         ach32 = BYTES_PER_CELL["f32"] * N ** 3 / (r32["kernel_ms"] * 1e-3)
         rel = diag_errors(r64["diag"], r32["diag"])
         rec["fp32"] = {"value": r32["value"], "ms_per_step": r32["ms_per_step"], "kernel_ms": r32["kernel_ms"],
@@ -560,18 +607,23 @@ def main():
                        "steps_compared": args.steps + args.warmup,
                        "diag_rel_err_vs_fp64_from_rest": rel,
                        "diag_rel_err_vs_fp64_gradient_ic_100_steps": fp32_developed(N, 100),
+                       "diag_rel_err_vs_fp64_developed_turbulence_100_steps": fp32_developed_turbulence(N, 1.0, 100),
                        "note": "FP32 state and fluxes; FP64 pressure recovery, CFL max and diagnostics"}
     if not args.no_ct:
+        note("fp32 diagnostics compared; constrained transport next")
         rec["ct_precision"] = run_ct_precision(args.ct_N, args.ct_steps)
     if not args.no_adjoint:
+        note("adjoint next")
         rec["adjoint"] = run_adjoint(args.adj_N, args.adj_steps)
     if not args.no_ensemble:
+        note("ensemble next")
         rec["ensemble"] = run_ensemble(args.steps, args.warmup, args.ens_boxes)
     if not args.no_gradient:
         rec["gradient_ic_fp64"] = run_gradient_ic(N, max(5, args.steps // 3), "f64")
     if args.copy_calib:
         rec["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
+        note("CPU baseline next")
         rec["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(rec))
 

@@ -251,8 +251,9 @@ extern "C" {
 
 int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
 {
-    if (!ctx || !q || !cmax) return set_err(GLM_ERR_ARG, "glm_ctot_max: null argument");
-    int e = check_params(p);
+    if (!ctx || !cmax) return set_err(GLM_ERR_ARG, "glm_ctot_max: null argument");
+    int e = check_planes(q, 9, "glm_ctot_max: null plane pointer");
+    if (!e) e = check_params(p);
     if (e) return e;
     HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
     if (ctx->prec == 8) HIPCHK(launch_ctot<double>(cptr9<double>(q), ctx->n, make_phys<double>(p), ctx->d_cmax + 3, ctx->stream));
@@ -266,8 +267,9 @@ int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax
 
 int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* out)
 {
-    if (!ctx || !q || !out) return set_err(GLM_ERR_ARG, "glm_diagnostics: null argument");
-    int e = check_params(p);
+    if (!ctx || !out) return set_err(GLM_ERR_ARG, "glm_diagnostics: null argument");
+    int e = check_planes(q, 9, "glm_diagnostics: null plane pointer");
+    if (!e) e = check_params(p);
     if (e) return e;
     if (ctx->prec == 8)
         HIPCHK(launch_diag<double>(cptr9<double>(q), ctx->nx, ctx->ny, ctx->nz, p->dx, p->smallr, ctx->d_parts, ctx->d_diag, ctx->stream));
@@ -280,7 +282,8 @@ int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* o
 
 int glm_ic(glm_ctx* ctx, void* const* q, int kind, const double* args)
 {
-    if (!ctx || !q || !args) return set_err(GLM_ERR_ARG, "glm_ic: null argument");
+    if (!ctx || !args) return set_err(GLM_ERR_ARG, "glm_ic: null argument");
+    if (int e = check_planes(q, 9, "glm_ic: null plane pointer")) return e;
     if (kind < 0 || kind > 2) return set_err(GLM_ERR_ARG, "glm_ic: unknown kind");
     if (ctx->prec == 8) HIPCHK(launch_ic<double>(ptr9<double>(q), ctx->nx, ctx->ny, ctx->nz, kind, args, ctx->stream));
     else HIPCHK(launch_ic<float>(ptr9<float>(q), ctx->nx, ctx->ny, ctx->nz, kind, args, ctx->stream));
@@ -465,8 +468,9 @@ int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_p
 
 int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
 {
-    if (!ctx || !q || !cmax) return set_err(GLM_ERR_ARG, "glm_ct_ctot_max: null argument");
-    int e = check_params(p);
+    if (!ctx || !cmax) return set_err(GLM_ERR_ARG, "glm_ct_ctot_max: null argument");
+    int e = check_planes(q, 8, "glm_ct_ctot_max: null plane pointer");
+    if (!e) e = check_params(p);
     if (e) return e;
     double smallp = p->pfl > 0.0 ? p->pfl : p->smallr * p->smallc * p->smallc / p->gamma;
     HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
@@ -483,7 +487,8 @@ int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* c
 
 int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out)
 {
-    if (!ctx || !q || !out || !(dx > 0.0)) return set_err(GLM_ERR_ARG, "glm_ct_divb_max: bad argument");
+    if (!ctx || !out || !(dx > 0.0)) return set_err(GLM_ERR_ARG, "glm_ct_divb_max: bad argument");
+    if (int e = check_planes(q, 8, "glm_ct_divb_max: null plane pointer")) return e;
     HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
     if (ctx->prec == 8) HIPCHK(launch_ct_divb<double>(q, ctx->nx, ctx->ny, ctx->nz, dx, ctx->d_cmax + 3, ctx->stream));
     else HIPCHK(launch_ct_divb<float>(q, ctx->nx, ctx->ny, ctx->nz, dx, ctx->d_cmax + 3, ctx->stream));

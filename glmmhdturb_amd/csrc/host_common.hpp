@@ -34,6 +34,15 @@ inline Phys<T> make_phys(const glm_params* p)
     return s;
 }
 
+// Every one of the nv plane addresses is set (a kernel would otherwise fault on the null plane).
+inline int check_planes(void* const* q, int nv, const char* who)
+{
+    if (!q) return set_err(GLM_ERR_ARG, who);
+    for (int v = 0; v < nv; ++v)
+        if (!q[v]) return set_err(GLM_ERR_ARG, who);
+    return GLM_OK;
+}
+
 inline int check_params(const glm_params* p)
 {
     if (!p) return set_err(GLM_ERR_ARG, "null glm_params");

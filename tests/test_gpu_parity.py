@@ -219,6 +219,55 @@ def test_orszag_tang_reference_figures(orc, prec, n, recon, riem, vrms, divb, rh
         assert d["divbmax"] == pytest.approx(do["divbmax"], rel=1e-6)
 
 
+def brio_wu(n, axis, gamma=2.0):
+    """Brio & Wu's shock tube (rho, p, B_t) = (1, 1, 1) | (0.125, 0.1, -1), B_n = 0.75, on a periodic
+    line of n cells along `axis` (the grid is 1 cell wide across it), as a conserved state."""
+    shape = [1, 1, 1]
+    shape[2 - axis] = n                       # state order (var, z, y, x)
+    left = (np.arange(n) < n // 2).reshape(shape)
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    bt = np.where(left, 1.0, -1.0)
+    u = np.zeros((9, *shape))
+    u[0] = rho
+    u[5 + axis] = 0.75
+    u[5 + (axis + 1) % 3] = bt
+    u[4] = p / (gamma - 1.0) + 0.5 * (0.75 ** 2 + bt ** 2)
+    return u
+
+
+@pytest.mark.parametrize("riem,recon", [("hlld", "ppm"), ("hll", "plm"), ("llf", "plm")])
+def test_brio_wu_tube_along_each_axis(orc, riem, recon):
+    """Brio-Wu tube (shocks, a compound wave, a contact) along x, y and z on 1-cell-wide grids: 80
+    steps at the oracle's CFL dt against the oracle, and the three directions against each other
+    (the tube along y and z is the x tube with its components cycled)."""
+    n, res = 128, []
+    for axis in range(3):
+        shape = [1, 1, 1]
+        shape[axis] = n
+        p = Params(N=n, nx=shape[0], ny=shape[1], nz=shape[2], gamma=2.0, courant=0.4)
+        s = Solver(p, "f64")
+        uo = brio_wu(n, axis)
+        ud, un = to_dev(s, uo), s.new_state()
+        for it in range(80):
+            dt = p.courant * (1.0 / n) / orc.ctot_max(uo, p.gamma, p.smallr, pfloor(p))
+            a = step_args(p, dt, 0.0, riemann=riem, recon=recon, want_cfl=False)
+            s.step(ud, un, a)
+            ud, un = un, ud
+            uo = orc.step(uo, oracle_scheme(a))
+        ug = to_np(ud)
+        err = rel_l1(ug, uo, ch=a.ch)
+        assert np.all(err <= TOL64), (axis, err)
+        assert ug[0].min() < 0.3 and ug[0].max() > 0.9          # the tube has not smeared out
+        perm = [0, 1 + axis, 1 + (axis + 1) % 3, 1 + (axis + 2) % 3, 4,
+                5 + axis, 5 + (axis + 1) % 3, 5 + (axis + 2) % 3, 8]
+        res.append(ug[perm].reshape(9, -1))                     # back to the x tube's components
+        s.close()
+    for axis in (1, 2):
+        err = rel_l1(res[axis], res[0], ch=a.ch)
+        assert np.all(err <= 1e-13), (axis, err)
+
+
 def test_forced_turbulence_parity(orc):
     """OU-forced turbulence 24^3: forcing field and 12 steps (PPM + HLLD, kick on)."""
     N = 24

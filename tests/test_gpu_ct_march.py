@@ -4,6 +4,8 @@ against the CPU oracle and against the multi-pass CT step (glm_ct_step_passes).
 Bars: FP64 relative L1 <= 1e-12 per variable against the oracle (reduction order and FMA
 contraction only); fused vs multi-pass FP64 <= 1e-13 (same point functions, different kernels);
 FP32 fused vs multi-pass <= 1e-6.  Face div B stays at round-off."""
+import math
+
 import numpy as np
 import pytest
 
@@ -146,6 +148,45 @@ def test_ct_march_matches_passes(orc, prec, tol, riem, recon):
     err = rel_l1(out[0], out[1], 8)
     assert np.all(err <= tol), err
     s.close()
+
+
+def test_ct_march_largest_grid_fp32():
+    """1300^3 FP32 CT (two 8-plane states: 141 GB of HBM; 2.2e9 cells per plane, past any 32-bit
+    index).  A uniform state stays exactly uniform; from a density/pressure wave in a uniform field
+    one fused step conserves mass and energy to half an FP32 ulp per cell and keeps face div B at
+    the rounding of the six face values around a cell."""
+    N = 1300
+    if torch.cuda.get_device_properties(0).total_memory < 200 * 2 ** 30:
+        pytest.skip("needs a GPU with >= 200 GiB")
+    torch.cuda.empty_cache()
+    p = Params(N=N, gamma=5.0 / 3.0)
+    s = Solver(p, "f32")
+    q, qn = s.new_state(8), s.new_state(8)
+    assert q[0].numel() > 2 ** 31
+    b = (0.3, 0.2, 0.5)
+    q[0].fill_(1.0); q[1:4].zero_()
+    q[5].fill_(b[0]); q[6].fill_(b[1]); q[7].fill_(b[2])
+    q[4].fill_(1.0 / (p.gamma - 1.0) + 0.5 * sum(c * c for c in b))
+    dx = 1.0 / N
+    a = ct_args(p, 1e-4, "hlld", "plm")
+    s.ct_step(q, qn, a, 0)
+    for v in range(8):
+        assert torch.equal(q[v], qn[v]), v
+    x = (torch.arange(N, device=q.device, dtype=torch.float32) + 0.5) / N
+    w = torch.sin(2 * math.pi * x)
+    q[0].copy_((1.0 + 0.2 * w.view(N, 1, 1) * w.view(1, 1, N)).expand(N, N, N))
+    q[4].copy_(((1.0 + 0.1 * w.view(1, N, 1)) / (p.gamma - 1.0) + 0.5 * sum(c * c for c in b)).expand(N, N, N))
+    tot0 = [float(q[v].sum(dtype=torch.float64)) for v in (0, 4)]
+    dt = 0.3 * dx / s.ct_ctot_max(q)
+    s.ct_step(q, qn, ct_args(p, dt, "hlld", "plm"), 0)
+    tot1 = [float(qn[v].sum(dtype=torch.float64)) for v in (0, 4)]
+    for t0, t1 in zip(tot0, tot1):
+        assert abs(t1 - t0) <= 3e-8 * abs(t0), (t0, t1)
+    assert s.ct_divb_max(qn, dx) <= 6 * 2 ** -24 * max(b) * 1.05 / dx
+    assert bool(torch.isfinite(qn[0]).all())
+    s.close()
+    del q, qn
+    torch.cuda.empty_cache()
 
 
 def test_ct_march_cfl_and_divb_large():

@@ -20,9 +20,12 @@
 //     a two-plane LDS ring); (6) the prefetch and publish of plane q+1.
 //   * HBM traffic: one read and one write of the 8 planes per cell and step, plus the halo and
 //     the update base read again from L2.  The CFL of the result is a separate reduction.
+//   * The OU kick's interpolation is cached per column in LDS (force.hpp): its corner loads happen
+//     only when the column crosses a forcing plane.
 #pragma once
 #include "ct_impl.hpp"
 #include "ct_launch.hpp"
+#include "force.hpp"
 #include "gpu_util.hpp"
 
 namespace glm {
@@ -33,6 +36,8 @@ constexpr int CM_NH = 2 * (CM_OX + 2) + 2 * CM_OY;             // one-cell ring 
 constexpr int CM_NT = CM_NO + CM_NH;                           // 252
 constexpr int CM_PX = CM_OX + 4, CM_PY = CM_OY + 4;            // LDS plane (two-cell ring)
 constexpr int CM_NPL = CM_PX * CM_PY;
+constexpr int CM_LX = CM_OX + 2;                               // row of the lanes' grown tile
+static_assert(CM_LX * (CM_OY + 2) == CM_OX * CM_OY + 2 * (CM_OX + 2) + 2 * CM_OY, "one lane per grown-tile cell");
 constexpr int CM_NHELP = 2 * (CM_OX + 2) + 2 * (CM_OY + 2);    // two-cell ring without its corners
 static_assert(CM_NT <= 256 && CM_NT >= CM_NHELP && CM_NO % 64 == 0, "CT tile shape");
 static_assert(CM_NT == CT_MARCH_THREADS, "launch width of the CT march");
@@ -76,12 +81,15 @@ GLM_DEV void cm_helper_cell(int h, int& px, int& py)
     else { px = CM_PX - 1; py = 1 + h - 2 * (CM_OX + 2) - (CM_OY + 2); }
 }
 
+// The exchange buffers XB/YB hold one slot per lane (the tile grown by one cell, row CM_LX); the
+// primitive plane and the EMF ring cover the tile grown by two cells (row CM_PX).
 template <typename T>
 struct CTMarchLDS {
     T P[8][CM_NPL];        // primitives of plane q (two-cell ring)
-    T XB[8][CM_NPL];       // +x face states, then x fluxes (0..4, By in 6) and the z flux's Bx (5), By (7)
-    T YB[8][CM_NPL];       // +y face states, then y fluxes (0..4, Bx in 5)
+    T XB[8][CM_NT];        // +x face states, then x fluxes (0..4, By in 6) and the z flux's Bx (5), By (7)
+    T YB[8][CM_NT];        // +y face states, then y fluxes (0..4, Bx in 5)
     T E[2][3][CM_NPL];     // edge EMFs ex, ey, ez of the last two planes (plane k in slot k & 1)
+    T FCB[6][CM_NT];       // forcing interpolation cache (ForceCache)
 };
 
 // Primitives with cell-centred B of cell (x, y, z) (ct_mhd.jl:26-34, :85-96).
@@ -101,33 +109,6 @@ GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[8])
     T vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
     T pr = ct_pressure(u[4], rho, vx, vy, vz, bx, by, bz, a.gamma, a.smallp);
     p[0] = rho; p[1] = vx; p[2] = vy; p[3] = vz; p[4] = pr; p[5] = bx; p[6] = by; p[7] = bz;
-}
-
-// OU acceleration at cell (i, j, k) (ct_impl.hpp ct_update_kernel; glmmhd_turb.jl:483-505).
-template <typename T>
-GLM_DEV void cm_force(const CTMArgs<T>& a, int i, int j, int k, T acc[3])
-{
-    const int g = a.tgs;
-    const T gg = T(g);
-    T cx = ((T(i) + T(0.5)) * a.dx) / a.lx * gg, cy = ((T(j) + T(0.5)) * a.dx) / a.ly * gg;
-    T cz = ((T(k) + T(0.5)) * a.dx) / a.lz * gg;
-    T fx = floor(cx), fy = floor(cy), fz = floor(cz);
-    int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-    T wx = cx - fx, wy = cy - fy, wz = cz - fz;
-    int i0w = cwrap(i0, g), i1w = cwrap(i0 + 1, g), j0w = cwrap(j0, g), j1w = cwrap(j0 + 1, g);
-    int k0w = cwrap(k0, g), k1w = cwrap(k0 + 1, g);
-    size_t g3 = (size_t)g * g * g;
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-        const T* af = a.afield + cc * g3;
-        auto at = [&](int x, int y, int z) { return gload(af + ((size_t)x + (size_t)g * ((size_t)y + (size_t)g * z))); };
-        T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
-                (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
-               ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
-                (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
-        acc[cc] = a.ramp * t;
-        __asm__ volatile("" ::: "memory");   // one component's eight corner loads in flight at a time
-    }
 }
 
 // Tiles and z-chunks of the CT march on `slots` concurrent workgroups: rounds x (chunk + 2 lead-in
@@ -158,8 +139,8 @@ ct_march_kernel(CTMArgs<T> a)
 {
     __shared__ CTMarchLDS<T> L;
     T (&P)[8][CM_NPL] = L.P;
-    T (&XB)[8][CM_NPL] = L.XB;
-    T (&YB)[8][CM_NPL] = L.YB;
+    T (&XB)[8][CM_NT] = L.XB;
+    T (&YB)[8][CM_NT] = L.YB;
     const int tid = threadIdx.x;
     int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
     int tx, ty;
@@ -182,6 +163,7 @@ ct_march_kernel(CTMArgs<T> a)
     const bool emf_y = tx >= 1 && ty >= 1 && ty <= CM_OY;
     const size_t col = (size_t)gx + (size_t)nx * gy;
     const int lo = (ty + 1) * CM_PX + (tx + 1);
+    const int li = ty * CM_LX + tx;
     const bool helper = tid < CM_NHELP;
     int hx = 0, hy = 0;
     cm_helper_cell(helper ? tid : 0, hx, hy);
@@ -214,6 +196,9 @@ ct_march_kernel(CTMArgs<T> a)
     for (int v = 0; v < 8; ++v) zprev[v] = T(0);
 #pragma unroll
     for (int v = 0; v < 5; ++v) { fzp[v] = T(0); pend[v] = T(0); }
+    ForceCache<T, CM_NT> fc;
+    fc.k0 = -0x7fffffff;
+    fc.b = &L.FCB[0][tid];
 
     for (int q = zs - 1; q <= ze; ++q) {
         const int zq = cwrap(q, nz);
@@ -231,12 +216,12 @@ ct_march_kernel(CTMArgs<T> a)
             for (int v = 0; v < 8; ++v) { l[v] = P[v][lo - 1]; c0[v] = P[v][lo]; r[v] = P[v][lo + 1]; }
             ct_cell_states<T, 0>(l, c0, r, a, xlo, hi);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) XB[v][lo] = hi[v];
+            for (int v = 0; v < 8; ++v) XB[v][li] = hi[v];
 #pragma unroll
             for (int v = 0; v < 8; ++v) { l[v] = P[v][lo - CM_PX]; r[v] = P[v][lo + CM_PX]; }
             ct_cell_states<T, 1>(l, c0, r, a, ylo, hi);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) YB[v][lo] = hi[v];
+            for (int v = 0; v < 8; ++v) YB[v][li] = hi[v];
         }
         __syncthreads();
 
@@ -247,20 +232,20 @@ ct_march_kernel(CTMArgs<T> a)
         if (need_x) {
             T Lx[8];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) Lx[v] = XB[v][lo - 1];
+            for (int v = 0; v < 8; ++v) Lx[v] = XB[v][li - 1];
             ct_riemann<T, RIEM, 0>(Lx, xlo, bnx, a, fx);
         }
         if (need_y) {
             T Ly[8];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) Ly[v] = YB[v][lo - CM_PX];
+            for (int v = 0; v < 8; ++v) Ly[v] = YB[v][li - CM_LX];
             ct_riemann<T, RIEM, 1>(Ly, ylo, bny, a, fy);
         }
         __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { XB[v][lo] = fx[v]; YB[v][lo] = fy[v]; }
-        XB[6][lo] = fx[6];
-        YB[5][lo] = fy[5];
+        for (int v = 0; v < 5; ++v) { XB[v][li] = fx[v]; YB[v][li] = fy[v]; }
+        XB[6][li] = fx[6];
+        YB[5][li] = fy[5];
         const T fx7 = fx[7], fy7 = fy[7];
         __asm__ volatile("" ::: "memory");   // the other flux components are read back from LDS below
 
@@ -278,16 +263,16 @@ ct_march_kernel(CTMArgs<T> a)
 #pragma unroll
             for (int v = 0; v < 8; ++v) zprev[v] = zhi[v];
         }
-        if (!lead) { XB[5][lo] = fz[5]; XB[7][lo] = fz[6]; }
+        if (!lead) { XB[5][li] = fz[5]; XB[7][li] = fz[6]; }
         __syncthreads();
 
         if (!lead) {
             // ---- edge EMFs of plane q (ct_mhd.jl:139-154) ----------------------------------------
             // ez at (i-1/2, j-1/2), ex at (j-1/2, k-1/2), ey at (i-1/2, k-1/2) of this lane's cell
             T ex = T(0), ey = T(0), ez = T(0);
-            if (emf_z) ez = T(0.25) * (YB[5][lo] + YB[5][lo - 1] - XB[6][lo] - XB[6][lo - CM_PX]);
-            if (emf_x) ex = T(0.25) * (fz[6] + XB[7][lo - CM_PX] - fy7 - fybz);
-            if (emf_y) ey = T(0.25) * (fx7 + fxbz - fz[5] - XB[5][lo - 1]);
+            if (emf_z) ez = T(0.25) * (YB[5][li] + YB[5][li - 1] - XB[6][li] - XB[6][li - CM_LX]);
+            if (emf_x) ex = T(0.25) * (fz[6] + XB[7][li - CM_LX] - fy7 - fybz);
+            if (emf_y) ey = T(0.25) * (fx7 + fxbz - fz[5] - XB[5][li - 1]);
             T (&Ec)[3][CM_NPL] = L.E[q & 1];
             T (&Ep)[3][CM_NPL] = L.E[(q - 1) & 1];
             Ec[0][lo] = ex; Ec[1][lo] = ey; Ec[2][lo] = ez;
@@ -309,7 +294,7 @@ ct_march_kernel(CTMArgs<T> a)
                 }
                 if (a.do_turb && r[0] >= a.turb_min_rho) {
                     T acc[3];
-                    cm_force(a, gx, gy, zu, acc);
+                    force_accel(fc, a, gx, gy, zu, acc);
                     T rho = r[0], rhom = tmax(rho, a.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc, e = r[4];
                     e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
                     e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
@@ -329,7 +314,7 @@ ct_march_kernel(CTMArgs<T> a)
             // in-plane flux differences of plane q for its update in the next pass
             if (owned) {
 #pragma unroll
-                for (int t = 0; t < 5; ++t) pend[t] = (XB[t][lo] - XB[t][lo + 1]) + (YB[t][lo] - YB[t][lo + CM_PX]);
+                for (int t = 0; t < 5; ++t) pend[t] = (XB[t][li] - XB[t][li + 1]) + (YB[t][li] - YB[t][li + CM_LX]);
             }
 #pragma unroll
             for (int t = 0; t < 5; ++t) fzp[t] = fz[t];

@@ -6,6 +6,13 @@ namespace glm {
 
 constexpr int NXCD = 8;   // XCDs of an MI355X
 
+// i mod n in [0, n) for any int i (periodic wrap of a cell or forcing index).
+GLM_DEV int wrapi(int i, int n)
+{
+    int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
 // Logical tile of hardware workgroup id w when ids are dealt round-robin over NXCD XCDs: XCD x
 // gets the contiguous logical range [start_x, start_x + count_x).  A bijection on [0, total).
 GLM_DEV int xcd_block(int w, int total)

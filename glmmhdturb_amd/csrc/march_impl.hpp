@@ -31,6 +31,7 @@
 //   * The epilogue also folds the next step's CFL signal speed (glmmhd_turb.jl:1963-1971) into a
 //     per-thread max, reduced per wavefront and merged with one atomic per wavefront.
 #pragma once
+#include "force.hpp"
 #include "gpu_util.hpp"
 #include "physics.hpp"
 
@@ -91,12 +92,6 @@ struct ColdArgs {
     T dtdx, glm_fac, smallc, turb_min_rho, ramp, dt, dx, lx, ly, lz;
     int update_floor, do_turb, tgs, pad_;
 };
-
-GLM_DEV int wrapi(int i, int n)
-{
-    int m = i % n;
-    return m < 0 ? m + n : m;
-}
 
 // Cell (tx, ty) of lane t in the tile grown by one cell: tx in 0..OX+1, ty in 0..OY+1.
 GLM_DEV void lane_cell(int t, int& tx, int& ty)
@@ -217,73 +212,11 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
     guard(hi, m0, a.face_guard, s);
 }
 
-// Bilinear (x,y) part of the trilinear forcing interpolation (glmmhd_turb.jl:483-505) at forcing
-// plane kw for component c, in the reference's association order.
-template <typename T>
-GLM_DEV T force_bilinear(const T* af, int g, int i0w, int i1w, int j0w, int j1w, int kw, T wx, T wy)
-{
-    auto at = [&](int i, int j) { return gload(af + ((size_t)i + (size_t)g * ((size_t)j + (size_t)g * kw))); };
-    return (at(i0w, j0w) * (T(1) - wx) + at(i1w, j0w) * wx) * (T(1) - wy) +
-           (at(i0w, j1w) * (T(1) - wx) + at(i1w, j1w) * wx) * wy;
-}
-
-// Per-column cache of the forcing interpolation: the (x,y) bilinear values on the two forcing
-// planes that bracket the current cell.  A z-march crosses a forcing plane only every n/g cells,
-// so the 24 corner loads of the reference's trilinear interpolation happen once per crossing.
-// The bilinear values live in LDS (one slot per thread): they are touched once per cell only when
-// the kick is on, and keeping them out of registers leaves the unforced march unchanged.
-template <typename T>
-struct ForceCache {
-    int k0;        // lower forcing plane of b0, or a sentinel when empty
-    T* b;          // this thread's 6 LDS slots: b0[3], b1[3], stride NT
-    GLM_DEV T& b0(int c) { return b[(size_t)c * NT]; }
-    GLM_DEV T& b1(int c) { return b[(size_t)(3 + c) * NT]; }
-};
-
-template <typename T>
-GLM_DEV void force_accel(ForceCache<T>& fc, const ColdArgs<T>& a, int gx, int gy, int gz, T acc3[3])
-{
-    const int g = a.tgs;
-    const T gg = T(g);
-    T cz = ((T(gz) + T(0.5)) * a.dx) / a.lz * gg;
-    T fz = floor(cz);
-    int k0 = (int)fz;
-    T wz = cz - fz;
-    if (k0 != fc.k0) {   // the column's (x, y) position is needed only when a forcing plane is crossed
-        T cx = ((T(gx) + T(0.5)) * a.dx) / a.lx * gg;
-        T cy = ((T(gy) + T(0.5)) * a.dx) / a.ly * gg;
-        T fx = floor(cx), fy = floor(cy);
-        int i0 = (int)fx, j0 = (int)fy;
-        T wx = cx - fx, wy = cy - fy;
-        int i0w = wrapi(i0, g), i1w = wrapi(i0 + 1, g), j0w = wrapi(j0, g), j1w = wrapi(j0 + 1, g);
-        const size_t g3 = (size_t)g * g * g;
-        const bool shift = k0 == fc.k0 + 1;
-        // all corner loads are issued before the first result is stored
-        T n0[3], n1[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            n1[c] = force_bilinear(a.afield + c * g3, g, i0w, i1w, j0w, j1w, wrapi(k0 + 1, g), wx, wy);
-        if (shift) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) n0[c] = fc.b1(c);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                n0[c] = force_bilinear(a.afield + c * g3, g, i0w, i1w, j0w, j1w, wrapi(k0, g), wx, wy);
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { fc.b0(c) = n0[c]; fc.b1(c) = n1[c]; }
-        fc.k0 = k0;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc3[c] = a.ramp * (fc.b0(c) * (T(1) - wz) + fc.b1(c) * wz);
-}
-
 // Update epilogue of one cell (glmmhd_turb.jl:440-476), CFL speed of the result
 // (glmmhd_turb.jl:1963-1971).  acc = (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) + (Fz_lo - Fz_hi).
 template <typename T>
 GLM_DEV double update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl,
-                           ForceCache<T>& fc, int gx, int gy, int gz)
+                           ForceCache<T, NT>& fc, int gx, int gy, int gz)
 {
 #pragma unroll
     for (int v = 0; v < 9; ++v) r[v] = r[v] + a.dtdx * acc[v];
@@ -394,7 +327,7 @@ GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
 #pragma unroll
     for (int v = 0; v < 9; ++v) { zprev[v] = T(0); pend[v] = T(0); }
     double cmax = 0.0;
-    ForceCache<T> fc;
+    ForceCache<T, NT> fc;
     fc.k0 = -0x7fffffff;
     fc.b = &L.FCB[0][tid];
 

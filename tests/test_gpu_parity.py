@@ -394,6 +394,41 @@ def test_large_grid_invariants():
         assert abs(float(tot1[v] - tot0[v])) <= 1e-12 * float(scale[v]) + 1e-12, v
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_runs_are_bit_reproducible(prec):
+    """The step has no floating-point atomics and every face is solved by one lane, so two runs of
+    the same forced trajectory (192^3, 30 steps from the gradient state, CFL dt folded into each
+    step) give the same bits: a race between lanes or workgroups would show here."""
+    from test_gpu_ct_march import ct_args
+    N = 192
+    p = Params(N=N, gamma=5.0 / 3.0, courant=0.3, turb_gs=64, turb_kmin=1.0, turb_kmax=3.0, comp_frac=0.0,
+               turb_T=0.5, turb_Ndt=100, turb_seed=3)
+    out = []
+    for run in range(2):
+        s = Solver(p, prec)
+        u, un = s.new_state(), s.new_state()
+        s.ic_gradient(u, p.gamma)
+        f = s.setup_forcing(p, amp=1e4)
+        t, cmax = 0.0, s.ctot_max(u)
+        for it in range(30):
+            while t >= f.next_time:
+                f.advance()
+            dt = min(p.courant / N / cmax, f.next_time - t)
+            s.step(u, un, step_args(p, dt, t, do_turb=True, riemann="hlld", recon="ppm", want_cfl=True))
+            u, un = un, u
+            t += dt
+            cmax = s.cfl_max()
+        q, qn = s.new_state(8), s.new_state(8)
+        q.copy_(u[:8])
+        for it in range(5):
+            s.ct_step(q, qn, ct_args(p, 0.2 * dt, "hlld", "plm", do_turb=True, t=t), 0)
+            q, qn = qn, q
+        out.append((u.clone(), q.clone(), cmax))
+        s.close()
+    assert torch.equal(out[0][0], out[1][0]) and out[0][2] == out[1][2]
+    assert torch.equal(out[0][1], out[1][1])
+
+
 def test_largest_grid_fp32():
     """1300^3 in FP32 (two states: 158 GB of HBM): each plane holds 2.2e9 cells, more than a 32-bit
     index reaches.  A uniform state stays exactly uniform through the step, and on the gradient

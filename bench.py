@@ -321,6 +321,35 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, max(0, int(math.ceil(q / 100.0 * len(xs))) - 1))]
 
 
+def run_device_loop(N, steps, warmup):
+    """The headline workload driven from the device: the same box through the ensemble path with
+    one box, so dt, ch, the psi decay, the ramp and the OU cadence are computed on the GPU by the
+    step-control kernel, the step is a replayed HIP graph, and nothing is read back per step
+    (run_turb's loop, glmmhd_turb.jl:2213-2226, without its host round trip)."""
+    import torch
+    from glmmhdturb_amd import BoxSpec, Ensemble, Params
+    ens = Ensemble([BoxSpec(N=N)], Params(gamma=5.0 / 3.0), "f64")
+    best = ens.tune(reps=3)
+    ens.graph(True)
+    ens.step(warmup)
+    ens.sync()
+    st = ens.stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    ens.step(steps)
+    e1.record(st)
+    ens.sync()
+    wall = time.perf_counter() - t0
+    d = ens.diagnostics(0)
+    out = {"value": N ** 3 * steps / wall, "ms_per_step": wall / steps * 1e3,
+           "gpu_ms_per_step": e0.elapsed_time(e1) / steps, "tuned": best, "mach": d["mach"], "finite": d["finite"],
+           "note": "same workload as the headline, one-box ensemble: step control on the device, HIP graph, "
+                   "no per-step read-back"}
+    ens.close()
+    return out
+
+
 def run_ensemble(sweeps, warmup, nbox):
     """The batched-ensemble workload: nbox boxes with sizes drawn uniformly from {32,48,64,96,128}^3
     by seed, each with its own seed, forcing amplitude (rms Mach target 0.3-2.0) and B0 (beta
@@ -616,6 +645,8 @@ def main():
         note("adjoint next")
         rec["adjoint"] = run_adjoint(args.adj_N, args.adj_steps)
     if not args.no_ensemble:
+        note("device-driven loop next")
+        rec["device_loop"] = run_device_loop(N, args.steps, args.warmup)
         note("ensemble next")
         rec["ensemble"] = run_ensemble(args.steps, args.warmup, args.ens_boxes)
     if not args.no_gradient:

@@ -576,58 +576,84 @@ GLM_DEV int xcd_start(int x, int total)
 }
 GLM_DEV int xcd_count(int x, int total) { return total / NXCD + (x < total % NXCD ? 1 : 0); }
 
+// Thread 0 claims the workgroup's next tile (flat: its own, once; persistent: a ticket from the
+// XCD queues, stealing when its own is empty), finds the box by binary search over the tile
+// offsets and stages the box's cold arguments; the job is published in s_job after a barrier.
+template <typename T, bool PERSIST>
+GLM_DEV void ens_claim(const EnsLaunch<T>& e, int it, int& xq, unsigned long long* s_tb, MarchLDS<T>& L, int* s_job)
+{
+    int tile = -1;
+    if (!PERSIST) {
+        if (it == 0) tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    } else {
+        for (int k = 0; k < NXCD && tile < 0; ++k) {
+            const int x = (xq + k) % NXCD;
+            const unsigned tk = atomicAdd(&e.queue[x], 1u);
+            if (tk < (unsigned)xcd_count(x, e.total)) {
+                tile = xcd_start(x, e.total) + (int)tk;
+                xq = x;
+            }
+        }
+    }
+    int box = -1, lt = 0;
+    if (tile >= 0) {
+        int lo = 0, hi = e.npos;   // tile_off[lo] <= tile < tile_off[lo + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (e.tile_off[mid] <= tile) lo = mid; else hi = mid;
+        }
+        box = e.order[lo];
+        lt = tile - e.tile_off[lo];
+        stage_cold(e.boxes[box], L.ca);
+        *s_tb = (unsigned long long)wall_clock64();
+    }
+    s_job[0] = box;
+    s_job[1] = lt;
+}
+
+// The march of one claimed tile, its CFL merge into the box's slot and the box's timing record
+// (the start time waits in LDS, out of the registers the march needs).
 template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
+GLM_DEV void ens_tile(const EnsLaunch<T>& e, MarchLDS<T>& L, int box, int lt, int slot, const unsigned long long* s_tb)
+{
+    const MarchArgs<T> a = uniform_load(e.boxes + box);
+    const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE>(a, L, lt);
+    cfl_merge(L, cmax, a.cmax);
+    if (threadIdx.x == 0) {
+        const size_t k = (size_t)slot * e.nbox + box;
+        atomicMin(&e.t0[k], *s_tb);
+        atomicMax(&e.t1[k], (unsigned long long)wall_clock64());
+    }
+}
+
+// PERSIST = false: one workgroup per flat tile, straight-line (the same register budget as the
+// single-grid kernel).  PERSIST = true: a loop over claimed tiles until the queues are empty.
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE, bool PERSIST>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) ens_march_kernel(EnsLaunch<T> e)
 {
     __shared__ MarchLDS<T> L;
     __shared__ int s_job[2];   // box (-1: no more work) and its local tile
+    __shared__ unsigned long long s_tb;   // wall clock at the claim
     const int tid = threadIdx.x;
     const long long sw = *e.sweep;   // -1 before the first sweep (tuning launches)
     const int slot = (int)((sw < 0 ? 0 : sw) % ENS_LOG);
     int xq = (int)blockIdx.x % NXCD;
-    unsigned long long tb = 0ull;
+    if (!PERSIST) {
+        if (tid == 0) ens_claim<T, false>(e, 0, xq, &s_tb, L, s_job);
+        __syncthreads();
+        const int box = __builtin_amdgcn_readfirstlane(s_job[0]);
+        const int lt = __builtin_amdgcn_readfirstlane(s_job[1]);
+        if (box < 0) return;
+        ens_tile<T, PRED, RIEM, RECON, SLOPE>(e, L, box, lt, slot, &s_tb);
+        return;
+    }
     for (int it = 0;; ++it) {
-        if (tid == 0) {
-            int tile = -1;
-            if (!e.persistent) {
-                if (it == 0) tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
-            } else {
-                for (int k = 0; k < NXCD && tile < 0; ++k) {
-                    const int x = (xq + k) % NXCD;
-                    const unsigned tk = atomicAdd(&e.queue[x], 1u);
-                    if (tk < (unsigned)xcd_count(x, e.total)) {
-                        tile = xcd_start(x, e.total) + (int)tk;
-                        xq = x;
-                    }
-                }
-            }
-            int box = -1, lt = 0;
-            if (tile >= 0) {
-                int lo = 0, hi = e.npos;   // tile_off[lo] <= tile < tile_off[lo + 1]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (e.tile_off[mid] <= tile) lo = mid; else hi = mid;
-                }
-                box = e.order[lo];
-                lt = tile - e.tile_off[lo];
-                stage_cold(e.boxes[box], L.ca);
-                tb = (unsigned long long)wall_clock64();
-            }
-            s_job[0] = box;
-            s_job[1] = lt;
-        }
+        if (tid == 0) ens_claim<T, true>(e, it, xq, &s_tb, L, s_job);
         __syncthreads();
         const int box = __builtin_amdgcn_readfirstlane(s_job[0]);
         const int lt = __builtin_amdgcn_readfirstlane(s_job[1]);
         if (box < 0) break;
-        const MarchArgs<T> a = uniform_load(e.boxes + box);
-        const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE>(a, L, lt);
-        cfl_merge(L, cmax, a.cmax);
-        if (tid == 0) {
-            const size_t k = (size_t)slot * e.nbox + box;
-            atomicMin(&e.t0[k], tb);
-            atomicMax(&e.t1[k], (unsigned long long)wall_clock64());
-        }
+        ens_tile<T, PRED, RIEM, RECON, SLOPE>(e, L, box, lt, slot, &s_tb);
         __syncthreads();   // LDS (job, cold args, reduction) is free for the next tile
     }
 }

@@ -41,7 +41,10 @@ hipError_t launch_ens_pr<GLM_T, GLM_PRED, GLM_RIEM>(const EnsLaunch<GLM_T>& e, i
 {
 #define GLM_CASE(RC, SL)                                                                                 \
     if (recon == RC && slope == SL) {                                                                    \
-        hipLaunchKernelGGL((ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>), dim3((unsigned)grid), dim3(NT), 0, st, e); \
+        if (e.persistent)                                                                                \
+            hipLaunchKernelGGL((ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL, true>), dim3((unsigned)grid), dim3(NT), 0, st, e); \
+        else                                                                                             \
+            hipLaunchKernelGGL((ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL, false>), dim3((unsigned)grid), dim3(NT), 0, st, e); \
         return hipGetLastError();                                                                        \
     }
     GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
@@ -54,7 +57,7 @@ template <>
 const void* ens_kernel_ptr_pr<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
 {
 #define GLM_CASE(RC, SL) \
-    if (recon == RC && slope == SL) return (const void*)&ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL>;
+    if (recon == RC && slope == SL) return (const void*)&ens_march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL, true>;
     GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
     GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
 #undef GLM_CASE

@@ -272,8 +272,8 @@ GLM_DEV void stage_cold(const MarchArgs<T>& a, ColdArgs<T>& ca)
 // z-chunk zc, tile = tile_x + tiles_x * (tile_y + tiles_y * zc), and write the updated planes.
 // L.ca must hold `a`'s cold arguments, published by a barrier.  Returns the lane's max CFL speed
 // over the cells it updated.
-template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
-GLM_DEV double march_tile(const MarchArgs<T>& a, MarchLDS<T>& L, int tile)
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE, typename A = MarchArgs<T>>
+GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
 {
     T (&P)[9][NPL] = L.P;
     T (&XB)[9][NPL] = L.XB;
@@ -557,9 +557,11 @@ struct EnsLaunch {
     unsigned long long* t1;      // [ENS_LOG][nbox] wall-clock end of its last tile
 };
 
-// A uniform struct load through the constant address space, so that it lands in scalar registers.
+// A box's argument block as a reference into the constant address space: the march reads each
+// field with a scalar load where it needs it, as it reads a kernel argument, instead of holding a
+// copy of the whole block in registers (which cost the ensemble kernel 64 B per lane of spills).
 template <typename S>
-GLM_DEV S uniform_load(const S* p)
+GLM_DEV const auto& constant_ref(const S* p)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     return *(const __attribute__((address_space(4))) S*)p;
@@ -616,7 +618,7 @@ GLM_DEV void ens_claim(const EnsLaunch<T>& e, int it, int& xq, unsigned long lon
 template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
 GLM_DEV void ens_tile(const EnsLaunch<T>& e, MarchLDS<T>& L, int box, int lt, int slot, const unsigned long long* s_tb)
 {
-    const MarchArgs<T> a = uniform_load(e.boxes + box);
+    const auto& a = constant_ref(e.boxes + box);
     const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE>(a, L, lt);
     cfl_merge(L, cmax, a.cmax);
     if (threadIdx.x == 0) {

@@ -144,6 +144,25 @@ def test_zchunk_invariance(orc, grid3d):
         assert np.array_equal(o, outs[0])
 
 
+def test_zchunk_autotune_keeps_bits():
+    """Solver.tune_zchunk times the march over several z-chunkings, keeps the fastest, and the
+    tuned steps give the same bits as the library's own choice."""
+    p = Params(N=40, nx=40, ny=30, nz=96, gamma=5.0 / 3.0, courant=0.3)
+    s = Solver(p, "f64")
+    u, un = s.new_state(), s.new_state()
+    s.ic_gradient(u, p.gamma)
+    a = step_args(p, 0.3 / 40 / s.ctot_max(u), 0.0, riemann="hlld", recon="ppm", want_cfl=False)
+    s.step(u, un, a)
+    ref = un.clone()
+    r = s.tune_zchunk(u, un, a, reps=1)
+    assert s.zchunk == r["zchunk"] and (r["zchunk"] == 0 or 8 <= r["zchunk"] <= 24)
+    un.zero_()
+    s.step(u, un, a)
+    assert torch.equal(un, ref)
+    assert a.zchunk == 0                                     # the caller's block is not modified
+    s.close()
+
+
 def test_cfl_and_diagnostics(orc, grid3d):
     p, s, u0 = grid3d
     ud = to_dev(s, u0)

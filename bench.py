@@ -458,6 +458,8 @@ def run_ct_precision(N, nsteps):
         q1[7] = 0.2
         f = s.setup_forcing(p, AMP)
         t, cmax = 0.0, s.ct_ctot_max(q1)
+        tuned = s.tune_zchunk(q1, q2, step_args(p, p.courant * dxof(p) / cmax, 0.0, do_turb=True, riemann="hlld",
+                                                recon="plm", predictor="oned"), ct=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(nsteps):
@@ -475,6 +477,7 @@ def run_ct_precision(N, nsteps):
         d["divbmax"] = s.ct_divb_max(q1)
         d["value"] = N ** 3 * nsteps / wall
         d["t"] = t
+        d["zchunk"] = tuned
         # GPU time of one step at the last dt without the CFL pass: the fused march kernel
         # against the multi-pass kernel sequence (ct_mhd.jl step!), 5 steps each, HIP events
         a = step_args(p, dt, t, do_turb=True, riemann="hlld", recon="plm", predictor="oned", want_cfl=False)

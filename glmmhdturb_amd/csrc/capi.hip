@@ -432,7 +432,7 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
     h.dt = p->dt; h.dx = p->dx; h.smallc = p->smallc; h.turb_min_rho = p->turb_min_rho; h.ramp = p->ramp;
     h.face_slope = ct_face_slope; h.riemann = p->riemann; h.recon = p->recon;
     h.update_floor = p->update_floor; h.do_turb = p->do_turb; h.tgs = ctx->g; h.afield = ctx->afield;
-    h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2;
+    h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2; h.zchunk = p->zchunk;
     if (p->want_cfl) HIPCHK(launch_clear_u64(ctx->d_cmax + 2, 1, ctx->stream));
     if (fused) {
         const int key = 1000 + p->riemann;

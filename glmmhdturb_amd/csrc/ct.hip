@@ -69,7 +69,7 @@ hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, 
         a.o[v] = (T*)o[v];
     }
     a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
-    ct_march_tiling(h.nx, h.ny, h.nz, slots, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
+    ct_march_tiling(h.nx, h.ny, h.nz, h.zchunk, slots, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
     a.gamma = (T)h.gamma; a.igm1 = (T)(1.0 / (h.gamma - 1.0)); a.smallr = (T)h.smallr; a.smallp = (T)h.smallp;
     double dtdx = h.dt / h.dx;
     a.dtdx = (T)dtdx; a.half_dtdx = (T)(0.5 * dtdx); a.dt = (T)h.dt; a.dx = (T)h.dx;

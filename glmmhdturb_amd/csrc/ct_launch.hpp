@@ -9,6 +9,7 @@ struct CTArgsHost {
     int nx, ny, nz;
     double gamma, smallr, smallp, dt, dx, smallc, turb_min_rho, ramp;
     int face_slope, riemann, recon, update_floor, do_turb, tgs, want_cfl;
+    int zchunk;                    // z planes per march tile (0: the occupancy model)
     const void* afield;
     unsigned long long* cmax;
 };

@@ -113,10 +113,16 @@ GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[8])
 
 // Tiles and z-chunks of the CT march on `slots` concurrent workgroups: rounds x (chunk + 2 lead-in
 // planes), the same model as the GLM march's (launch.hpp march_tiling) with its longer lead-in.
-inline void ct_march_tiling(int nx, int ny, int nz, int slots, int& tiles_x, int& tiles_y, int& nzc, int& zchunk)
+inline void ct_march_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int& tiles_x, int& tiles_y, int& nzc,
+                            int& zchunk)
 {
     tiles_x = (nx + CM_OX - 1) / CM_OX;
     tiles_y = (ny + CM_OY - 1) / CM_OY;
+    if (zchunk_req > 0) {
+        zchunk = zchunk_req < nz ? zchunk_req : nz;
+        nzc = (nz + zchunk - 1) / zchunk;
+        return;
+    }
     const long cols = (long)tiles_x * tiles_y;
     const long sl = slots > 0 ? slots : 256;
     const int maxc = nz / 8 > 1 ? nz / 8 : 1;

@@ -155,7 +155,8 @@ class Solver:
                 raise _lib.GLMError(self.lib.glm_error_string(self.lib.glm_last_error()).decode())
             check(self.lib.glm_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         self.forcing = None
-        self.zchunk = 0   # z planes per march tile for steps whose args leave it at 0 (0: the library's model)
+        self.zchunk = 0      # z planes per march tile for steps whose args leave it at 0 (0: the library's model)
+        self.ct_zchunk = 0   # the same for the CT march
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -204,29 +205,37 @@ class Solver:
             args.zchunk = self.zchunk
         check(self.lib.glm_step(self.ctx, qa, oa, C.byref(args)))
 
-    def tune_zchunk(self, uold, unew, args: GLMParams, reps: int = 3) -> dict:
-        """Time the march for a set of z-chunkings of this grid and keep the fastest for later
-        steps whose args leave zchunk at 0.  Every chunking gives the same bits (each chunk
-        recomputes the faces at its boundary), so this changes speed only.  Writes only unew.
-        Candidates: 0 (the library's occupancy model) and ceil(nz / n) for n = 4..16 chunks."""
+    def tune_zchunk(self, uold, unew, args: GLMParams, reps: int = 3, ct: bool = False, face_slope: int = 0) -> dict:
+        """Time the march (ct=True: the CT march) for a set of z-chunkings of this grid and keep the
+        fastest for later steps whose args leave zchunk at 0.  Every chunking gives the same bits
+        (each chunk recomputes the faces at its boundary), so this changes speed only.  Writes only
+        unew.  Candidates: 0 (the library's occupancy model) and ceil(nz / n) for n = 4..16 chunks."""
         torch = _torch()
         cands = [0] + sorted({-(-self.nz // n) for n in range(4, 17) if -(-self.nz // n) >= 8}, reverse=True)
         st = torch.cuda.current_stream()
         times = {}
-        self.zchunk = 0
+        if ct:
+            self.ct_zchunk = 0
+            run = lambda a: self.ct_step(uold, unew, a, face_slope)  # noqa: E731
+        else:
+            self.zchunk = 0
+            run = lambda a: self.step(uold, unew, a)  # noqa: E731
         for zc in cands:
             a = GLMParams.from_buffer_copy(args)
             a.zchunk, a.want_cfl = zc, 0
-            self.step(uold, unew, a)                      # untimed first launch
+            run(a)                                        # untimed first launch
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(st)
             for _ in range(reps):
-                self.step(uold, unew, a)
+                run(a)
             e1.record(st)
             torch.cuda.synchronize()
             times[zc] = e0.elapsed_time(e1) / reps
         best = min(times, key=times.get)
-        self.zchunk = best
+        if ct:
+            self.ct_zchunk = best
+        else:
+            self.zchunk = best
         return {"zchunk": best, "ms": times[best], "auto_ms": times[0]}
 
     def step_plm(self, uold, unew, p: Params, dt: float, t: float, *, do_turb=False, riemann="hll",
@@ -321,6 +330,9 @@ class Solver:
         the multi-pass kernel sequence (same results to rounding)."""
         qa, kq = self._planes(qold, 8)
         oa, ko = self._planes(qnew, 8)
+        if args.zchunk == 0 and self.ct_zchunk and not passes:
+            args = GLMParams.from_buffer_copy(args)
+            args.zchunk = self.ct_zchunk
         fn = self.lib.glm_ct_step_passes if passes else self.lib.glm_ct_step
         check(fn(self.ctx, qa, oa, C.byref(args), int(face_slope)))
 

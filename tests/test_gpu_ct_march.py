@@ -150,6 +150,34 @@ def test_ct_march_matches_passes(orc, prec, tol, riem, recon):
     s.close()
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ct_march_zchunk_invariance_and_tuner(orc, prec):
+    """Each z-chunk recomputes the faces and EMFs at its boundary, so every chunking gives the same
+    bits; the z-chunk tuner keeps one of them and changes nothing in the result."""
+    nx, ny, nz = 24, 20, 40
+    p = Params(N=nx, nx=nx, ny=ny, nz=nz, gamma=5.0 / 3.0)
+    s = Solver(p, prec)
+    u = ct_state(orc, nx, ny, nz, seed=6)
+    dx = 1.0 / nx
+    a = ct_args(p, 0.3 * dx / orc.ct_ctot_max(u, p.gamma, 1e-6, 1e-6), "hlld", "plm")
+    qd = to_dev(s, u)
+    outs = []
+    for zc in (0, 1, 3, 7, 16, 40):
+        b = ct_args(p, a.dt, "hlld", "plm", zchunk=zc)
+        qn = s.new_state(8)
+        s.ct_step(qd, qn, b, 0)
+        outs.append(qn)
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    qn = s.new_state(8)
+    r = s.tune_zchunk(qd, qn, a, reps=1, ct=True)
+    assert s.ct_zchunk == r["zchunk"]
+    qn.zero_()
+    s.ct_step(qd, qn, a, 0)
+    assert torch.equal(qn, outs[0])
+    s.close()
+
+
 def test_ct_march_largest_grid_fp32():
     """1300^3 FP32 CT (two 8-plane states: 141 GB of HBM; 2.2e9 cells per plane, past any 32-bit
     index).  A uniform state stays exactly uniform; from a density/pressure wave in a uniform field

@@ -60,10 +60,10 @@ static bool dispatch_faceswise(const AdjArgs& a, dim3 g, hipStream_t st, bool up
     return false;
 }
 
-static bool dispatch_back(const AdjArgs& a, hipStream_t st, int pred, int recon, int slope)
+static bool dispatch_back(const AdjArgs& a, int zchunk, hipStream_t st, int pred, int recon, int slope)
 {
     int tx, ty, nzc, zc;
-    march_tiling(a.nx, a.ny, a.nz, 0, 256, tx, ty, nzc, zc);   // one workgroup per CU
+    march_tiling(a.nx, a.ny, a.nz, zchunk, 256, tx, ty, nzc, zc);   // one workgroup per CU
     const dim3 g((unsigned)(tx * ty * nzc));
 #define GLM_CASE(P, RC, SL)                                                                          \
     if (pred == P && recon == RC && slope == SL) {                                                   \
@@ -79,7 +79,7 @@ static bool dispatch_back(const AdjArgs& a, hipStream_t st, int pred, int recon,
 }
 
 hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int recon, int slope, double* fdot,
-                               bool fused, hipStream_t st)
+                               bool fused, int zchunk, hipStream_t st)
 {
     if (recon == RECON_PPM_SPIKE) return hipErrorInvalidValue;
     const size_t n = (size_t)a.nx * a.ny * a.nz;
@@ -92,7 +92,7 @@ hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int
     hipLaunchKernelGGL(adj_fdot_kernel, dim3(1), dim3(ADJ_NT), 0, st, (const double*)a.part, (int)g.x, fdot);
     dispatch_faceswise(a, g, st, false, riem);
     if (fused) {
-        dispatch_back(a, st, pred, recon, slope);
+        dispatch_back(a, zchunk, st, pred, recon, slope);
     } else {
         dispatch_cellwise(a, g, st, false, pred, recon, slope);
         hipLaunchKernelGGL(adj_gather_kernel, g, dim3(ADJ_NT), 0, st, a);

@@ -30,10 +30,10 @@ struct AdjArgs {
 size_t adj_scratch_bytes(size_t n);
 // Launch the reverse step and add the step's sum of (kick acceleration adjoint . acceleration)
 // to *fdot (device).  With fused, the face-state and predictor adjoints and the gather run as one
-// z-march kernel (adj_march.hpp); otherwise as the separate passes 5 and 6.  a.scr and a.part must
-// come from a scratch of adj_scratch_bytes(n).  Returns hipErrorInvalidValue for an unsupported
-// scheme.
+// z-march kernel (adj_march.hpp, z planes per tile: zchunk, 0 = the occupancy model); otherwise as
+// the separate passes 5 and 6.  a.scr and a.part must come from a scratch of adj_scratch_bytes(n).
+// Returns hipErrorInvalidValue for an unsupported scheme.
 hipError_t launch_step_adjoint(AdjArgs a, void* scratch, int pred, int riem, int recon, int slope, double* fdot,
-                               bool fused, hipStream_t st);
+                               bool fused, int zchunk, hipStream_t st);
 
 }  // namespace glm

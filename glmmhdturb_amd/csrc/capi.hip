@@ -381,7 +381,7 @@ static int step_adjoint_impl(glm_ctx* ctx, void* const* u0, void* const* lam1, v
     a.do_turb = p->do_turb;
     a.afield = (const double*)afield;
     hipError_t he = launch_step_adjoint(a, ctx->adj_scratch, p->predictor, p->riemann, p->recon, p->slope, ctx->d_fdot,
-                                        fused, ctx->stream);
+                                        fused, p->zchunk, ctx->stream);
     if (he == hipErrorInvalidValue) return set_err(GLM_ERR_ARG, "glm_step_adjoint: unsupported scheme");
     HIPCHK(he);
     return GLM_OK;

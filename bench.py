@@ -46,7 +46,16 @@ def parse():
     ap.add_argument("--ct-N", type=int, default=512)
     ap.add_argument("--ct-steps", type=int, default=200)
     ap.add_argument("--ens-boxes", type=int, default=64)
+    ap.add_argument("--zchunk", type=int, default=0,
+                    help="z planes per march tile (0: autotune on the first step); fixed for profiling runs")
+    ap.add_argument("--ct-zchunk", default="0",
+                    help="the same for the CT march: one value, or FP64,FP32 (0: autotune)")
+    ap.add_argument("--ens-config", default="",
+                    help="zchunk,persistent,order for the ensemble and the device loop (empty: autotune)")
     return ap.parse_args()
+
+
+ARGS = None   # the parsed command line (launch-configuration overrides)
 
 
 def copy_calibration():
@@ -89,8 +98,12 @@ def run_gpu(N, steps, warmup, precision):
     ev = []
 
     # the launch's z-chunking is autotuned on the first step's arguments (same bits for every choice)
-    tuned = s.tune_zchunk(u1, u2, step_args(p, min(p.courant * dxof(p) / cmax, f.next_time), 0.0, do_turb=True,
-                                            riemann="hlld", recon="ppm", predictor="full", scheme="cube"))
+    if ARGS is not None and ARGS.zchunk > 0:
+        s.zchunk = ARGS.zchunk
+        tuned = {"zchunk": ARGS.zchunk, "fixed": True}
+    else:
+        tuned = s.tune_zchunk(u1, u2, step_args(p, min(p.courant * dxof(p) / cmax, f.next_time), 0.0, do_turb=True,
+                                                riemann="hlld", recon="ppm", predictor="full", scheme="cube"))
 
     def one_step(timed):
         nonlocal u1, u2, t, cmax
@@ -325,6 +338,15 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, max(0, int(math.ceil(q / 100.0 * len(xs))) - 1))]
 
 
+def ens_setup(ens):
+    """The ensemble's launch configuration: autotuned, or fixed by --ens-config for profiling runs."""
+    if ARGS is not None and ARGS.ens_config:
+        z, pm, od = (int(x) for x in ARGS.ens_config.split(","))
+        ens.configure(zchunk=z, persistent=pm, order=od)
+        return {"zchunk": z, "persistent": pm, "order": od, "fixed": True}
+    return ens.tune(reps=3)
+
+
 def run_device_loop(N, steps, warmup):
     """The headline workload driven from the device: the same box through the ensemble path with
     one box, so dt, ch, the psi decay, the ramp and the OU cadence are computed on the GPU by the
@@ -333,7 +355,7 @@ def run_device_loop(N, steps, warmup):
     import torch
     from glmmhdturb_amd import BoxSpec, Ensemble, Params
     ens = Ensemble([BoxSpec(N=N)], Params(gamma=5.0 / 3.0), "f64")
-    best = ens.tune(reps=3)
+    best = ens_setup(ens)
     ens.graph(True)
     ens.step(warmup)
     ens.sync()
@@ -363,7 +385,7 @@ def run_ensemble(sweeps, warmup, nbox):
     from glmmhdturb_amd import Ensemble, Params, dxof, random_boxes, step_args
     boxes = random_boxes(nbox, seed=2024)
     ens = Ensemble(boxes, Params(gamma=5.0 / 3.0), "f64")
-    best = ens.tune(reps=3)
+    best = ens_setup(ens)
     ens.graph(True)
     ens.step(warmup)
     ens.sync()
@@ -458,8 +480,14 @@ def run_ct_precision(N, nsteps):
         q1[7] = 0.2
         f = s.setup_forcing(p, AMP)
         t, cmax = 0.0, s.ct_ctot_max(q1)
-        tuned = s.tune_zchunk(q1, q2, step_args(p, p.courant * dxof(p) / cmax, 0.0, do_turb=True, riemann="hlld",
-                                                recon="plm", predictor="oned"), ct=True)
+        czs = [int(x) for x in (ARGS.ct_zchunk.split(",") if ARGS is not None else ["0"])]
+        cz = czs[0] if prec == "f64" or len(czs) == 1 else czs[1]
+        if cz > 0:
+            s.ct_zchunk = cz
+            tuned = {"zchunk": cz, "fixed": True}
+        else:
+            tuned = s.tune_zchunk(q1, q2, step_args(p, p.courant * dxof(p) / cmax, 0.0, do_turb=True, riemann="hlld",
+                                                    recon="plm", predictor="oned"), ct=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(nsteps):
@@ -591,7 +619,9 @@ def note(msg):
 
 
 def main():
+    global ARGS
     args = parse()
+    ARGS = args
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU path is "
               f"design-only)", file=sys.stderr)

@@ -360,6 +360,51 @@ def test_legacy_march_run_dev(orc32, orc):
     assert np.all(rel_l1(got, uo) <= 2e-6)
 
 
+def test_legacy_ct_run_and_march_run_dev_f64(orc32, orc):
+    """The other legacy entry points: ct_run (spike_ctm.cu:300-328: 8 float planes, the CT march
+    with HLL + PLM) against the FP32 CT oracle, march_run_dev_f64 (9 double planes, the spike
+    scheme) against the FP64 oracle, and the ct_regs / ct_shmem figures of the CT march."""
+    import ctypes as C
+    from test_gpu_ct_march import ct_state
+    lib = _lib.load()
+    N = 24
+    g32, dtdx32 = float(np.float32(5.0 / 3.0)), float(np.float32(0.02))
+    assert lib.march_set_grid(N, N, N) == 0
+    lib.march_set_gamma(C.c_float(g32))
+    lib.march_set_dtdx(C.c_float(dtdx32))
+    lib.march_set_glm(C.c_float(1.0), C.c_float(1.0))
+    # ct_run: 3 steps, result left in q (odd step count: copied back)
+    u = ct_state(orc, N, N, N).astype(np.float32)
+    q = [torch.from_numpy(u[v].copy()).cuda() for v in range(8)]
+    o = [torch.empty_like(x) for x in q]
+    cc = [torch.empty_like(x) for x in q[:3]]
+    qa, kq = _lib.plane_array([x.data_ptr() for x in q])
+    oa, ko = _lib.plane_array([x.data_ptr() for x in o])
+    ca, kc = _lib.plane_array([x.data_ptr() for x in cc])
+    assert lib.ct_run(qa, oa, ca, 3) >= 0.0
+    cs = CTScheme(gamma=g32, smallr=1e-6, smallp=1e-6, dt=dtdx32, dx=1.0, riemann="hll", recon="plm", face_slope=0)
+    uo = u.copy()
+    for it in range(3):
+        uo = orc32.ct_step(uo, cs)
+    got = np.stack([x.cpu().numpy() for x in q])
+    assert np.all(rel_l1(got, uo, 8) <= 2e-6)
+    assert lib.ct_regs() > 0 and lib.ct_shmem() > 0
+    # march_run_dev_f64: 2 steps (even: the result is in q without a copy)
+    u64 = smooth_state(orc, N, N, N, seed=9)
+    q = [torch.from_numpy(u64[v].copy()).cuda() for v in range(9)]
+    o = [torch.empty_like(x) for x in q]
+    qa, kq = _lib.plane_array([x.data_ptr() for x in q])
+    oa, ko = _lib.plane_array([x.data_ptr() for x in o])
+    assert lib.march_run_dev_f64(qa, oa, 2) >= 0.0
+    sc = Scheme(gamma=g32, smallr=1e-6, pfl=1e-6, dt=dtdx32, dx=1.0, ch=1.0, glm_fac=1.0, predictor="oned",
+                recon="plm", slope="moncen", riemann="hll", mh_revert=False, face_guard="clamp", update_floor=False)
+    uo = u64.copy()
+    for it in range(2):
+        uo = orc.step(uo, sc)
+    got = np.stack([x.cpu().numpy() for x in q])
+    assert np.all(rel_l1(got, uo) <= TOL64)
+
+
 @pytest.mark.parametrize("riem,recon,fs", [("hll", "plm", 0), ("hll", "plm", 1), ("hlld", "plm", 0),
                                            ("llf", "plm", 0), ("hll", "ppm_spike", 0)])
 def test_ct_step_parity(orc, riem, recon, fs):

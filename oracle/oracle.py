@@ -46,6 +46,27 @@ class OCTScheme(C.Structure):
         "face_slope", "riemann", "recon", "do_turb", "turb_gs", "update_floor")]
 
 
+class OHyScheme(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "dtdx", "smallr", "smallp")] + [
+        ("fallback", C.c_int), ("fb_dmin", C.c_double), ("fb_pmin", C.c_double)]
+
+
+@dataclass
+class HydroScheme:
+    """The 5-variable hydro march (spike_25d.cu:180 constants; LLF fallback :335-376)."""
+    gamma: float = 1.4
+    dtdx: float = 0.05
+    smallr: float = 1e-6
+    smallp: float = 1e-6
+    fallback: bool = False
+    fb_dmin: float = 1e-3
+    fb_pmin: float = 1e-3
+
+    def to_c(self) -> OHyScheme:
+        return OHyScheme(self.gamma, self.dtdx, self.smallr, self.smallp, int(self.fallback), self.fb_dmin,
+                         self.fb_pmin)
+
+
 @dataclass
 class Scheme:
     """Scheme switches + run scalars, with names from Params (glmmhd_turb.jl:28-52)."""
@@ -161,6 +182,16 @@ class Oracle:
             "o_ct_face_flux": (None, [vp, vp, vp, vp, rl, i, vp, vp]),
             "o_ct_dflux": (None, [vp, i, rl, vp]),
             "o_ct_cellprim": (None, [vp, vp, rl, rl, rl, vp]),
+            "oh_c2p": (None, [vp, vp, vp]),
+            "oh_moncen": (rl, [rl, rl, rl]),
+            "oh_edge": (None, [vp, vp, rl, vp, vp]),
+            "oh_flux": (None, [vp, i, vp, vp]),
+            "oh_tocons": (None, [vp, vp, vp]),
+            "oh_hanc1d": (None, [vp, vp, i, vp, vp]),
+            "oh_hll": (None, [vp, vp, i, vp, vp]),
+            "oh_riemann_fb": (None, [vp, vp, i, vp, vp]),
+            "oh_step": (i, [vp, vp, i, i, i, vp, vp]),
+            "oh_step_scratch_reals": (C.c_size_t, [i, i, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -345,6 +376,52 @@ class Oracle:
     def ct_ctot_max(self, q, gamma, smallr, smallp):
         nv, nz, ny, nx = q.shape
         return self.lib.o_ct_ctot_max(_ptr(self.arr(q)), nx, ny, nz, gamma, smallr, smallp)
+
+    # ---- 5-variable hydro march ----------------------------------------------------
+    def _hy(self, name, *vecs, extra=(), sc: HydroScheme, nout=5):
+        cs = sc.to_c()
+        out = self.vec(nout)
+        args = [_ptr(self.arr(v, 5)) for v in vecs]
+        getattr(self.lib, name)(*args, *extra, _ptr(out), C.byref(cs))
+        return out
+
+    def hydro_c2p(self, c, sc: HydroScheme):
+        return self._hy("oh_c2p", c, sc=sc)
+
+    def hydro_tocons(self, q, sc: HydroScheme):
+        return self._hy("oh_tocons", q, sc=sc)
+
+    def hydro_flux(self, q, d, sc: HydroScheme):
+        return self._hy("oh_flux", q, extra=(d,), sc=sc)
+
+    def hydro_hanc1d(self, m, s, d, sc: HydroScheme):
+        return self._hy("oh_hanc1d", m, s, extra=(d,), sc=sc)
+
+    def hydro_hll(self, L, R_, d, sc: HydroScheme):
+        return self._hy("oh_hll", L, R_, extra=(d,), sc=sc)
+
+    def hydro_riemann_fb(self, L, R_, d, sc: HydroScheme):
+        return self._hy("oh_riemann_fb", L, R_, extra=(d,), sc=sc)
+
+    def hydro_edge(self, b, s, sign, sc: HydroScheme):
+        cs = sc.to_c()
+        out = self.vec(5)
+        self.lib.oh_edge(_ptr(self.arr(b, 5)), _ptr(self.arr(s, 5)), sign, _ptr(out), C.byref(cs))
+        return out
+
+    def hydro_moncen(self, a, b, c):
+        return self.lib.oh_moncen(a, b, c)
+
+    def hydro_step(self, u, sc: HydroScheme):
+        """One step of the march on a (5, nz, ny, nx) conserved state."""
+        nv, nz, ny, nx = u.shape
+        assert nv == 5, nv
+        u = self.arr(u)
+        out = np.zeros_like(u)
+        scratch = np.empty(self.lib.oh_step_scratch_reals(nx, ny, nz), dtype=self.dtype)
+        cs = sc.to_c()
+        self.lib.oh_step(_ptr(u), _ptr(out), nx, ny, nz, C.byref(cs), _ptr(scratch))
+        return out
 
 
 def twiddles(g: int):

@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--adj-N", type=int, default=128)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--no-ct", action="store_true")
+    ap.add_argument("--no-hydro", action="store_true")
+    ap.add_argument("--hydro-N", type=int, default=480)
     ap.add_argument("--ct-N", type=int, default=512)
     ap.add_argument("--ct-steps", type=int, default=200)
     ap.add_argument("--ens-boxes", type=int, default=64)
@@ -627,6 +629,50 @@ def run_adjoint(N, nsteps, window=50, snapshots=16):
     return out
 
 
+def run_hydro(N=480, iters=30, warmup=3, dtdx=0.05):
+    """The 5-variable hydro march (spike_25d.cu as build_march.sh builds it) on the reference's
+    own timing protocol (spike_25d.cu:530-626): N^3 periodic, IC ph = 0.001 (i mod 911) with
+    rho = 1 + 0.3 ph, m = (0.1, -0.1, 0.05) ph, E = 2.5 + 0.5 ph; dt/dx 0.05, gamma 1.4;
+    3 untimed and 30 timed steps; FP32 (the reference's precision) and FP64."""
+    import torch
+    from glmmhdturb_amd import Params, Solver, hydro_args
+    out = {"workload": f"hydro line march {N}^3 (normal-only Hancock, PLM MonCen, HLL), reference IC "
+                       f"ph = 0.001 (i mod 911), dt/dx {dtdx}, {warmup} + {iters} steps"}
+    for prec in ("f32", "f64"):
+        s = Solver(Params(N=N, gamma=1.4), prec)
+        q, o = s.new_state(5), s.new_state(5)
+        i = torch.arange(N ** 3, device=s.device, dtype=torch.int64)
+        ph = 0.001 * (i % 911).to(s.dtype)
+        del i
+        for v, (c0, c1) in enumerate(((1.0, 0.3), (0.0, 0.1), (0.0, -0.1), (0.0, 0.05), (2.5, 0.5))):
+            q[v].view(-1).copy_(c0 + c1 * ph)
+        del ph
+        a = hydro_args(dtdx, 1.4)
+        for _ in range(warmup):
+            s.hydro_step(q, o, a)
+            q, o = o, q
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(iters):
+            s.hydro_step(q, o, a)
+            q, o = o, q
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        bpc = 5 * 2 * (4 if prec == "f32" else 8)
+        out[prec] = {"value": N ** 3 / (ms * 1e-3), "unit": "cell-updates/s", "ms_per_step": ms,
+                     "bytes_per_cell": bpc, "achieved_GBs": bpc * N ** 3 / (ms * 1e-3) / 1e9,
+                     "hbm_frac": bpc * N ** 3 / (ms * 1e-3) / HBM_PEAK, "finite": bool(torch.isfinite(q).all())}
+        note(f"hydro {prec} {N}^3: {ms:.3f} ms per step")
+        del q, o
+        s.close()
+        torch.cuda.empty_cache()
+    out["fp32_over_fp64"] = out["f32"]["value"] / out["f64"]["value"]
+    return out
+
+
 def note(msg):
     """A progress line on stderr (the JSON record is the only line on stdout)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -696,6 +742,9 @@ def main():
     if not args.no_ct:
         note("fp32 diagnostics compared; constrained transport next")
         rec["ct_precision"] = run_ct_precision(args.ct_N, args.ct_steps)
+    if not args.no_hydro:
+        note("hydro march next")
+        rec["hydro"] = run_hydro(args.hydro_N)
     if not args.no_adjoint:
         note("adjoint next")
         rec["adjoint"] = run_adjoint(args.adj_N, args.adj_steps)

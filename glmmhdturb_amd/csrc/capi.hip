@@ -11,6 +11,7 @@
 #include "aux_launch.hpp"
 #include "ct_launch.hpp"
 #include "host_common.hpp"
+#include "hydro_launch.hpp"
 #include "launch.hpp"
 #include "ou_launch.hpp"
 
@@ -73,6 +74,8 @@ struct glm_ctx {
     int slots;
     int ct_slots_key;
     int ct_slots;
+    int hydro_slots_key;
+    int hydro_slots;
 };
 
 static void ctx_free(glm_ctx* c)
@@ -499,6 +502,40 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out)
     return GLM_OK;
 }
 
+// The 5-variable hydro march (spike_25d.cu:383-516 as build_march.sh builds it).
+int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
+{
+    if (!ctx || !q || !o || !p) return set_err(GLM_ERR_ARG, "glm_hydro_step: null argument");
+    if (!(p->gamma > 1.0) || !(p->dx > 0.0) || !(p->smallr > 0.0) || !(p->dt >= 0.0))
+        return set_err(GLM_ERR_ARG, "glm_hydro_step: need gamma > 1, dx > 0, dt >= 0, smallr > 0");
+    for (int v = 0; v < 5; ++v)
+        if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_hydro_step: null plane pointer");
+    for (int v = 0; v < 5; ++v)
+        for (int w = 0; w < 5; ++w)
+            if (q[v] == o[w]) return set_err(GLM_ERR_ARG, "glm_hydro_step: input and output planes alias");
+    HydroArgsHost h;
+    h.nx = ctx->nx; h.ny = ctx->ny; h.nz = ctx->nz;
+    h.gamma = p->gamma; h.dtdx = p->dt / p->dx; h.smallr = p->smallr;
+    h.smallp = p->pfl > 0.0 ? p->pfl : p->smallr * p->smallc * p->smallc / p->gamma;
+    h.fallback = p->llf_dmin > 0.0 || p->llf_pmin > 0.0;
+    h.fb_dmin = p->llf_dmin > 0.0 ? p->llf_dmin : 0.0;
+    h.fb_pmin = p->llf_pmin > 0.0 ? p->llf_pmin : 0.0;
+    h.zchunk = p->zchunk;
+    const int key = 2000 + h.fallback;
+    if (ctx->hydro_slots_key != key) {
+        int dev = 0, cus = 0, occ = 0;
+        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback) : hydro_march_ptr<float>(h.fallback);
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, HYDRO_THREADS, 0));
+        ctx->hydro_slots = cus * (occ > 0 ? occ : 1);
+        ctx->hydro_slots_key = key;
+    }
+    if (ctx->prec == 8) HIPCHK(launch_hydro_march<double>(q, o, h, ctx->hydro_slots, ctx->stream));
+    else HIPCHK(launch_hydro_march<float>(q, o, h, ctx->hydro_slots, ctx->stream));
+    return GLM_OK;
+}
+
 int glm_last_error(void) { return t_last_error; }
 
 const char* glm_error_string(int code)
@@ -532,7 +569,7 @@ int glm_kernel_info(int precision, int predictor, int riemann, int recon, int sl
 // ------------------------------------------------------------------------------------------
 static std::mutex g_mu;
 static int g_nx = GLM_DEFAULT_N, g_ny = GLM_DEFAULT_N, g_nz = GLM_DEFAULT_N;
-static float g_dtdx = 0.02f, g_ch = 1.0f, g_glmfac = 1.0f, g_gamma = 1.4f;   // spike_mhd.cu:52
+static float g_dtdx = GLM_LIB_NV == 5 ? 0.05f : 0.02f, g_ch = 1.0f, g_glmfac = 1.0f, g_gamma = 1.4f;   // spike_mhd.cu:52, spike_25d.cu:180
 static glm_ctx* g_ctx32 = nullptr;
 static glm_ctx* g_ctx64 = nullptr;
 
@@ -581,7 +618,7 @@ static glm_params legacy_params()
 }
 
 template <typename T>
-static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, bool ct)
+static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kind)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     glm_ctx*& c = sizeof(T) == 8 ? g_ctx64 : g_ctx32;
@@ -594,7 +631,9 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, bool ct
     (void)hipEventRecord(t0, c->stream);
     int err = GLM_OK;
     for (int it = 0; it < nsteps && !err; ++it) {
-        err = ct ? glm_ct_step(c, q.data(), o.data(), &p, 0) : glm_step(c, q.data(), o.data(), &p);
+        err = kind == 1   ? glm_ct_step(c, q.data(), o.data(), &p, 0)
+              : kind == 2 ? glm_hydro_step(c, q.data(), o.data(), &p)
+                          : glm_step(c, q.data(), o.data(), &p);
         std::swap(q, o);
     }
     (void)hipEventRecord(t1, c->stream);
@@ -614,12 +653,22 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, bool ct
 
 extern "C" {
 
-double march_run_dev(float* const* qv, float* const* ov, int nsteps) { return legacy_run<float>(qv, ov, nsteps, 9, false); }
-double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps) { return legacy_run<double>(qv, ov, nsteps, 9, false); }
+// A 5-variable library (make legacy NV=5, build_march.sh) runs the hydro march of spike_25d.cu;
+// the others run the GLM-MHD march of spike_mhd.cu.
+constexpr int LEGACY_NV = GLM_LIB_NV == 5 ? 5 : 9;
+constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : 0;
+double march_run_dev(float* const* qv, float* const* ov, int nsteps)
+{
+    return legacy_run<float>(qv, ov, nsteps, LEGACY_NV, LEGACY_KIND);
+}
+double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps)
+{
+    return legacy_run<double>(qv, ov, nsteps, LEGACY_NV, LEGACY_KIND);
+}
 double ct_run(float* const* qv, float* const* ov, float* const* cc, int nsteps)
 {
     (void)cc;
-    return legacy_run<float>(qv, ov, nsteps, 8, true);
+    return legacy_run<float>(qv, ov, nsteps, 8, 1);
 }
 int ct_regs(void)
 {

@@ -1,0 +1,21 @@
+// hydro_launch.hpp -- host launcher of the 5-variable hydro march (hydro.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace glm {
+
+struct HydroArgsHost {
+    int nx, ny, nz;
+    double gamma, dtdx, smallr, smallp;
+    int fallback;              // HLL with the LLF fallback below fb_dmin / fb_pmin
+    double fb_dmin, fb_pmin;
+    int zchunk;                // z planes per workgroup (0: the occupancy model)
+};
+
+constexpr int HYDRO_THREADS = 256;   // lanes of a hydro march workgroup (hydro_march.hpp HM_NT)
+template <typename T>
+hipError_t launch_hydro_march(void* const* q, void* const* o, const HydroArgsHost& h, int slots, hipStream_t st);
+template <typename T>
+const void* hydro_march_ptr(int fallback);
+
+}  // namespace glm

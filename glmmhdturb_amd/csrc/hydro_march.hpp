@@ -1,0 +1,271 @@
+// hydro_march.hpp -- the 5-variable hydro line march (spike_25d.cu:383-516 as build_march.sh
+// builds it: -DHANCOCK1D, optional -DLLF_FALLBACK) as one z-march kernel for gfx950.
+//
+// Scheme per cell and direction d: primitive state (:196-202), MonCen slopes of the cells at
+// -1, 0, +1 (:207-221), the normal-only Hancock half step (:287-301), face states at mh -/+ s/2
+// clamped at SMALLR/SMALLP (:222-230), HLL (:302-330) or HLL with the LLF fallback (:335-376),
+// div += F(+1/2) - F(-1/2) in the order x, y, z (:486-487), U1 = toCons(prim(U0)) - dtdx*div
+// (:432, :492).  Periodic in all three axes.
+//
+// Layout: 5 SoA planes [rho, mx, my, mz, E], cell (i,j,k) at i + nx*(j + ny*k).
+//
+// Kernel shape.  A workgroup owns a 32 x 8 column tile and a chunk of z planes; lane (tx, ty)
+// owns one column.  Each plane of the tile plus a 2-cell halo (36 x 12 cells) is read from HBM
+// once, converted to primitives once, and kept in a ring of 4 LDS planes: while plane k is
+// updated, plane k+3 loads (one barrier per plane).  The x and y stencils read plane k; the z
+// stencil is the lane's own column of planes k-1..k+2, kept in registers as it passes through
+// the ring.  The z face k+1/2 solved for plane k is the face k+1/2 of plane k+1, so it is
+// carried, not solved again; the first z face of a chunk is solved by the same code in a priming
+// iteration, so every chunking gives the same bits.  x and y faces are solved once per adjacent
+// cell, as in the reference.
+#pragma once
+#include "gpu_util.hpp"
+
+namespace glm {
+
+constexpr int HM_OX = 32, HM_OY = 8, HM_G = 2;
+constexpr int HM_LX = HM_OX + 2 * HM_G, HM_LY = HM_OY + 2 * HM_G;
+constexpr int HM_NPL = HM_LX * HM_LY;   // 432 cells of a haloed plane
+constexpr int HM_NT = HM_OX * HM_OY;    // 256 lanes, one per owned column
+
+template <typename T>
+struct HydroArgs {
+    const T* q[5];
+    T* o[5];
+    int nx, ny, nz;
+    int tiles_x, tiles_y, nzc, zchunk;
+    T gm1, igm1, gamma, dtdx, half_dtdx, smallr, smallp, fb_dmin, fb_pmin;
+};
+
+// spike_25d.cu:196-202 prim_from_cons
+template <typename T>
+GLM_DEV void hy_c2p(const T c[5], T q[5], const HydroArgs<T>& a)
+{
+    const T r = tmax(c[0], a.smallr);
+    const T ir = trcp(r);
+    q[0] = r;
+    q[1] = c[1] * ir;
+    q[2] = c[2] * ir;
+    q[3] = c[3] * ir;
+    const T ke = T(0.5) * r * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    q[4] = tmax(a.gm1 * (c[4] - ke), a.smallp);
+}
+
+// spike_25d.cu:207-213 moncen (full slope)
+template <typename T>
+GLM_DEV T hy_moncen(T l, T m, T r)
+{
+    const T dl = m - l, dr = r - m, dc = T(0.5) * (r - l);
+    const T s = tmin(tabs(dc), tmin(T(2) * tabs(dl), T(2) * tabs(dr)));
+    return dl * dr <= T(0) ? T(0) : (dl > T(0) ? s : -s);
+}
+
+// spike_25d.cu:231-244 primFlux and :245-255 toCons
+template <typename T, int D>
+GLM_DEV void hy_flux_cons(const T q[5], T f[5], T u[5], const HydroArgs<T>& a)
+{
+    const T un = q[1 + D];
+    const T E = q[4] * a.igm1 + T(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const T mu = q[0] * un;
+    f[0] = mu;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[1 + c] = q[0] * q[1 + c] * un + (c == D ? q[4] : T(0));
+    f[4] = (E + q[4]) * un;
+    u[0] = q[0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[1 + c] = q[0] * q[1 + c];
+    u[4] = E;
+}
+
+// State of one side of face: edge(hanc1d(m, s), s, sign) (spike_25d.cu:287-301, :222-230).
+template <typename T, int D>
+GLM_DEV void hy_face_state(const T m[5], const T s[5], T sign, T e[5], const HydroArgs<T>& a)
+{
+    const T un = m[1 + D], sn = s[1 + D];
+    const T ir = trcp(m[0]);
+    const T h = a.half_dtdx;
+    T mh[5];
+    mh[0] = m[0] - h * (un * s[0] + m[0] * sn);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mh[1 + c] = m[1 + c] - h * (un * s[1 + c] + (c == D ? s[4] * ir : T(0)));
+    mh[4] = m[4] - h * (un * s[4] + a.gamma * m[4] * sn);
+    const T hs = sign * T(0.5);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) e[v] = mh[v] + hs * s[v];
+    e[0] = tmax(e[0], a.smallr);
+    e[4] = tmax(e[4], a.smallp);
+}
+
+// HLL (spike_25d.cu:302-330); with FB, LLF where a face state is below the fallback floors
+// (riemann_fb, :335-376).  Both combine the same FL, FR, UL, UR.
+template <typename T, int D, bool FB>
+GLM_DEV void hy_riemann(const T L[5], const T R[5], T F[5], const HydroArgs<T>& a)
+{
+    const T unL = L[1 + D], unR = R[1 + D];
+    const T aL = tsqrt(a.gamma * L[4] * trcp(L[0])), aR = tsqrt(a.gamma * R[4] * trcp(R[0]));
+    T FL[5], FR[5], UL[5], UR[5];
+    hy_flux_cons<T, D>(L, FL, UL, a);
+    hy_flux_cons<T, D>(R, FR, UR, a);
+    const T sL = tmin(unL - aL, unR - aR), sR = tmax(unL + aL, unR + aR);
+    const T inv = trcp(sR - sL), slsr = sL * sR;
+    const bool llf = FB && (L[0] < a.fb_dmin || L[4] < a.fb_pmin || R[0] < a.fb_dmin || R[4] < a.fb_pmin);
+    const T sm = tmax(tabs(unL) + aL, tabs(unR) + aR);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const T hll = (sR * FL[v] - sL * FR[v] + slsr * (UR[v] - UL[v])) * inv;
+        const T h = sL >= T(0) ? FL[v] : (sR <= T(0) ? FR[v] : hll);
+        F[v] = llf ? T(0.5) * (FL[v] + FR[v]) - T(0.5) * sm * (UR[v] - UL[v]) : h;
+    }
+}
+
+// Flux through the face between cells c1 and c2 of the stencil c0 c1 | c2 c3 along D.
+template <typename T, int D, bool FB>
+GLM_DEV void hy_face(const T c0[5], const T c1[5], const T c2[5], const T c3[5], T F[5], const HydroArgs<T>& a)
+{
+    T sl[5], sr[5], L[5], R[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        sl[v] = hy_moncen(c0[v], c1[v], c2[v]);
+        sr[v] = hy_moncen(c1[v], c2[v], c3[v]);
+    }
+    hy_face_state<T, D>(c1, sl, T(1), L, a);
+    hy_face_state<T, D>(c2, sr, T(-1), R, a);
+    hy_riemann<T, D, FB>(L, R, F, a);
+}
+
+// z chunking: the chunk length minimising (waves of workgroups) x (planes + priming) for
+// `slots` resident workgroups, or the requested length.
+inline void hydro_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int& tiles_x, int& tiles_y, int& nzc,
+                         int& zchunk)
+{
+    tiles_x = (nx + HM_OX - 1) / HM_OX;
+    tiles_y = (ny + HM_OY - 1) / HM_OY;
+    if (zchunk_req > 0) {
+        zchunk = zchunk_req < nz ? zchunk_req : nz;
+        nzc = (nz + zchunk - 1) / zchunk;
+        return;
+    }
+    const long cols = (long)tiles_x * tiles_y;
+    const long sl = slots > 0 ? slots : 256;
+    const int maxc = nz / 8 > 1 ? nz / 8 : 1;
+    long best = -1;
+    zchunk = nz;
+    for (int c = 1; c <= maxc; ++c) {
+        const int zc = (nz + c - 1) / c;
+        const int n = (nz + zc - 1) / zc;
+        if (n != c) continue;
+        const long cost = ((cols * n + sl - 1) / sl) * (long)(zc + 2);
+        if (best < 0 || cost < best) { best = cost; zchunk = zc; }
+    }
+    nzc = (nz + zchunk - 1) / zchunk;
+}
+
+template <typename T, bool FB>
+__global__ void __launch_bounds__(HM_NT) hydro_march_kernel(HydroArgs<T> a)
+{
+    // ring of 4 haloed primitive planes: plane j lives in buffer (j - k0 + 2) & 3
+    __shared__ T P[4][5][HM_NPL];
+    const int tid = threadIdx.x;
+    const int tx = tid % HM_OX, ty = tid / HM_OX;
+    const int ntile = a.tiles_x * a.tiles_y;
+    const int w = xcd_block((int)blockIdx.x, (int)gridDim.x);   // neighbouring tiles share an XCD's L2
+    const int tile = w % ntile, zc = w / ntile;
+    const int x0 = (tile % a.tiles_x) * HM_OX, y0 = (tile / a.tiles_x) * HM_OY;
+    const int k0 = zc * a.zchunk;
+    const int k1 = k0 + a.zchunk < a.nz ? k0 + a.zchunk : a.nz;
+    const size_t plane = (size_t)a.nx * a.ny;
+    // the lane's column; lanes past the grid edge compute a wrapped column and store nothing
+    const int gx = x0 + tx, gy = y0 + ty;
+    const bool own = gx < a.nx && gy < a.ny;
+    const size_t col = (size_t)wrapi(gx, a.nx) + (size_t)a.nx * wrapi(gy, a.ny);
+    const int li = (ty + HM_G) * HM_LX + tx + HM_G;
+
+    // every primitive value of the kernel comes from this one conversion, so a cell has the same
+    // bits in every stencil and every chunking
+    auto load_plane = [&](int k) {
+        const int b = (k - k0 + 2) & 3;
+        const size_t zoff = (size_t)wrapi(k, a.nz) * plane;
+        for (int c = tid; c < HM_NPL; c += HM_NT) {
+            const int lx = c % HM_LX, ly = c / HM_LX;
+            const size_t g = zoff + wrapi(x0 - HM_G + lx, a.nx) + (size_t)a.nx * wrapi(y0 - HM_G + ly, a.ny);
+            T cc[5], qq[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) cc[v] = gload(a.q[v] + g);
+            hy_c2p(cc, qq, a);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) P[b][v][c] = qq[v];
+        }
+    };
+    auto own_cell = [&](int k, T qq[5]) {
+        const int b = (k - k0 + 2) & 3;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qq[v] = P[b][v][li];
+    };
+
+    T W[4][5];   // the lane's column at planes k-1 .. k+2 (iteration k)
+    T Fz[5];     // z flux through face k-1/2
+    load_plane(k0 - 2);
+    load_plane(k0 - 1);
+    load_plane(k0);
+    load_plane(k0 + 1);
+    __syncthreads();
+    own_cell(k0 - 2, W[1]);
+    own_cell(k0 - 1, W[2]);
+    own_cell(k0, W[3]);
+    __syncthreads();   // plane k0-2's buffer is reloaded in the first iteration
+
+    // iteration k0-1 only computes the z face k0-1/2 (with the same code as every later face)
+#pragma clang loop unroll(disable)
+    for (int k = k0 - 1; k < k1; ++k) {
+        if (k + 3 < k1 + 2) load_plane(k + 3);   // planes up to k1+1 are read
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            W[0][v] = W[1][v];
+            W[1][v] = W[2][v];
+            W[2][v] = W[3][v];
+        }
+        own_cell(k + 2, W[3]);
+        T div[5] = {T(0), T(0), T(0), T(0), T(0)};
+        const bool upd = k >= k0;
+        if (upd) {
+            const int b = (k - k0 + 2) & 3;
+            T m2[5], m1[5], c0[5], p1[5], p2[5], Fm[5], Fp[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                m2[v] = P[b][v][li - 2]; m1[v] = P[b][v][li - 1]; c0[v] = P[b][v][li];
+                p1[v] = P[b][v][li + 1]; p2[v] = P[b][v][li + 2];
+            }
+            hy_face<T, 0, FB>(m2, m1, c0, p1, Fm, a);
+            hy_face<T, 0, FB>(m1, c0, p1, p2, Fp, a);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fm[v];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                m2[v] = P[b][v][li - 2 * HM_LX]; m1[v] = P[b][v][li - HM_LX];
+                p1[v] = P[b][v][li + HM_LX]; p2[v] = P[b][v][li + 2 * HM_LX];
+            }
+            hy_face<T, 1, FB>(m2, m1, c0, p1, Fm, a);
+            hy_face<T, 1, FB>(m1, c0, p1, p2, Fp, a);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fm[v];
+        }
+        T Fp[5];
+        hy_face<T, 2, FB>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
+        if (upd) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fz[v];
+            T U[5], f[5];
+            hy_flux_cons<T, 0>(W[1], f, U, a);   // toCons(prim(U0)); the flux is unused
+            if (own) {
+                const size_t g = (size_t)k * plane + col;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) a.o[v][g] = U[v] - a.dtdx * div[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Fz[v] = Fp[v];
+        __syncthreads();
+    }
+}
+
+}  // namespace glm

@@ -131,6 +131,19 @@ def step_args(p: Params, dt: float, t: float, *, do_turb=False, riemann="hll", r
     return a
 
 
+def hydro_args(dtdx: float, gamma: float = 1.4, *, smallr: float = 1e-6, smallp: float = 1e-6,
+               fallback: tuple[float, float] | None = None, zchunk: int = 0) -> GLMParams:
+    """Parameters of the 5-variable hydro march (spike_25d.cu:180 constants: gamma 1.4,
+    SMALLR = SMALLP = 1e-6).  fallback = (dmin, pmin) selects HLL with the LLF fallback below
+    those floors (-DLLF_FALLBACK, (1e-3, 1e-3) in spike_25d.cu:339)."""
+    a = GLMParams()
+    _lib.load().glm_params_default(C.byref(a))
+    a.gamma, a.dt, a.dx, a.smallr, a.pfl = gamma, dtdx, 1.0, smallr, smallp
+    a.llf_dmin, a.llf_pmin = fallback if fallback is not None else (0.0, 0.0)
+    a.zchunk, a.want_cfl, a.do_turb = int(zchunk), 0, 0
+    return a
+
+
 def _torch():
     import torch
     return torch
@@ -335,6 +348,13 @@ class Solver:
             args.zchunk = self.ct_zchunk
         fn = self.lib.glm_ct_step_passes if passes else self.lib.glm_ct_step
         check(fn(self.ctx, qa, oa, C.byref(args), int(face_slope)))
+
+    def hydro_step(self, qold, qnew, args: GLMParams):
+        """One step of the 5-variable hydro march (spike_25d.cu march, build_march.sh scheme)
+        on (5, nz, ny, nx) conserved states [rho, mx, my, mz, E]; see hydro_args."""
+        qa, kq = self._planes(qold, 5)
+        oa, ko = self._planes(qnew, 5)
+        check(self.lib.glm_hydro_step(self.ctx, qa, oa, C.byref(args)))
 
     def ct_ctot_max(self, q) -> float:
         pa, keep = self._planes(q, 8)

@@ -29,7 +29,8 @@ extern "C" {
 /*    GLM cleaning and psi damping, smallr = smallp = 1e-6 (spike_mhd.cu:52, :257-288).    */
 /* ===================================================================================== */
 
-/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library) or 8 (CT library, spike_ctm.cu:301) */
+/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library), 8 (CT library, spike_ctm.cu:301) or 5
+ * (hydro library, spike_25d.cu:560) */
 int march_nv(void);
 /* spike_mhd.cu:322  grid size N (compile-time NX in the reference; runtime here, see march_set_grid) */
 int march_nx(void);
@@ -42,7 +43,10 @@ void march_set_glm(float ch, float fac);
 /* spike_mhd.cu:326  adiabatic index (default 1.4, spike_mhd.cu:52) */
 void march_set_gamma(float g);
 /* spike_mhd.cu:328-339  run nsteps periodic marches over device planes qv[0..8]; ov is scratch of
- * the same shape; the result is always left in qv.  Returns elapsed GPU time in ms (HIP events). */
+ * the same shape; the result is always left in qv.  Returns elapsed GPU time in ms (HIP events).
+ * In a 5-plane library (build_march.sh) this is spike_25d.cu:563-575: the hydro march over
+ * qv[0..4] = [rho, mx, my, mz, E] (normal-only Hancock, PLM MonCen, HLL, smallr = smallp = 1e-6,
+ * default gamma 1.4 and dt/dx 0.05, spike_25d.cu:180). */
 double march_run_dev(float* const* qv, float* const* ov, int nsteps);
 
 /* No reference counterpart (the reference fixes NX/NY/NZ at compile time, build_mhd.sh:26-28):
@@ -154,6 +158,14 @@ int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_p
 int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
 /* max |div B| of a CT state from the face fields (ct_mhd.jl:191-194); synchronous. */
 int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
+
+/* The 5-variable hydro line march (spike_25d.cu:383-516, the march kernel as build_march.sh
+ * builds it with -DHANCOCK1D; -DLLF_FALLBACK when llf_dmin or llf_pmin > 0): q, o = 5 planes
+ * [rho, mx, my, mz, E] (distinct buffers).  Uses gamma, dt/dx, smallr (SMALLR), pfl (SMALLP;
+ * <= 0 selects smallr*smallc^2/gamma), llf_dmin/llf_pmin (the fallback's density and pressure
+ * floors, 1e-3 in spike_25d.cu:339) and zchunk from p; the other fields are ignored.
+ * Asynchronous on the context stream.  Returns 0 or an error code. */
+int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
 
 /* ===================================================================================== */
 /* 4. Adjoint (no reference counterpart: the reference computes no derivatives).             */

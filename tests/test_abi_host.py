@@ -190,3 +190,17 @@ def test_build_scripts_reference_hipcc_for_gfx950():
     for script in ("build_mhd.sh", "build_ct.sh"):
         txt = open(os.path.join(ROOT, "glmmhdturb_amd", script)).read()
         assert "legacy" in txt and "gfx950" in txt
+
+
+def test_hydro_library_exports_the_bridge_symbols():
+    """libmarch480.so (build_march.sh): the symbols MarchBridge.open_lib resolves
+    (MarchBridge.jl:34-41, spike_25d.cu:559-576) with march_nv = 5, no GPU calls."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", "libmarch480.so")
+    assert os.path.exists(path), "build it with glmmhdturb_amd/build_march.sh"
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    for sym in ("march_nv", "march_nx", "march_set_dtdx", "march_run_dev", "march_set_gamma", "march_set_grid",
+                "march_run_dev_f64", "glm_hydro_step"):
+        assert sym in exported, sym
+    lib = C.CDLL(path)
+    assert lib.march_nv() == 5 and lib.march_nx() == 480

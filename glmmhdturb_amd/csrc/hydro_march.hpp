@@ -11,8 +11,8 @@
 //
 // Kernel shape.  A workgroup owns a 32 x 8 column tile and a chunk of z planes; lane (tx, ty)
 // owns one column.  Each plane of the tile plus a 2-cell halo (36 x 12 cells) is read from HBM
-// once, converted to primitives once, and kept in a ring of 4 LDS planes: while plane k is
-// updated, plane k+3 loads (one barrier per plane).  The x and y stencils read plane k; the z
+// once, converted to primitives once, and kept in a ring of 4 LDS planes: plane k+3 is fetched
+// before plane k is updated and stored after it (one barrier per plane).  The x and y stencils read plane k; the z
 // stencil is the lane's own column of planes k-1..k+2, kept in registers as it passes through
 // the ring.  The z face k+1/2 solved for plane k is the face k+1/2 of plane k+1, so it is
 // carried, not solved again; the first z face of a chunk is solved by the same code in a priming
@@ -41,6 +41,7 @@ struct HydroArgs {
 template <typename T>
 GLM_DEV void hy_c2p(const T c[5], T q[5], const HydroArgs<T>& a)
 {
+#pragma clang fp contract(off)   // the same bits at every call site of the kernel
     const T r = tmax(c[0], a.smallr);
     const T ir = trcp(r);
     q[0] = r;
@@ -160,8 +161,13 @@ inline void hydro_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int&
     nzc = (nz + zchunk - 1) / zchunk;
 }
 
+// Resident waves per SIMD the register allocation is held to: two workgroups per CU in FP64
+// (2 x 69 KB of LDS), four in FP32.
+#define GLM_HYDRO_WAVES_PER_EU(T) (sizeof(T) == 8 ? 2 : 4)
+
 template <typename T, bool FB>
-__global__ void __launch_bounds__(HM_NT) hydro_march_kernel(HydroArgs<T> a)
+__global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T))))
+hydro_march_kernel(HydroArgs<T> a)
 {
     // ring of 4 haloed primitive planes: plane j lives in buffer (j - k0 + 2) & 3
     __shared__ T P[4][5][HM_NPL];
@@ -180,21 +186,40 @@ __global__ void __launch_bounds__(HM_NT) hydro_march_kernel(HydroArgs<T> a)
     const size_t col = (size_t)wrapi(gx, a.nx) + (size_t)a.nx * wrapi(gy, a.ny);
     const int li = (ty + HM_G) * HM_LX + tx + HM_G;
 
-    // every primitive value of the kernel comes from this one conversion, so a cell has the same
-    // bits in every stencil and every chunking
-    auto load_plane = [&](int k) {
-        const int b = (k - k0 + 2) & 3;
+    // every primitive value of the kernel comes from hy_c2p, which is not contracted, so a cell
+    // has the same bits in every stencil and every chunking.  A plane is fetched from HBM into
+    // registers (`pre`, two cells per lane) at the top of an iteration and converted into its
+    // LDS buffer at the bottom, so the fetch latency is hidden behind the plane's compute.
+    T pre[2][5];
+    auto fetch = [&](int k) {
         const size_t zoff = (size_t)wrapi(k, a.nz) * plane;
-        for (int c = tid; c < HM_NPL; c += HM_NT) {
-            const int lx = c % HM_LX, ly = c / HM_LX;
-            const size_t g = zoff + wrapi(x0 - HM_G + lx, a.nx) + (size_t)a.nx * wrapi(y0 - HM_G + ly, a.ny);
-            T cc[5], qq[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) cc[v] = gload(a.q[v] + g);
-            hy_c2p(cc, qq, a);
+        for (int r = 0; r < 2; ++r) {
+            const int c = tid + r * HM_NT;
+            if (c < HM_NPL) {
+                const int lx = c % HM_LX, ly = c / HM_LX;
+                const size_t g = zoff + wrapi(x0 - HM_G + lx, a.nx) + (size_t)a.nx * wrapi(y0 - HM_G + ly, a.ny);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) P[b][v][c] = qq[v];
+                for (int v = 0; v < 5; ++v) pre[r][v] = gload(a.q[v] + g);
+            }
         }
+    };
+    auto commit = [&](int k) {
+        const int b = (k - k0 + 2) & 3;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int c = tid + r * HM_NT;
+            if (c < HM_NPL) {
+                T qq[5];
+                hy_c2p(pre[r], qq, a);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) P[b][v][c] = qq[v];
+            }
+        }
+    };
+    auto load_plane = [&](int k) {
+        fetch(k);
+        commit(k);
     };
     auto own_cell = [&](int k, T qq[5]) {
         const int b = (k - k0 + 2) & 3;
@@ -217,7 +242,8 @@ __global__ void __launch_bounds__(HM_NT) hydro_march_kernel(HydroArgs<T> a)
     // iteration k0-1 only computes the z face k0-1/2 (with the same code as every later face)
 #pragma clang loop unroll(disable)
     for (int k = k0 - 1; k < k1; ++k) {
-        if (k + 3 < k1 + 2) load_plane(k + 3);   // planes up to k1+1 are read
+        const bool more = k + 3 <= k1 + 1;   // planes up to k1+1 are read
+        if (more) fetch(k + 3);
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             W[0][v] = W[1][v];
@@ -264,6 +290,7 @@ __global__ void __launch_bounds__(HM_NT) hydro_march_kernel(HydroArgs<T> a)
         }
 #pragma unroll
         for (int v = 0; v < 5; ++v) Fz[v] = Fp[v];
+        if (more) commit(k + 3);   // into plane k-1's buffer, last read before the previous barrier
         __syncthreads();
     }
 }

@@ -78,18 +78,23 @@ GLM_DEV void hy_flux_cons(const T q[5], T f[5], T u[5], const HydroArgs<T>& a)
     u[4] = E;
 }
 
-// State of one side of face: edge(hanc1d(m, s), s, sign) (spike_25d.cu:287-301, :222-230).
+// spike_25d.cu:287-301 hanc1d: normal-only Hancock half step along D
 template <typename T, int D>
-GLM_DEV void hy_face_state(const T m[5], const T s[5], T sign, T e[5], const HydroArgs<T>& a)
+GLM_DEV void hy_hanc1d(const T m[5], const T s[5], T mh[5], const HydroArgs<T>& a)
 {
     const T un = m[1 + D], sn = s[1 + D];
     const T ir = trcp(m[0]);
     const T h = a.half_dtdx;
-    T mh[5];
     mh[0] = m[0] - h * (un * s[0] + m[0] * sn);
 #pragma unroll
     for (int c = 0; c < 3; ++c) mh[1 + c] = m[1 + c] - h * (un * s[1 + c] + (c == D ? s[4] * ir : T(0)));
     mh[4] = m[4] - h * (un * s[4] + a.gamma * m[4] * sn);
+}
+
+// spike_25d.cu:222-230 edge: mh + sign*s/2, clamped at SMALLR/SMALLP
+template <typename T>
+GLM_DEV void hy_edge(const T mh[5], const T s[5], T sign, T e[5], const HydroArgs<T>& a)
+{
     const T hs = sign * T(0.5);
 #pragma unroll
     for (int v = 0; v < 5; ++v) e[v] = mh[v] + hs * s[v];
@@ -119,19 +124,49 @@ GLM_DEV void hy_riemann(const T L[5], const T R[5], T F[5], const HydroArgs<T>& 
     }
 }
 
+template <typename T>
+GLM_DEV void hy_slopes(const T l[5], const T m[5], const T r[5], T s[5])
+{
+#pragma unroll
+    for (int v = 0; v < 5; ++v) s[v] = hy_moncen(l[v], m[v], r[v]);
+}
+
 // Flux through the face between cells c1 and c2 of the stencil c0 c1 | c2 c3 along D.
 template <typename T, int D, bool FB>
 GLM_DEV void hy_face(const T c0[5], const T c1[5], const T c2[5], const T c3[5], T F[5], const HydroArgs<T>& a)
 {
-    T sl[5], sr[5], L[5], R[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        sl[v] = hy_moncen(c0[v], c1[v], c2[v]);
-        sr[v] = hy_moncen(c1[v], c2[v], c3[v]);
-    }
-    hy_face_state<T, D>(c1, sl, T(1), L, a);
-    hy_face_state<T, D>(c2, sr, T(-1), R, a);
+    T sl[5], sr[5], mh[5], L[5], R[5];
+    hy_slopes(c0, c1, c2, sl);
+    hy_slopes(c1, c2, c3, sr);
+    hy_hanc1d<T, D>(c1, sl, mh, a);
+    hy_edge(mh, sl, T(1), L, a);
+    hy_hanc1d<T, D>(c2, sr, mh, a);
+    hy_edge(mh, sr, T(-1), R, a);
     hy_riemann<T, D, FB>(L, R, F, a);
+}
+
+// div += F(+1/2) - F(-1/2) along D for the cell c0 of the stencil m2 m1 c0 p1 p2: the centre
+// cell's slope and half step are shared by its two faces (spike_25d.cu:462-466 forms them once
+// per face).
+template <typename T, int D, bool FB>
+GLM_DEV void hy_dir(const T m2[5], const T m1[5], const T c0[5], const T p1[5], const T p2[5], T div[5],
+                    const HydroArgs<T>& a)
+{
+    T s[5], mh[5], L[5], R[5], Fm[5], Fp[5];
+    hy_slopes(m2, m1, c0, s);
+    hy_hanc1d<T, D>(m1, s, mh, a);
+    hy_edge(mh, s, T(1), L, a);
+    hy_slopes(m1, c0, p1, s);
+    hy_hanc1d<T, D>(c0, s, mh, a);
+    hy_edge(mh, s, T(-1), R, a);
+    hy_riemann<T, D, FB>(L, R, Fm, a);
+    hy_edge(mh, s, T(1), L, a);
+    hy_slopes(c0, p1, p2, s);
+    hy_hanc1d<T, D>(p1, s, mh, a);
+    hy_edge(mh, s, T(-1), R, a);
+    hy_riemann<T, D, FB>(L, R, Fp, a);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fm[v];
 }
 
 // z chunking: the chunk length minimising (waves of workgroups) x (planes + priming) for
@@ -255,25 +290,19 @@ hydro_march_kernel(HydroArgs<T> a)
         const bool upd = k >= k0;
         if (upd) {
             const int b = (k - k0 + 2) & 3;
-            T m2[5], m1[5], c0[5], p1[5], p2[5], Fm[5], Fp[5];
+            T m2[5], m1[5], c0[5], p1[5], p2[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 m2[v] = P[b][v][li - 2]; m1[v] = P[b][v][li - 1]; c0[v] = P[b][v][li];
                 p1[v] = P[b][v][li + 1]; p2[v] = P[b][v][li + 2];
             }
-            hy_face<T, 0, FB>(m2, m1, c0, p1, Fm, a);
-            hy_face<T, 0, FB>(m1, c0, p1, p2, Fp, a);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fm[v];
+            hy_dir<T, 0, FB>(m2, m1, c0, p1, p2, div, a);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 m2[v] = P[b][v][li - 2 * HM_LX]; m1[v] = P[b][v][li - HM_LX];
                 p1[v] = P[b][v][li + HM_LX]; p2[v] = P[b][v][li + 2 * HM_LX];
             }
-            hy_face<T, 1, FB>(m2, m1, c0, p1, Fm, a);
-            hy_face<T, 1, FB>(m1, c0, p1, p2, Fp, a);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fm[v];
+            hy_dir<T, 1, FB>(m2, m1, c0, p1, p2, div, a);
         }
         T Fp[5];
         hy_face<T, 2, FB>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2

@@ -10,6 +10,10 @@
  * Each face is solved once per adjacent cell, as in the reference (the two solves see the
  * same inputs).  Cell (i,j,k) at i + nx*(j + ny*k); plane v at v*nx*ny*nz.
  *
+ * Passive scalars (-DSCALARS, nsp = 2): planes 5.. hold rho*s, reconstructed as s = (rho*s)/rho
+ * and advected with the flow; their flux is the HLL (or LLF) flux of rho*s, or with `cma`
+ * (-DCMA, consistent multi-fluid advection) the mass flux times the upwind face value of s.
+ *
  * Compiled with -DREAL=double and -DREAL=float, -ffp-contract=off.
  */
 #include <math.h>
@@ -21,15 +25,19 @@
 #endif
 typedef REAL real;
 #define R(x) ((real)(x))
-#define HV 5
+#define HV 7   /* largest state: 5 hydro variables + 2 passive scalars */
 
 typedef struct {
     double gamma, dtdx, smallr, smallp; /* spike_25d.cu:180 constants */
     int fallback;                       /* 0: HLL (:302-330); 1: riemann_fb (:335-376) */
     double fb_dmin, fb_pmin;            /* the fallback's floors (1e-3 in :339) */
+    int nsp;                            /* passive scalars: 0 or 2 (-DSCALARS, :115-119) */
+    int cma;                            /* scalar flux rides the mass flux (-DCMA, :321, :367) */
 } HyScheme;
 
-/* spike_25d.cu:196-202 prim_from_cons */
+static int nvar(const HyScheme* s) { return 5 + s->nsp; }
+
+/* spike_25d.cu:196-202 prim_from_cons (:183-194 with scalars) */
 void oh_c2p(const real c[HV], real q[HV], const HyScheme* s)
 {
     real g = R(s->gamma), sr = R(s->smallr), sp = R(s->smallp);
@@ -40,6 +48,7 @@ void oh_c2p(const real c[HV], real q[HV], const HyScheme* s)
     q[3] = c[3] * ir;
     real ke = R(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     q[4] = fmax((g - R(1)) * (c[4] - ke), sp);
+    for (int v = 5; v < nvar(s); ++v) q[v] = c[v] * ir;   /* :191 */
 }
 
 /* spike_25d.cu:207-213 moncen: the full monotonized-central slope */
@@ -52,15 +61,15 @@ real oh_moncen(real a, real b, real c)
 }
 
 /* spike_25d.cu:214-221 slopeP */
-static void slope5(const real a[HV], const real b[HV], const real c[HV], real sl[HV])
+static void slope5(const real a[HV], const real b[HV], const real c[HV], real sl[HV], const HyScheme* s)
 {
-    for (int v = 0; v < HV; ++v) sl[v] = oh_moncen(a[v], b[v], c[v]);
+    for (int v = 0; v < nvar(s); ++v) sl[v] = oh_moncen(a[v], b[v], c[v]);
 }
 
 /* spike_25d.cu:222-230 edge: b + sign*s/2, clamped at SMALLR/SMALLP */
 void oh_edge(const real b[HV], const real sl[HV], real sign, real e[HV], const HyScheme* s)
 {
-    for (int v = 0; v < HV; ++v) e[v] = b[v] + sign * R(0.5) * sl[v];
+    for (int v = 0; v < nvar(s); ++v) e[v] = b[v] + sign * R(0.5) * sl[v];
     e[0] = fmax(e[0], R(s->smallr));
     e[4] = fmax(e[4], R(s->smallp));
 }
@@ -74,6 +83,7 @@ void oh_flux(const real q[HV], int dir, real f[HV], const HyScheme* s)
     f[0] = q[0] * un;
     for (int c = 0; c < 3; ++c) f[1 + c] = q[0] * q[1 + c] * un + (dir == c ? q[4] : R(0));
     f[4] = (E + q[4]) * un;
+    for (int v = 5; v < nvar(s); ++v) f[v] = q[0] * un * q[v];   /* :241 */
 }
 
 /* spike_25d.cu:245-255 toCons */
@@ -85,6 +95,7 @@ void oh_tocons(const real q[HV], real c[HV], const HyScheme* s)
     c[2] = q[0] * q[2];
     c[3] = q[0] * q[3];
     c[4] = q[4] / (g - R(1)) + R(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int v = 5; v < nvar(s); ++v) c[v] = q[0] * q[v];   /* :251 */
 }
 
 /* spike_25d.cu:287-301 hanc1d: normal-only Hancock half step in direction dir */
@@ -97,6 +108,7 @@ void oh_hanc1d(const real m[HV], const real sl[HV], int dir, real mh[HV], const 
     mh[0] = m[0] - h * (un * sl[0] + m[0] * sn);
     for (int c = 0; c < 3; ++c) mh[1 + c] = m[1 + c] - h * (un * sl[1 + c] + (dir == c ? sl[4] * ir : R(0)));
     mh[4] = m[4] - h * (un * sl[4] + g * m[4] * sn);
+    for (int v = 5; v < nvar(s); ++v) mh[v] = m[v] - h * un * sl[v];   /* :298 */
 }
 
 /* spike_25d.cu:302-330 hll (the scalar part of the 5-variable build) */
@@ -111,10 +123,12 @@ void oh_hll(const real L[HV], const real Rt[HV], int dir, real F[HV], const HySc
     oh_flux(Rt, dir, FR, s);
     oh_tocons(L, UL, s);
     oh_tocons(Rt, UR, s);
-    if (sL >= R(0)) { for (int v = 0; v < HV; ++v) F[v] = FL[v]; return; }
-    if (sR <= R(0)) { for (int v = 0; v < HV; ++v) F[v] = FR[v]; return; }
+    if (sL >= R(0)) { for (int v = 0; v < nvar(s); ++v) F[v] = FL[v]; return; }
+    if (sR <= R(0)) { for (int v = 0; v < nvar(s); ++v) F[v] = FR[v]; return; }
     real inv = R(1) / (sR - sL);
-    for (int v = 0; v < HV; ++v) F[v] = (sR * FL[v] - sL * FR[v] + sL * sR * (UR[v] - UL[v])) * inv;
+    for (int v = 0; v < nvar(s); ++v) F[v] = (sR * FL[v] - sL * FR[v] + sL * sR * (UR[v] - UL[v])) * inv;
+    if (s->cma)   /* :321 */
+        for (int v = 5; v < nvar(s); ++v) F[v] = F[0] >= R(0) ? F[0] * L[v] : F[0] * Rt[v];
 }
 
 /* spike_25d.cu:335-376 riemann_fb: LLF where a face state is below the fallback floors,
@@ -132,7 +146,7 @@ void oh_riemann_fb(const real L[HV], const real Rt[HV], int dir, real F[HV], con
         oh_tocons(L, UL, s);
         oh_tocons(Rt, UR, s);
         real sm = fmax(fabs(unL) + aL, fabs(unR) + aR);
-        for (int v = 0; v < HV; ++v) F[v] = R(0.5) * (FL[v] + FR[v]) - R(0.5) * sm * (UR[v] - UL[v]);
+        for (int v = 0; v < nvar(s); ++v) F[v] = R(0.5) * (FL[v] + FR[v]) - R(0.5) * sm * (UR[v] - UL[v]);
         return;
     }
     oh_hll(L, Rt, dir, F, s);
@@ -153,14 +167,14 @@ static void prim_all(const real* u, real* prim, int nx, int ny, int nz, const Hy
 #pragma omp parallel for schedule(static)
     for (long c = 0; c < (long)n; ++c) {
         real cc[HV], q[HV];
-        for (int v = 0; v < HV; ++v) cc[v] = u[(size_t)v * n + c];
+        for (int v = 0; v < nvar(s); ++v) cc[v] = u[(size_t)v * n + c];
         oh_c2p(cc, q, s);
-        for (int v = 0; v < HV; ++v) prim[(size_t)v * n + c] = q[v];
+        for (int v = 0; v < nvar(s); ++v) prim[(size_t)v * n + c] = q[v];
     }
 }
 
 /* Scratch reals oh_step needs: the primitive planes. */
-size_t oh_step_scratch_reals(int nx, int ny, int nz) { return (size_t)HV * nx * ny * nz; }
+size_t oh_step_scratch_reals(int nx, int ny, int nz) { return (size_t)HV * nx * ny * nz; }   /* enough for nsp = 2 */
 
 /* One step of the march, spike_25d.cu:430-516 (HANCOCK1D branch :462-466, update :492,
  * store :512).  Returns 0. */
@@ -176,20 +190,20 @@ int oh_step(const real* u, real* out, int nx, int ny, int nz, const HyScheme* s,
             for (int i = 0; i < nx; ++i) {
                 size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
                 real c0[HV];
-                for (int v = 0; v < HV; ++v) c0[v] = prim[(size_t)v * n + c];
-                real div[HV] = {0, 0, 0, 0, 0};
+                for (int v = 0; v < nvar(s); ++v) c0[v] = prim[(size_t)v * n + c];
+                real div[HV] = {0, 0, 0, 0, 0, 0, 0};
                 for (int d = 0; d < 3; ++d) {
                     int ox = d == 0, oy = d == 1, oz = d == 2;
                     real P[5][HV];   /* cells at offsets -2..+2 along d */
                     for (int o = -2; o <= 2; ++o) {
                         size_t cc = wrapi(i + o * ox, nx) +
                                     (size_t)nx * (wrapi(j + o * oy, ny) + (size_t)ny * wrapi(k + o * oz, nz));
-                        for (int v = 0; v < HV; ++v) P[o + 2][v] = prim[(size_t)v * n + cc];
+                        for (int v = 0; v < nvar(s); ++v) P[o + 2][v] = prim[(size_t)v * n + cc];
                     }
                     real sm1[HV], s0[HV], sp1[HV];
-                    slope5(P[0], P[1], P[2], sm1);
-                    slope5(P[1], P[2], P[3], s0);
-                    slope5(P[2], P[3], P[4], sp1);
+                    slope5(P[0], P[1], P[2], sm1, s);
+                    slope5(P[1], P[2], P[3], s0, s);
+                    slope5(P[2], P[3], P[4], sp1, s);
                     real mhm[HV], mh0[HV], mhp[HV];
                     oh_hanc1d(P[1], sm1, d, mhm, s);
                     oh_hanc1d(P[2], s0, d, mh0, s);
@@ -201,11 +215,11 @@ int oh_step(const real* u, real* out, int nx, int ny, int nz, const HyScheme* s,
                     oh_edge(mh0, s0, R(1), a, s);
                     oh_edge(mhp, sp1, R(-1), b, s);
                     solve(a, b, d, Fp, s);
-                    for (int v = 0; v < HV; ++v) div[v] += Fp[v] - Fm[v];
+                    for (int v = 0; v < nvar(s); ++v) div[v] += Fp[v] - Fm[v];
                 }
                 real U[HV];
                 oh_tocons(c0, U, s);
-                for (int v = 0; v < HV; ++v) out[(size_t)v * n + c] = U[v] - dtdx * div[v];
+                for (int v = 0; v < nvar(s); ++v) out[(size_t)v * n + c] = U[v] - dtdx * div[v];
             }
     return 0;
 }

@@ -48,12 +48,13 @@ class OCTScheme(C.Structure):
 
 class OHyScheme(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "dtdx", "smallr", "smallp")] + [
-        ("fallback", C.c_int), ("fb_dmin", C.c_double), ("fb_pmin", C.c_double)]
+        ("fallback", C.c_int), ("fb_dmin", C.c_double), ("fb_pmin", C.c_double), ("nsp", C.c_int), ("cma", C.c_int)]
 
 
 @dataclass
 class HydroScheme:
-    """The 5-variable hydro march (spike_25d.cu:180 constants; LLF fallback :335-376)."""
+    """The 5-variable hydro march (spike_25d.cu:180 constants; LLF fallback :335-376), with nsp = 2
+    passive scalars (-DSCALARS) and their flux riding the mass flux with cma (-DCMA)."""
     gamma: float = 1.4
     dtdx: float = 0.05
     smallr: float = 1e-6
@@ -61,10 +62,12 @@ class HydroScheme:
     fallback: bool = False
     fb_dmin: float = 1e-3
     fb_pmin: float = 1e-3
+    nsp: int = 0
+    cma: bool = False
 
     def to_c(self) -> OHyScheme:
         return OHyScheme(self.gamma, self.dtdx, self.smallr, self.smallp, int(self.fallback), self.fb_dmin,
-                         self.fb_pmin)
+                         self.fb_pmin, self.nsp, int(self.cma))
 
 
 @dataclass
@@ -378,12 +381,13 @@ class Oracle:
         return self.lib.o_ct_ctot_max(_ptr(self.arr(q)), nx, ny, nz, gamma, smallr, smallp)
 
     # ---- 5-variable hydro march ----------------------------------------------------
-    def _hy(self, name, *vecs, extra=(), sc: HydroScheme, nout=5):
+    def _hy(self, name, *vecs, extra=(), sc: HydroScheme):
         cs = sc.to_c()
-        out = self.vec(nout)
-        args = [_ptr(self.arr(v, 5)) for v in vecs]
+        nv = 5 + sc.nsp
+        out = self.vec(7)
+        args = [_ptr(np.pad(self.arr(v, nv), (0, 7 - nv))) for v in vecs]
         getattr(self.lib, name)(*args, *extra, _ptr(out), C.byref(cs))
-        return out
+        return out[:nv]
 
     def hydro_c2p(self, c, sc: HydroScheme):
         return self._hy("oh_c2p", c, sc=sc)
@@ -405,17 +409,19 @@ class Oracle:
 
     def hydro_edge(self, b, s, sign, sc: HydroScheme):
         cs = sc.to_c()
-        out = self.vec(5)
-        self.lib.oh_edge(_ptr(self.arr(b, 5)), _ptr(self.arr(s, 5)), sign, _ptr(out), C.byref(cs))
-        return out
+        nv = 5 + sc.nsp
+        out = self.vec(7)
+        b7, s7 = np.pad(self.arr(b, nv), (0, 7 - nv)), np.pad(self.arr(s, nv), (0, 7 - nv))
+        self.lib.oh_edge(_ptr(b7), _ptr(s7), sign, _ptr(out), C.byref(cs))
+        return out[:nv]
 
     def hydro_moncen(self, a, b, c):
         return self.lib.oh_moncen(a, b, c)
 
     def hydro_step(self, u, sc: HydroScheme):
-        """One step of the march on a (5, nz, ny, nx) conserved state."""
+        """One step of the march on a (5 + nsp, nz, ny, nx) conserved state."""
         nv, nz, ny, nx = u.shape
-        assert nv == 5, nv
+        assert nv == 5 + sc.nsp, nv
         u = self.arr(u)
         out = np.zeros_like(u)
         scratch = np.empty(self.lib.oh_step_scratch_reals(nx, ny, nz), dtype=self.dtype)

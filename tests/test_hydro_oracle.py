@@ -285,3 +285,62 @@ def test_sound_wave_second_order(orc, axis):
     dt cuts the error of a smooth acoustic wave by well over 2 (first-order terms would not)."""
     e32, e64 = sound_wave_error(orc, 32, axis), sound_wave_error(orc, 64, axis)
     assert e32 < 0.05 and e64 < 0.33 * e32, (e32, e64)
+
+
+def with_scalars(u, s0, s1):
+    return np.concatenate([u, (u[0] * s0)[None], (u[0] * s1)[None]])
+
+
+@pytest.mark.parametrize("cma", [False, True])
+def test_scalars_constant_fraction_and_conservation(orc, cma):
+    """A uniform mass fraction stays uniform in any flow, the scalars are conserved, and the
+    hydro variables do not see them (spike_25d.cu -DSCALARS / -DCMA)."""
+    sc = HydroScheme(gamma=G, dtdx=0.15, nsp=2, cma=cma)
+    u = smooth_hydro(12, 10, 8, 9)
+    us = with_scalars(u, 0.3, 0.7)
+    for _ in range(3):
+        us = orc.hydro_step(us, sc)
+    assert np.allclose(us[5] / us[0], 0.3, rtol=1e-13) and np.allclose(us[6] / us[0], 0.7, rtol=1e-13)
+    ref = u
+    for _ in range(3):
+        ref = orc.hydro_step(ref, HydroScheme(gamma=G, dtdx=0.15))
+    assert np.array_equal(us[:5], ref)
+
+
+@pytest.mark.parametrize("cma", [False, True])
+def test_scalars_sum_and_advection(orc, cma):
+    """Fractions X and 1 - X keep summing to one; a scalar profile carried by a uniform flow comes
+    back after one crossing with second-order convergence."""
+    sc = HydroScheme(gamma=G, dtdx=0.15, nsp=2, cma=cma)
+    u = smooth_hydro(12, 10, 8, 10)
+    rng = np.random.default_rng(11)
+    x0 = 0.5 + 0.3 * np.tanh(rng.normal(0, 1, u[0].shape))
+    us = with_scalars(u, x0, 1 - x0)
+    tot0 = us[5:].sum(axis=(1, 2, 3))
+    for _ in range(4):
+        us = orc.hydro_step(us, sc)
+    assert np.allclose((us[5] + us[6]) / us[0], 1.0, rtol=1e-13)
+    assert np.allclose(us[5:].sum(axis=(1, 2, 3)), tot0, rtol=1e-13)
+
+    def err(n):
+        x = (np.arange(n) + 0.5) / n
+        prof = 0.5 + 0.2 * np.sin(2 * np.pi * x)
+        line = cons_of(np.ones(n), np.full(n, 0.5), np.zeros(n), np.zeros(n), np.ones(n))
+        v = with_scalars(line, prof, 1 - prof).reshape(7, 1, 1, n)
+        steps = int(2.5 * n)
+        s2 = HydroScheme(gamma=G, dtdx=n * 2.0 / steps, nsp=2, cma=cma)   # 0.5 * t = 1 crossing
+        w = v
+        for _ in range(steps):
+            w = orc.hydro_step(w, s2)
+        return np.abs(w[5] / w[0] - v[5] / v[0]).mean()
+    e32, e64 = err(32), err(64)
+    assert e32 < 0.02 and e64 < 0.33 * e32, (e32, e64)
+
+
+def test_cma_flux_rides_the_mass_flux(orc):
+    L = np.array([1.0, 0.2, 0.0, 0.0, 1.0, 0.3, 0.6])
+    R_ = np.array([0.8, 0.1, 0.0, 0.0, 0.9, 0.9, 0.05])
+    f = orc.hydro_hll(L, R_, 0, HydroScheme(gamma=G, nsp=2, cma=True))
+    assert f[0] > 0 and f[5] == f[0] * L[5] and f[6] == f[0] * L[6]
+    g = orc.hydro_hll(L, R_, 0, HydroScheme(gamma=G, nsp=2))
+    assert np.array_equal(f[:5], g[:5]) and not np.isclose(f[5], g[5])

@@ -502,17 +502,36 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out)
     return GLM_OK;
 }
 
-// The 5-variable hydro march (spike_25d.cu:383-516 as build_march.sh builds it).
-int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
+// The hydro march (spike_25d.cu:383-516 as build_march.sh builds it), with nsp = 0 or 2 passive
+// scalars (-DSCALARS) whose flux rides the mass flux when cma is set (-DCMA).
+static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma,
+                           const char* who)
 {
-    if (!ctx || !q || !o || !p) return set_err(GLM_ERR_ARG, "glm_hydro_step: null argument");
-    if (!(p->gamma > 1.0) || !(p->dx > 0.0) || !(p->smallr > 0.0) || !(p->dt >= 0.0))
-        return set_err(GLM_ERR_ARG, "glm_hydro_step: need gamma > 1, dx > 0, dt >= 0, smallr > 0");
-    for (int v = 0; v < 5; ++v)
-        if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_hydro_step: null plane pointer");
-    for (int v = 0; v < 5; ++v)
-        for (int w = 0; w < 5; ++w)
-            if (q[v] == o[w]) return set_err(GLM_ERR_ARG, "glm_hydro_step: input and output planes alias");
+    char msg[160];
+    if (!ctx || !q || !o || !p) {
+        snprintf(msg, sizeof msg, "%s: null argument", who);
+        return set_err(GLM_ERR_ARG, msg);
+    }
+    if (nsp != 0 && nsp != 2) {
+        snprintf(msg, sizeof msg, "%s: nsp must be 0 or 2", who);
+        return set_err(GLM_ERR_ARG, msg);
+    }
+    if (!(p->gamma > 1.0) || !(p->dx > 0.0) || !(p->smallr > 0.0) || !(p->dt >= 0.0)) {
+        snprintf(msg, sizeof msg, "%s: need gamma > 1, dx > 0, dt >= 0, smallr > 0", who);
+        return set_err(GLM_ERR_ARG, msg);
+    }
+    const int nv = 5 + nsp;
+    for (int v = 0; v < nv; ++v)
+        if (!q[v] || !o[v]) {
+            snprintf(msg, sizeof msg, "%s: null plane pointer", who);
+            return set_err(GLM_ERR_ARG, msg);
+        }
+    for (int v = 0; v < nv; ++v)
+        for (int w = 0; w < nv; ++w)
+            if (q[v] == o[w]) {
+                snprintf(msg, sizeof msg, "%s: input and output planes alias", who);
+                return set_err(GLM_ERR_ARG, msg);
+            }
     HydroArgsHost h;
     h.nx = ctx->nx; h.ny = ctx->ny; h.nz = ctx->nz;
     h.gamma = p->gamma; h.dtdx = p->dt / p->dx; h.smallr = p->smallr;
@@ -521,10 +540,13 @@ int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_param
     h.fb_dmin = p->llf_dmin > 0.0 ? p->llf_dmin : 0.0;
     h.fb_pmin = p->llf_pmin > 0.0 ? p->llf_pmin : 0.0;
     h.zchunk = p->zchunk;
-    const int key = 2000 + h.fallback;
+    h.nsp = nsp;
+    h.cma = nsp > 0 && cma;
+    const int key = 2000 + h.fallback + 2 * nsp + 8 * h.cma;
     if (ctx->hydro_slots_key != key) {
         int dev = 0, cus = 0, occ = 0;
-        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback) : hydro_march_ptr<float>(h.fallback);
+        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback, nsp, h.cma)
+                                        : hydro_march_ptr<float>(h.fallback, nsp, h.cma);
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, HYDRO_THREADS, 0));
@@ -534,6 +556,16 @@ int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_param
     if (ctx->prec == 8) HIPCHK(launch_hydro_march<double>(q, o, h, ctx->hydro_slots, ctx->stream));
     else HIPCHK(launch_hydro_march<float>(q, o, h, ctx->hydro_slots, ctx->stream));
     return GLM_OK;
+}
+
+int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
+{
+    return hydro_step_impl(ctx, q, o, p, 0, 0, "glm_hydro_step");
+}
+
+int glm_hydro_step_scalars(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma)
+{
+    return hydro_step_impl(ctx, q, o, p, nsp, cma, "glm_hydro_step_scalars");
 }
 
 int glm_last_error(void) { return t_last_error; }

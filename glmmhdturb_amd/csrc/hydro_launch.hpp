@@ -10,12 +10,14 @@ struct HydroArgsHost {
     int fallback;              // HLL with the LLF fallback below fb_dmin / fb_pmin
     double fb_dmin, fb_pmin;
     int zchunk;                // z planes per workgroup (0: the occupancy model)
+    int nsp;                   // passive scalars: 0 or 2
+    int cma;                   // scalar flux rides the mass flux
 };
 
 constexpr int HYDRO_THREADS = 256;   // lanes of a hydro march workgroup (hydro_march.hpp HM_NT)
 template <typename T>
 hipError_t launch_hydro_march(void* const* q, void* const* o, const HydroArgsHost& h, int slots, hipStream_t st);
 template <typename T>
-const void* hydro_march_ptr(int fallback);
+const void* hydro_march_ptr(int fallback, int nsp, int cma);
 
 }  // namespace glm

@@ -7,7 +7,10 @@
 // div += F(+1/2) - F(-1/2) in the order x, y, z (:486-487), U1 = toCons(prim(U0)) - dtdx*div
 // (:432, :492).  Periodic in all three axes.
 //
-// Layout: 5 SoA planes [rho, mx, my, mz, E], cell (i,j,k) at i + nx*(j + ny*k).
+// Layout: 5 SoA planes [rho, mx, my, mz, E], cell (i,j,k) at i + nx*(j + ny*k); with NSP = 2
+// passive scalars (-DSCALARS, :115-119) two more planes of rho*s, advected as s (:191, :241,
+// :251, :298), their flux the HLL/LLF flux of rho*s or, with CMA (-DCMA), the HLL mass flux times
+// the upwind face value of s (:321, :367).
 //
 // Kernel shape.  A workgroup owns a 32 x 8 column tile and a chunk of z planes; lane (tx, ty)
 // owns one column.  Each plane of the tile plus a 2-cell halo (36 x 12 cells) is read from HBM
@@ -28,18 +31,20 @@ constexpr int HM_LX = HM_OX + 2 * HM_G, HM_LY = HM_OY + 2 * HM_G;
 constexpr int HM_NPL = HM_LX * HM_LY;   // 432 cells of a haloed plane
 constexpr int HM_NT = HM_OX * HM_OY;    // 256 lanes, one per owned column
 
+constexpr int HM_NVMAX = 7;   // 5 hydro variables + 2 passive scalars (-DSCALARS)
+
 template <typename T>
 struct HydroArgs {
-    const T* q[5];
-    T* o[5];
+    const T* q[HM_NVMAX];
+    T* o[HM_NVMAX];
     int nx, ny, nz;
     int tiles_x, tiles_y, nzc, zchunk;
     T gm1, igm1, gamma, dtdx, half_dtdx, smallr, smallp, fb_dmin, fb_pmin;
 };
 
-// spike_25d.cu:196-202 prim_from_cons
-template <typename T>
-GLM_DEV void hy_c2p(const T c[5], T q[5], const HydroArgs<T>& a)
+// spike_25d.cu:196-202 prim_from_cons (:183-194 with scalars: s = (rho s) / rho)
+template <typename T, int NV>
+GLM_DEV void hy_c2p(const T c[NV], T q[NV], const HydroArgs<T>& a)
 {
 #pragma clang fp contract(off)   // the same bits at every call site of the kernel
     const T r = tmax(c[0], a.smallr);
@@ -50,6 +55,8 @@ GLM_DEV void hy_c2p(const T c[5], T q[5], const HydroArgs<T>& a)
     q[3] = c[3] * ir;
     const T ke = T(0.5) * r * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     q[4] = tmax(a.gm1 * (c[4] - ke), a.smallp);
+#pragma unroll
+    for (int v = 5; v < NV; ++v) q[v] = c[v] * ir;
 }
 
 // spike_25d.cu:207-213 moncen (full slope)
@@ -62,8 +69,8 @@ GLM_DEV T hy_moncen(T l, T m, T r)
 }
 
 // spike_25d.cu:231-244 primFlux and :245-255 toCons
-template <typename T, int D>
-GLM_DEV void hy_flux_cons(const T q[5], T f[5], T u[5], const HydroArgs<T>& a)
+template <typename T, int D, int NV>
+GLM_DEV void hy_flux_cons(const T q[NV], T f[NV], T u[NV], const HydroArgs<T>& a)
 {
     const T un = q[1 + D];
     const T E = q[4] * a.igm1 + T(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
@@ -76,11 +83,16 @@ GLM_DEV void hy_flux_cons(const T q[5], T f[5], T u[5], const HydroArgs<T>& a)
 #pragma unroll
     for (int c = 0; c < 3; ++c) u[1 + c] = q[0] * q[1 + c];
     u[4] = E;
+#pragma unroll
+    for (int v = 5; v < NV; ++v) {
+        f[v] = mu * q[v];
+        u[v] = q[0] * q[v];
+    }
 }
 
 // spike_25d.cu:287-301 hanc1d: normal-only Hancock half step along D
-template <typename T, int D>
-GLM_DEV void hy_hanc1d(const T m[5], const T s[5], T mh[5], const HydroArgs<T>& a)
+template <typename T, int D, int NV>
+GLM_DEV void hy_hanc1d(const T m[NV], const T s[NV], T mh[NV], const HydroArgs<T>& a)
 {
     const T un = m[1 + D], sn = s[1 + D];
     const T ir = trcp(m[0]);
@@ -89,84 +101,90 @@ GLM_DEV void hy_hanc1d(const T m[5], const T s[5], T mh[5], const HydroArgs<T>& 
 #pragma unroll
     for (int c = 0; c < 3; ++c) mh[1 + c] = m[1 + c] - h * (un * s[1 + c] + (c == D ? s[4] * ir : T(0)));
     mh[4] = m[4] - h * (un * s[4] + a.gamma * m[4] * sn);
+    const T hu = h * un;
+#pragma unroll
+    for (int v = 5; v < NV; ++v) mh[v] = m[v] - hu * s[v];   // passive advection (:298)
 }
 
 // spike_25d.cu:222-230 edge: mh + sign*s/2, clamped at SMALLR/SMALLP
-template <typename T>
-GLM_DEV void hy_edge(const T mh[5], const T s[5], T sign, T e[5], const HydroArgs<T>& a)
+template <typename T, int NV>
+GLM_DEV void hy_edge(const T mh[NV], const T s[NV], T sign, T e[NV], const HydroArgs<T>& a)
 {
     const T hs = sign * T(0.5);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) e[v] = mh[v] + hs * s[v];
+    for (int v = 0; v < NV; ++v) e[v] = mh[v] + hs * s[v];
     e[0] = tmax(e[0], a.smallr);
     e[4] = tmax(e[4], a.smallp);
 }
 
 // HLL (spike_25d.cu:302-330); with FB, LLF where a face state is below the fallback floors
-// (riemann_fb, :335-376).  Both combine the same FL, FR, UL, UR.
-template <typename T, int D, bool FB>
-GLM_DEV void hy_riemann(const T L[5], const T R[5], T F[5], const HydroArgs<T>& a)
+// (riemann_fb, :335-376).  Both combine the same FL, FR, UL, UR.  With CMA the scalar fluxes of
+// the HLL fan are the mass flux times the upwind face value (:321, :367).
+template <typename T, int D, bool FB, int NV, bool CMA>
+GLM_DEV void hy_riemann(const T L[NV], const T R[NV], T F[NV], const HydroArgs<T>& a)
 {
     const T unL = L[1 + D], unR = R[1 + D];
     const T aL = tsqrt(a.gamma * L[4] * trcp(L[0])), aR = tsqrt(a.gamma * R[4] * trcp(R[0]));
-    T FL[5], FR[5], UL[5], UR[5];
-    hy_flux_cons<T, D>(L, FL, UL, a);
-    hy_flux_cons<T, D>(R, FR, UR, a);
+    T FL[NV], FR[NV], UL[NV], UR[NV];
+    hy_flux_cons<T, D, NV>(L, FL, UL, a);
+    hy_flux_cons<T, D, NV>(R, FR, UR, a);
     const T sL = tmin(unL - aL, unR - aR), sR = tmax(unL + aL, unR + aR);
     const T inv = trcp(sR - sL), slsr = sL * sR;
     const bool llf = FB && (L[0] < a.fb_dmin || L[4] < a.fb_pmin || R[0] < a.fb_dmin || R[4] < a.fb_pmin);
     const T sm = tmax(tabs(unL) + aL, tabs(unR) + aR);
+    const T hll0 = (sR * FL[0] - sL * FR[0] + slsr * (UR[0] - UL[0])) * inv;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        const T hll = (sR * FL[v] - sL * FR[v] + slsr * (UR[v] - UL[v])) * inv;
+    for (int v = 0; v < NV; ++v) {
+        T hll = v == 0 ? hll0 : (sR * FL[v] - sL * FR[v] + slsr * (UR[v] - UL[v])) * inv;
+        if (CMA && v >= 5) hll = hll0 >= T(0) ? hll0 * L[v] : hll0 * R[v];
         const T h = sL >= T(0) ? FL[v] : (sR <= T(0) ? FR[v] : hll);
         F[v] = llf ? T(0.5) * (FL[v] + FR[v]) - T(0.5) * sm * (UR[v] - UL[v]) : h;
     }
 }
 
-template <typename T>
-GLM_DEV void hy_slopes(const T l[5], const T m[5], const T r[5], T s[5])
+template <typename T, int NV>
+GLM_DEV void hy_slopes(const T l[NV], const T m[NV], const T r[NV], T s[NV])
 {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) s[v] = hy_moncen(l[v], m[v], r[v]);
+    for (int v = 0; v < NV; ++v) s[v] = hy_moncen(l[v], m[v], r[v]);
 }
 
 // Flux through the face between cells c1 and c2 of the stencil c0 c1 | c2 c3 along D.
-template <typename T, int D, bool FB>
-GLM_DEV void hy_face(const T c0[5], const T c1[5], const T c2[5], const T c3[5], T F[5], const HydroArgs<T>& a)
+template <typename T, int D, bool FB, int NV, bool CMA>
+GLM_DEV void hy_face(const T c0[NV], const T c1[NV], const T c2[NV], const T c3[NV], T F[NV], const HydroArgs<T>& a)
 {
-    T sl[5], sr[5], mh[5], L[5], R[5];
-    hy_slopes(c0, c1, c2, sl);
-    hy_slopes(c1, c2, c3, sr);
-    hy_hanc1d<T, D>(c1, sl, mh, a);
-    hy_edge(mh, sl, T(1), L, a);
-    hy_hanc1d<T, D>(c2, sr, mh, a);
-    hy_edge(mh, sr, T(-1), R, a);
-    hy_riemann<T, D, FB>(L, R, F, a);
+    T sl[NV], sr[NV], mh[NV], L[NV], R[NV];
+    hy_slopes<T, NV>(c0, c1, c2, sl);
+    hy_slopes<T, NV>(c1, c2, c3, sr);
+    hy_hanc1d<T, D, NV>(c1, sl, mh, a);
+    hy_edge<T, NV>(mh, sl, T(1), L, a);
+    hy_hanc1d<T, D, NV>(c2, sr, mh, a);
+    hy_edge<T, NV>(mh, sr, T(-1), R, a);
+    hy_riemann<T, D, FB, NV, CMA>(L, R, F, a);
 }
 
 // div += F(+1/2) - F(-1/2) along D for the cell c0 of the stencil m2 m1 c0 p1 p2: the centre
 // cell's slope and half step are shared by its two faces (spike_25d.cu:462-466 forms them once
 // per face).
-template <typename T, int D, bool FB>
-GLM_DEV void hy_dir(const T m2[5], const T m1[5], const T c0[5], const T p1[5], const T p2[5], T div[5],
+template <typename T, int D, bool FB, int NV, bool CMA>
+GLM_DEV void hy_dir(const T m2[NV], const T m1[NV], const T c0[NV], const T p1[NV], const T p2[NV], T div[NV],
                     const HydroArgs<T>& a)
 {
-    T s[5], mh[5], L[5], R[5], Fm[5], Fp[5];
-    hy_slopes(m2, m1, c0, s);
-    hy_hanc1d<T, D>(m1, s, mh, a);
-    hy_edge(mh, s, T(1), L, a);
-    hy_slopes(m1, c0, p1, s);
-    hy_hanc1d<T, D>(c0, s, mh, a);
-    hy_edge(mh, s, T(-1), R, a);
-    hy_riemann<T, D, FB>(L, R, Fm, a);
-    hy_edge(mh, s, T(1), L, a);
-    hy_slopes(c0, p1, p2, s);
-    hy_hanc1d<T, D>(p1, s, mh, a);
-    hy_edge(mh, s, T(-1), R, a);
-    hy_riemann<T, D, FB>(L, R, Fp, a);
+    T s[NV], mh[NV], L[NV], R[NV], Fm[NV], Fp[NV];
+    hy_slopes<T, NV>(m2, m1, c0, s);
+    hy_hanc1d<T, D, NV>(m1, s, mh, a);
+    hy_edge<T, NV>(mh, s, T(1), L, a);
+    hy_slopes<T, NV>(m1, c0, p1, s);
+    hy_hanc1d<T, D, NV>(c0, s, mh, a);
+    hy_edge<T, NV>(mh, s, T(-1), R, a);
+    hy_riemann<T, D, FB, NV, CMA>(L, R, Fm, a);
+    hy_edge<T, NV>(mh, s, T(1), L, a);
+    hy_slopes<T, NV>(c0, p1, p2, s);
+    hy_hanc1d<T, D, NV>(p1, s, mh, a);
+    hy_edge<T, NV>(mh, s, T(-1), R, a);
+    hy_riemann<T, D, FB, NV, CMA>(L, R, Fp, a);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fm[v];
+    for (int v = 0; v < NV; ++v) div[v] += Fp[v] - Fm[v];
 }
 
 // z chunking: the chunk length minimising (waves of workgroups) x (planes + priming) for
@@ -196,16 +214,17 @@ inline void hydro_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int&
     nzc = (nz + zchunk - 1) / zchunk;
 }
 
-// Resident waves per SIMD the register allocation is held to: two workgroups per CU in FP64
-// (2 x 69 KB of LDS), four in FP32.
-#define GLM_HYDRO_WAVES_PER_EU(T) (sizeof(T) == 8 ? 2 : 4)
+// Resident waves per SIMD the register allocation is held to: as many as the LDS ring allows
+// (FP64: 2 workgroups per CU of 69 KB, 1 of 97 KB with scalars; FP32: 4 of 35 KB, 3 of 48 KB).
+#define GLM_HYDRO_WAVES_PER_EU(T, NSP) (sizeof(T) == 8 ? ((NSP) ? 1 : 2) : ((NSP) ? 3 : 4))
 
-template <typename T, bool FB>
-__global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T))))
+template <typename T, bool FB, int NSP, bool CMA>
+__global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T, NSP))))
 hydro_march_kernel(HydroArgs<T> a)
 {
+    constexpr int NV = 5 + NSP;
     // ring of 4 haloed primitive planes: plane j lives in buffer (j - k0 + 2) & 3
-    __shared__ T P[4][5][HM_NPL];
+    __shared__ T P[4][NV][HM_NPL];
     const int tid = threadIdx.x;
     const int tx = tid % HM_OX, ty = tid / HM_OX;
     const int ntile = a.tiles_x * a.tiles_y;
@@ -225,7 +244,7 @@ hydro_march_kernel(HydroArgs<T> a)
     // has the same bits in every stencil and every chunking.  A plane is fetched from HBM into
     // registers (`pre`, two cells per lane) at the top of an iteration and converted into its
     // LDS buffer at the bottom, so the fetch latency is hidden behind the plane's compute.
-    T pre[2][5];
+    T pre[2][NV];
     auto fetch = [&](int k) {
         const size_t zoff = (size_t)wrapi(k, a.nz) * plane;
 #pragma unroll
@@ -235,7 +254,7 @@ hydro_march_kernel(HydroArgs<T> a)
                 const int lx = c % HM_LX, ly = c / HM_LX;
                 const size_t g = zoff + wrapi(x0 - HM_G + lx, a.nx) + (size_t)a.nx * wrapi(y0 - HM_G + ly, a.ny);
 #pragma unroll
-                for (int v = 0; v < 5; ++v) pre[r][v] = gload(a.q[v] + g);
+                for (int v = 0; v < NV; ++v) pre[r][v] = gload(a.q[v] + g);
             }
         }
     };
@@ -245,10 +264,10 @@ hydro_march_kernel(HydroArgs<T> a)
         for (int r = 0; r < 2; ++r) {
             const int c = tid + r * HM_NT;
             if (c < HM_NPL) {
-                T qq[5];
-                hy_c2p(pre[r], qq, a);
+                T qq[NV];
+                hy_c2p<T, NV>(pre[r], qq, a);
 #pragma unroll
-                for (int v = 0; v < 5; ++v) P[b][v][c] = qq[v];
+                for (int v = 0; v < NV; ++v) P[b][v][c] = qq[v];
             }
         }
     };
@@ -256,14 +275,14 @@ hydro_march_kernel(HydroArgs<T> a)
         fetch(k);
         commit(k);
     };
-    auto own_cell = [&](int k, T qq[5]) {
+    auto own_cell = [&](int k, T qq[NV]) {
         const int b = (k - k0 + 2) & 3;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) qq[v] = P[b][v][li];
+        for (int v = 0; v < NV; ++v) qq[v] = P[b][v][li];
     };
 
-    T W[4][5];   // the lane's column at planes k-1 .. k+2 (iteration k)
-    T Fz[5];     // z flux through face k-1/2
+    T W[4][NV];   // the lane's column at planes k-1 .. k+2 (iteration k)
+    T Fz[NV];     // z flux through face k-1/2
     load_plane(k0 - 2);
     load_plane(k0 - 1);
     load_plane(k0);
@@ -280,45 +299,47 @@ hydro_march_kernel(HydroArgs<T> a)
         const bool more = k + 3 <= k1 + 1;   // planes up to k1+1 are read
         if (more) fetch(k + 3);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
+        for (int v = 0; v < NV; ++v) {
             W[0][v] = W[1][v];
             W[1][v] = W[2][v];
             W[2][v] = W[3][v];
         }
         own_cell(k + 2, W[3]);
-        T div[5] = {T(0), T(0), T(0), T(0), T(0)};
+        T div[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) div[v] = T(0);
         const bool upd = k >= k0;
         if (upd) {
             const int b = (k - k0 + 2) & 3;
-            T m2[5], m1[5], c0[5], p1[5], p2[5];
+            T m2[NV], m1[NV], c0[NV], p1[NV], p2[NV];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 m2[v] = P[b][v][li - 2]; m1[v] = P[b][v][li - 1]; c0[v] = P[b][v][li];
                 p1[v] = P[b][v][li + 1]; p2[v] = P[b][v][li + 2];
             }
-            hy_dir<T, 0, FB>(m2, m1, c0, p1, p2, div, a);
+            hy_dir<T, 0, FB, NV, CMA>(m2, m1, c0, p1, p2, div, a);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 m2[v] = P[b][v][li - 2 * HM_LX]; m1[v] = P[b][v][li - HM_LX];
                 p1[v] = P[b][v][li + HM_LX]; p2[v] = P[b][v][li + 2 * HM_LX];
             }
-            hy_dir<T, 1, FB>(m2, m1, c0, p1, p2, div, a);
+            hy_dir<T, 1, FB, NV, CMA>(m2, m1, c0, p1, p2, div, a);
         }
-        T Fp[5];
-        hy_face<T, 2, FB>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
+        T Fp[NV];
+        hy_face<T, 2, FB, NV, CMA>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
         if (upd) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) div[v] += Fp[v] - Fz[v];
-            T U[5], f[5];
-            hy_flux_cons<T, 0>(W[1], f, U, a);   // toCons(prim(U0)); the flux is unused
+            for (int v = 0; v < NV; ++v) div[v] += Fp[v] - Fz[v];
+            T U[NV], f[NV];
+            hy_flux_cons<T, 0, NV>(W[1], f, U, a);   // toCons(prim(U0)); the flux is unused
             if (own) {
                 const size_t g = (size_t)k * plane + col;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) a.o[v][g] = U[v] - a.dtdx * div[v];
+                for (int v = 0; v < NV; ++v) a.o[v][g] = U[v] - a.dtdx * div[v];
             }
         }
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Fz[v] = Fp[v];
+        for (int v = 0; v < NV; ++v) Fz[v] = Fp[v];
         if (more) commit(k + 3);   // into plane k-1's buffer, last read before the previous barrier
         __syncthreads();
     }

@@ -349,12 +349,16 @@ class Solver:
         fn = self.lib.glm_ct_step_passes if passes else self.lib.glm_ct_step
         check(fn(self.ctx, qa, oa, C.byref(args), int(face_slope)))
 
-    def hydro_step(self, qold, qnew, args: GLMParams):
-        """One step of the 5-variable hydro march (spike_25d.cu march, build_march.sh scheme)
-        on (5, nz, ny, nx) conserved states [rho, mx, my, mz, E]; see hydro_args."""
-        qa, kq = self._planes(qold, 5)
-        oa, ko = self._planes(qnew, 5)
-        check(self.lib.glm_hydro_step(self.ctx, qa, oa, C.byref(args)))
+    def hydro_step(self, qold, qnew, args: GLMParams, nsp: int = 0, cma: bool = False):
+        """One step of the hydro march (spike_25d.cu march, build_march.sh scheme) on
+        (5 + nsp, nz, ny, nx) conserved states [rho, mx, my, mz, E, rho*s0, rho*s1]; nsp = 2 adds
+        the passive scalars of -DSCALARS, cma their -DCMA flux; see hydro_args."""
+        qa, kq = self._planes(qold, 5 + nsp)
+        oa, ko = self._planes(qnew, 5 + nsp)
+        if nsp == 0 and not cma:
+            check(self.lib.glm_hydro_step(self.ctx, qa, oa, C.byref(args)))
+        else:
+            check(self.lib.glm_hydro_step_scalars(self.ctx, qa, oa, C.byref(args), int(nsp), int(cma)))
 
     def ct_ctot_max(self, q) -> float:
         pa, keep = self._planes(q, 8)

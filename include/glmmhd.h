@@ -166,6 +166,11 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
  * floors, 1e-3 in spike_25d.cu:339) and zchunk from p; the other fields are ignored.
  * Asynchronous on the context stream.  Returns 0 or an error code. */
 int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
+/* The same march with nsp = 2 passive scalars (spike_25d.cu -DSCALARS, :115-119): q, o = 7 planes
+ * [rho, mx, my, mz, E, rho*s0, rho*s1].  cma != 0 makes the scalar flux the mass flux times the
+ * upwind face value of s (-DCMA, consistent multi-fluid advection, :321, :367); cma = 0 solves
+ * rho*s with the same HLL/LLF as the other variables.  nsp = 0 is glm_hydro_step. */
+int glm_hydro_step_scalars(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma);
 
 /* ===================================================================================== */
 /* 4. Adjoint (no reference counterpart: the reference computes no derivatives).             */

@@ -193,3 +193,38 @@ def test_hydro_sod_large_grid_exact_and_symmetric():
     t = steps * dtdx * dx
     ex = exact_riemann(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, (x[sel] - 1.5) / t)
     assert np.abs(res[0][0][sel] - ex[:, 0]).mean() < 0.0035
+
+
+@pytest.mark.parametrize("cma,fb,prec", [(False, None, "f64"), (True, None, "f64"), (True, (1e-3, 1e-3), "f64"),
+                                         (True, None, "f32"), (False, (1e-3, 1e-3), "f32")])
+def test_hydro_scalars_vs_oracle(orc, orc32, cma, fb, prec):
+    """Two passive scalars (glm_hydro_step_scalars; spike_25d.cu -DSCALARS, with -DCMA) against
+    the oracle, on partial tiles and, with the fallback, near-vacuum pockets."""
+    nx, ny, nz, dtdx = 33, 17, 9, 0.05
+    u = voids(nx, ny, nz, 40) if fb else smooth_hydro(nx, ny, nz, 41)
+    rng = np.random.default_rng(42)
+    x0 = 0.5 + 0.3 * np.tanh(rng.normal(0, 1, u[0].shape))
+    us = np.concatenate([u, (u[0] * x0)[None], (u[0] * (1 - x0))[None]])
+    o_ = orc if prec == "f64" else orc32
+    cast = (lambda v: v) if prec == "f64" else (lambda v: float(np.float32(v)))
+    if prec == "f32":
+        us = us.astype(np.float32)
+    sc = HydroScheme(gamma=cast(G), dtdx=cast(dtdx), fallback=fb is not None, fb_dmin=(fb or (0, 0))[0],
+                     fb_pmin=(fb or (0, 0))[1], nsp=2, cma=cma)
+    s = solver(nx, ny, nz, prec)
+    a, b = s.new_state(7), s.new_state(7)
+    a.copy_(torch.from_numpy(np.ascontiguousarray(us)).to(a.device, a.dtype))
+    args = hydro_args(cast(dtdx), cast(G), fallback=fb)
+    for _ in range(4):
+        s.hydro_step(a, b, args, nsp=2, cma=cma)
+        a, b = b, a
+    torch.cuda.synchronize()
+    got = a.cpu().numpy().astype(np.float64)
+    ref = run_oracle(o_, us, sc, 4).astype(np.float64)
+    err = np.array([np.abs(got[v] - ref[v]).sum() / np.abs(ref[v]).sum() for v in range(7)])
+    print(f"[hydro scalars {prec} cma={cma} fb={fb}] rel-L1 {err.max():.2e}", flush=True)
+    assert np.all(err <= (1e-12 if prec == "f64" else 2e-6)), err
+    # the hydro planes are those of the 5-variable march (to the rounding of its own kernel)
+    h = run_gpu(s, us[:5], args, 4)
+    assert np.all(np.abs(h - got[:5]) <= (1e-13 if prec == "f64" else 1e-5) * (np.abs(h) + 1))
+    s.close()

@@ -228,3 +228,47 @@ def test_hydro_scalars_vs_oracle(orc, orc32, cma, fb, prec):
     h = run_gpu(s, us[:5], args, 4)
     assert np.all(np.abs(h - got[:5]) <= (1e-13 if prec == "f64" else 1e-5) * (np.abs(h) + 1))
     s.close()
+
+
+def test_hydro_largest_grid_fp32():
+    """1300^3 in FP32 (two 5-plane states: 88 GB of HBM; each plane holds 2.2e9 cells, more than a
+    32-bit index reaches).  A uniform state stays uniform; on a state that varies in x and z only,
+    every y row of the result is the same bit for bit, mass changes by less than half an FP32 ulp
+    per cell, and the result is finite."""
+    N = 1300
+    if torch.cuda.get_device_properties(0).total_memory < 200 * 2 ** 30:
+        pytest.skip("needs a GPU with >= 200 GiB")
+    torch.cuda.empty_cache()
+    s = solver(N, N, N, "f32")
+    u, un = s.new_state(5), s.new_state(5)
+    assert u[0].numel() > 2 ** 31
+    for v, c in enumerate(cons_of(1.2, 0.3, -0.2, 0.1, 0.8)):
+        u[v].fill_(float(c))
+    args = hydro_args(0.1, G)
+    s.hydro_step(u, un, args)
+    assert torch.equal(un[0], u[0])
+    for v in range(1, 5):
+        assert bool((un[v] == un[v].view(-1)[0]).all()), v
+    x = (torch.arange(N, device=u.device, dtype=torch.float64) + 0.5) / N
+    X, Z = x[None, :], x[:, None]                       # (z, x) planes
+    tp = 2 * np.pi
+    rho = 1 + 0.2 * torch.sin(tp * X) + 0.1 * torch.cos(tp * Z)
+    vx, vz = 0.3 * torch.sin(tp * Z) + 0 * X, 0.2 * torch.sin(tp * X) + 0 * Z
+    p = 1 + 0.1 * torch.cos(tp * X) + 0 * Z
+    cons = [rho, rho * vx, 0 * rho, rho * vz, p / (G - 1) + 0.5 * rho * (vx ** 2 + vz ** 2)]
+    for v in range(5):
+        u[v].copy_(cons[v][:, None, :].to(torch.float32).expand(N, N, N))
+    del rho, vx, vz, p, cons
+    m0 = float(u[0].sum(dtype=torch.float64))
+    s.hydro_step(u, un, args)
+    torch.cuda.synchronize()
+    m1 = float(un[0].sum(dtype=torch.float64))
+    assert abs(m1 - m0) <= 3e-8 * m0
+    for v in range(5):
+        assert bool(torch.isfinite(un[v]).all())
+        for j in (1, 777, N - 1):
+            assert torch.equal(un[v][:, j, :], un[v][:, 0, :]), (v, j)
+    assert not bool(un[2].any())                        # no y momentum appears
+    s.close()
+    del u, un
+    torch.cuda.empty_cache()

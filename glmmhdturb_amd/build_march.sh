@@ -4,11 +4,20 @@
 # march_set_dtdx, march_set_gamma, march_set_grid, march_run_dev and march_run_dev_f64.  N is
 # the default grid (march_set_grid changes it at run time).
 #
-#   ./build_march.sh [N] [outdir]    -> <outdir>/libmarch<N>.so
+#   ./build_march.sh [N] [outdir]              -> <outdir>/libmarch<N>.so
+#   SCALARS=1 ./build_march.sh [N] [outdir]    -> libmarch<N>_scalars.so (march_nv = 7, -DSCALARS)
+#   SCALARS=1 CMA=1 ./build_march.sh ...       -> libmarch<N>_cma.so     (-DSCALARS -DCMA)
 set -euo pipefail
 N="${1:-480}"
 HERE="$(cd "$(dirname "$0")" && pwd)"
 OUT="${2:-$HERE}"
-make -s -C "$HERE" -j"${JOBS:-8}" legacy N="$N" NV=5 ARCH="${ARCH:-gfx950}"
-[ "$OUT" = "$HERE" ] || cp "$HERE/libmarch${N}.so" "$OUT/"
-echo "ok: $OUT/libmarch${N}.so"
+if [ -n "${SCALARS:-}" ]; then
+    C="${CMA:-0}"
+    LIB="libmarch${N}_$([ "$C" = 1 ] && echo cma || echo scalars).so"
+    make -s -C "$HERE" -j"${JOBS:-8}" legacy N="$N" NV=7 CMA="$C" ARCH="${ARCH:-gfx950}"
+else
+    LIB="libmarch${N}.so"
+    make -s -C "$HERE" -j"${JOBS:-8}" legacy N="$N" NV=5 ARCH="${ARCH:-gfx950}"
+fi
+[ "$OUT" = "$HERE" ] || cp "$HERE/$LIB" "$OUT/"
+echo "ok: $OUT/$LIB"

@@ -21,6 +21,9 @@
 #ifndef GLM_LIB_NV
 #define GLM_LIB_NV 9
 #endif
+#ifndef GLM_LIB_CMA
+#define GLM_LIB_CMA 0
+#endif
 
 using namespace glm;
 
@@ -601,13 +604,13 @@ int glm_kernel_info(int precision, int predictor, int riemann, int recon, int sl
 // ------------------------------------------------------------------------------------------
 static std::mutex g_mu;
 static int g_nx = GLM_DEFAULT_N, g_ny = GLM_DEFAULT_N, g_nz = GLM_DEFAULT_N;
-static float g_dtdx = GLM_LIB_NV == 5 ? 0.05f : 0.02f, g_ch = 1.0f, g_glmfac = 1.0f, g_gamma = 1.4f;   // spike_mhd.cu:52, spike_25d.cu:180
+static float g_dtdx = GLM_LIB_NV == 5 || GLM_LIB_NV == 7 ? 0.05f : 0.02f, g_ch = 1.0f, g_glmfac = 1.0f, g_gamma = 1.4f;   // spike_mhd.cu:52, spike_25d.cu:180
 static glm_ctx* g_ctx32 = nullptr;
 static glm_ctx* g_ctx64 = nullptr;
 
 int march_nv(void) { return GLM_LIB_NV; }
 int march_nx(void) { return g_nx; }
-int march_nsp(void) { return 0; }
+int march_nsp(void) { return GLM_LIB_NV == 7 ? 2 : 0; }
 void march_set_dtdx(float v) { g_dtdx = v; }
 void march_set_glm(float ch, float fac) { g_ch = ch; g_glmfac = fac; }
 void march_set_gamma(float g) { g_gamma = g; }
@@ -665,6 +668,7 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kin
     for (int it = 0; it < nsteps && !err; ++it) {
         err = kind == 1   ? glm_ct_step(c, q.data(), o.data(), &p, 0)
               : kind == 2 ? glm_hydro_step(c, q.data(), o.data(), &p)
+              : kind == 3 ? glm_hydro_step_scalars(c, q.data(), o.data(), &p, 2, GLM_LIB_CMA)
                           : glm_step(c, q.data(), o.data(), &p);
         std::swap(q, o);
     }
@@ -685,10 +689,11 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kin
 
 extern "C" {
 
-// A 5-variable library (make legacy NV=5, build_march.sh) runs the hydro march of spike_25d.cu;
-// the others run the GLM-MHD march of spike_mhd.cu.
-constexpr int LEGACY_NV = GLM_LIB_NV == 5 ? 5 : 9;
-constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : 0;
+// A 5-variable library (make legacy NV=5, build_march.sh) runs the hydro march of spike_25d.cu, a
+// 7-variable one its -DSCALARS build (with -DCMA when built with CMA=1); the others run the GLM-MHD
+// march of spike_mhd.cu.
+constexpr int LEGACY_NV = GLM_LIB_NV == 5 || GLM_LIB_NV == 7 ? GLM_LIB_NV : 9;
+constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : GLM_LIB_NV == 7 ? 3 : 0;
 double march_run_dev(float* const* qv, float* const* ov, int nsteps)
 {
     return legacy_run<float>(qv, ov, nsteps, LEGACY_NV, LEGACY_KIND);

@@ -29,12 +29,12 @@ extern "C" {
 /*    GLM cleaning and psi damping, smallr = smallp = 1e-6 (spike_mhd.cu:52, :257-288).    */
 /* ===================================================================================== */
 
-/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library), 8 (CT library, spike_ctm.cu:301) or 5
- * (hydro library, spike_25d.cu:560) */
+/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library), 8 (CT library, spike_ctm.cu:301), 5
+ * (hydro library, spike_25d.cu:560) or 7 (its -DSCALARS build, spike_25d.cu:115-119) */
 int march_nv(void);
 /* spike_mhd.cu:322  grid size N (compile-time NX in the reference; runtime here, see march_set_grid) */
 int march_nx(void);
-/* spike_mhd.cu:323  passive species count (0) */
+/* spike_mhd.cu:323  passive species count (2 in the 7-plane hydro library, else 0) */
 int march_nsp(void);
 /* spike_mhd.cu:324  fixed dt/dx of subsequent march_run_dev / ct_run calls */
 void march_set_dtdx(float v);

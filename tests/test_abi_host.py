@@ -204,3 +204,11 @@ def test_hydro_library_exports_the_bridge_symbols():
         assert sym in exported, sym
     lib = C.CDLL(path)
     assert lib.march_nv() == 5 and lib.march_nx() == 480
+
+
+def test_species_library_exports_the_bridge_symbols():
+    """libmarch480_cma.so (SCALARS=1 CMA=1 build_march.sh): march_nv = 7, march_nsp = 2."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", "libmarch480_cma.so")
+    assert os.path.exists(path), "build it with SCALARS=1 CMA=1 glmmhdturb_amd/build_march.sh"
+    lib = C.CDLL(path)
+    assert lib.march_nv() == 7 and lib.march_nsp() == 2 and lib.march_nx() == 480

@@ -272,3 +272,26 @@ def test_hydro_largest_grid_fp32():
     s.close()
     del u, un
     torch.cuda.empty_cache()
+
+
+def test_legacy_species_library_run_dev(orc32):
+    """libmarch480_cma.so (the -DSCALARS -DCMA build): 7 float planes through the MarchBridge
+    mirror, 3 steps, against the oracle with two CMA scalars."""
+    from glmmhdturb_amd import march_bridge as mb
+    m = mb.open_lib(os.path.join(ROOT, "glmmhdturb_amd", "libmarch480_cma.so"))
+    assert m.NV == 7 and m.f_glm is None
+    n = 32
+    mb.set_grid(m, n, n, n)
+    g, dtdx = float(np.float32(G)), float(np.float32(0.1))
+    mb.set_gamma(m, g)
+    mb.set_dtdx(m, dtdx)
+    u = smooth_hydro(n, n, n, 51)
+    x0 = 0.5 + 0.3 * np.sin(2 * np.pi * np.arange(n) / n)[None, None, :] * np.ones_like(u[0])
+    us = np.concatenate([u, (u[0] * x0)[None], (u[0] * (1 - x0))[None]]).astype(np.float32)
+    q = [torch.from_numpy(us[v].copy()).cuda() for v in range(7)]
+    o = [torch.empty_like(x) for x in q]
+    assert mb.run_(m, q, o, 3) >= 0.0
+    ref = run_oracle(orc32, us, HydroScheme(gamma=g, dtdx=dtdx, nsp=2, cma=True), 3).astype(np.float64)
+    got = np.stack([x.cpu().numpy() for x in q]).astype(np.float64)
+    err = np.array([np.abs(got[v] - ref[v]).sum() / np.abs(ref[v]).sum() for v in range(7)])
+    assert np.all(err <= 2e-6), err

@@ -298,8 +298,31 @@ def cpu_baseline(seconds, extra_seconds=5.0):
     main["other_legs"] = {
         "fp32_same_scheme": _cpu_leg(extra_seconds, "f32", "ppm", "hlld"),
         "fp64_cpu_simd_scheme": _cpu_leg(extra_seconds, "f64", "plm", "llf"),
+        "hydro_march_fp32": _cpu_hydro_leg(extra_seconds),
     }
     return main
+
+
+def _cpu_hydro_leg(seconds, M=128):
+    """The hydro march's oracle (oracle/hydro_oracle.c, OpenMP) on M^3 of the hydro leg's input
+    (IC ph = 0.001 (i mod 911), dt/dx 0.05, FP32), as many steps as fit in about `seconds`."""
+    import numpy as np
+    from oracle import HydroScheme, Oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    orc = Oracle("f32")
+    ph = (0.001 * (np.arange(M ** 3) % 911)).astype(np.float32).reshape(M, M, M)
+    u = np.stack([1 + 0.3 * ph, 0.1 * ph, -0.1 * ph, 0.05 * ph, 2.5 + 0.5 * ph]).astype(np.float32)
+    sc = HydroScheme(gamma=1.4, dtdx=0.05)
+    steps, busy = 0, 0.0
+    while busy < seconds or steps < 1:
+        t0 = time.perf_counter()
+        u = orc.hydro_step(u, sc)
+        busy += time.perf_counter() - t0
+        steps += 1
+    return dict(value=M ** 3 * steps / busy, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"hydro march oracle (FP32, OpenMP, {threads} threads), {M}^3 of the hydro leg's input, "
+                       f"{steps} steps, {busy:.1f} s")
 
 
 def _cpu_leg(seconds, prec, recon, riemann):

@@ -135,6 +135,13 @@ def test_hydro_errors():
     ob, ko = _lib.plane_array([b[v].data_ptr() for v in range(5)])
     with pytest.raises(_lib.GLMError, match="gamma"):
         _lib.check(s.lib.glm_hydro_step(s.ctx, qa, ob, C.byref(bad)))
+    with pytest.raises(_lib.GLMError, match="nsp must be 0 or 2"):
+        _lib.check(s.lib.glm_hydro_step_scalars(s.ctx, qa, ob, C.byref(hydro_args(0.1)), 1, 0))
+    nulls = (C.c_void_p * 7)(*([a[v].data_ptr() for v in range(5)] + [None, None]))
+    outs = (C.c_void_p * 7)(*([b[v].data_ptr() for v in range(5)] + [b[0].data_ptr(), b[1].data_ptr()]))
+    with pytest.raises(_lib.GLMError, match="null plane"):
+        _lib.check(s.lib.glm_hydro_step_scalars(s.ctx, C.cast(nulls, C.POINTER(C.c_void_p)),
+                                                C.cast(outs, C.POINTER(C.c_void_p)), C.byref(hydro_args(0.1)), 2, 1))
     s.close()
 
 

@@ -59,6 +59,54 @@ typedef struct {
     int pad_;
 } o_scheme;
 
+/* Branch counters: how often each guarded branch of the scheme fired since the last reset.  Tests
+ * read them to show that a run really exercised the branch they compare.
+ * 0 HLLD degenerate star (|denK| < 1e-12), 1 predictor revert mh -> m0, 2 iflo face revert,
+ * 3 update void floor, 4 HLLD -> LLF switch. */
+#define O_NHITS 5
+static long long o_hits[O_NHITS];
+#define O_HIT(w) do { _Pragma("omp atomic") o_hits[w]++; } while (0)
+long long o_branch_hits(int which) { return (which >= 0 && which < O_NHITS) ? o_hits[which] : -1; }
+void o_branch_hits_reset(void) { for (int w = 0; w < O_NHITS; ++w) o_hits[w] = 0; }
+
+/* Branch trace (built only with -DO_BRANCH_TRACE, liboracle_trace.so): for every decision point of
+ * the step (limiter case, PPM case, wave-speed min/max choice, HLLD fan, floors, reverts) the count
+ * of each outcome.  Two runs whose traces are equal took the same branches everywhere (up to
+ * cancelling swaps), so a finite difference between them sees a smooth map.  Sites:
+ *  0 c2p rho floor   1 c2p p floor   2 fast-speed radicand clamp   3 MonCen (0 zero, 1 2|dl|, 2 2|dr|, 3 |dc|)
+ *  4 minmod (0 zero, 1 a, 2 b)   5 PPM CW84 (0 flat, 1 lo reset, 2 hi reset, 3 none)   6 PPM spj
+ *  7 spike PPM (as 5)   8 LLF speed (0 L, 1 R, 2 ch)   9 LLF-bound speed (as 8)
+ * 10 HLL SL (0 L, 1 R, 2 zero)  11 HLL SR (as 10)  12 HLLD SL clamped at 0  13 HLLD SR clamped
+ * 14 HLLD min(uL,uR) side  15 HLLD max(cfL,cfR) side  16 HLLD rho*_L floor  17 HLLD rho*_R floor
+ * 18 HLLD degenerate star   19 HLLD fan (0..3)   20 HLLD sign(Bn)   21 interface floors (0 rhoL,
+ * 1 pL, 2 rhoR, 3 pR)   22 HLLD->LLF switch   23 predictor revert   24 iflo face revert
+ * 25 clamped face floors   26 update floor   27 kick density threshold   28 kick energy floor
+ * (removal: component)   29 kick energy floor (addition: component)   30 spike edge floors */
+#define O_NSITES 32
+#define O_NOUT 4
+#ifdef O_BRANCH_TRACE
+static long long o_trace[O_NSITES][O_NOUT];
+#define O_BR(site, k) do { _Pragma("omp atomic") o_trace[site][k]++; } while (0)
+#else
+#define O_BR(site, k) ((void)0)
+#endif
+void o_trace_get(long long* out)
+{
+    for (int a = 0; a < O_NSITES * O_NOUT; ++a) {
+#ifdef O_BRANCH_TRACE
+        out[a] = o_trace[a / O_NOUT][a % O_NOUT];
+#else
+        out[a] = -1;
+#endif
+    }
+}
+void o_trace_reset(void)
+{
+#ifdef O_BRANCH_TRACE
+    memset(o_trace, 0, sizeof o_trace);
+#endif
+}
+
 /* ------------------------------------------------------------------------- */
 /* Point physics                                                               */
 /* ------------------------------------------------------------------------- */
@@ -67,12 +115,14 @@ typedef struct {
 void o_cons2prim(const real* c, real* q, real gamma, real smallr, real pfl)
 {
     real rho = fmax(c[0], smallr);
+    O_BR(0, c[0] < smallr);
     real ir = R(1) / rho;
     real vx = c[1] * ir, vy = c[2] * ir, vz = c[3] * ir;
     real bx = c[5], by = c[6], bz = c[7];
     real ekin = R(0.5) * (c[1] * c[1] + c[2] * c[2] + c[3] * c[3]) * ir;
     real emag = R(0.5) * (bx * bx + by * by + bz * bz);
     real p = fmax((gamma - R(1)) * (c[4] - ekin - emag), pfl);
+    O_BR(1, (gamma - R(1)) * (c[4] - ekin - emag) < pfl);
     q[0] = rho; q[1] = vx; q[2] = vy; q[3] = vz; q[4] = p;
     q[5] = bx; q[6] = by; q[7] = bz; q[8] = c[8];
 }
@@ -96,6 +146,7 @@ real o_fast_speed(const real* q, real gamma, real bn)
     real c2 = gamma * q[4] / rho;
     real b2 = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) / rho;
     real d2 = R(0.5) * (b2 + c2);
+    O_BR(2, d2 * d2 - c2 * bn * bn / rho < R(0));
     return sqrt(d2 + sqrt(fmax(d2 * d2 - c2 * bn * bn / rho, R(0))));
 }
 
@@ -161,12 +212,14 @@ real o_moncen(real dl, real dr)
     real dc = R(0.5) * (dl + dr);
     real sgn = dc >= R(0) ? R(1) : R(-1);
     real val = sgn * fmin(R(2) * fmin(fabs(dl), fabs(dr)), fabs(dc));
+    O_BR(3, (dl * dr <= R(0)) ? 0 : (R(2) * fmin(fabs(dl), fabs(dr)) < fabs(dc) ? (fabs(dl) <= fabs(dr) ? 1 : 2) : 3));
     return (dl * dr <= R(0)) ? R(0) : val;
 }
 
 /* glmmhd_turb.jl:300-301 minmod */
 real o_minmod(real a, real b)
 {
+    O_BR(4, (a * b <= R(0)) ? 0 : (fabs(a) < fabs(b) ? 1 : 2));
     return (a * b <= R(0)) ? R(0) : (fabs(a) < fabs(b) ? a : b);
 }
 
@@ -218,10 +271,10 @@ void o_ppm_mono(real ql, real q0, real qr, real* lo, real* hi)
 {
     real dq = qr - ql;
     real diff = (q0 - R(0.5) * (ql + qr)) * dq;
-    if ((qr - q0) * (q0 - ql) <= R(0)) { *lo = q0; *hi = q0; }
-    else if (diff > dq * dq / R(6)) { *lo = R(3) * q0 - R(2) * qr; *hi = qr; }
-    else if (diff < -(dq * dq / R(6))) { *lo = ql; *hi = R(3) * q0 - R(2) * ql; }
-    else { *lo = ql; *hi = qr; }
+    if ((qr - q0) * (q0 - ql) <= R(0)) { O_BR(5, 0); *lo = q0; *hi = q0; }
+    else if (diff > dq * dq / R(6)) { O_BR(5, 1); *lo = R(3) * q0 - R(2) * qr; *hi = qr; }
+    else if (diff < -(dq * dq / R(6))) { O_BR(5, 2); *lo = ql; *hi = R(3) * q0 - R(2) * ql; }
+    else { O_BR(5, 3); *lo = ql; *hi = qr; }
 }
 
 /* glmmhd_turb.jl:923 ppm_edges */
@@ -237,6 +290,7 @@ int o_spj(real a, real b, real c)
 {
     real hi = fmax(a, fmax(b, c));
     real lo = fmax(fmin(a, fmin(b, c)), R(1e-20));
+    O_BR(6, hi / lo > R(2));
     return hi / lo > R(2);
 }
 
@@ -245,9 +299,10 @@ real o_ppm1_spike(real qm, real q0, real qp, real sgn)
 {
     real d = R(0.25) * (qp - qm), cu = (qm - R(2) * q0 + qp) * (R(1) / R(6));
     real qr = q0 + d - cu, ql = q0 - d - cu;
-    if ((qr - q0) * (q0 - ql) <= R(0)) { qr = q0; ql = q0; }
+    if ((qr - q0) * (q0 - ql) <= R(0)) { O_BR(7, 0); qr = q0; ql = q0; }
     else {
         real dqe = qr - ql, d6 = R(6) * (q0 - R(0.5) * (ql + qr));
+        O_BR(7, dqe * d6 > dqe * dqe ? 1 : (dqe * d6 < -dqe * dqe ? 2 : 3));
         if (dqe * d6 > dqe * dqe) ql = R(3) * q0 - R(2) * qr;
         if (dqe * d6 < -dqe * dqe) qr = R(3) * q0 - R(2) * ql;
     }
@@ -267,8 +322,9 @@ static void llf_x(const real* L, const real* Rq, real gamma, real ch, real fbn, 
     real fL[NV], fR[NV], uL[NV], uR[NV];
     o_phys_flux_x(L, fL, gamma); o_phys_flux_x(Rq, fR, gamma);
     o_prim2cons(L, uL, gamma); o_prim2cons(Rq, uR, gamma);
-    real smax = fmax(fmax(fabs(L[1]) + o_fast_speed(L, gamma, L[5]),
-                          fabs(Rq[1]) + o_fast_speed(Rq, gamma, Rq[5])), ch);
+    real sa = fabs(L[1]) + o_fast_speed(L, gamma, L[5]), sb = fabs(Rq[1]) + o_fast_speed(Rq, gamma, Rq[5]);
+    real smax = fmax(fmax(sa, sb), ch);
+    O_BR(8, fmax(sa, sb) < ch ? 2 : (sa >= sb ? 0 : 1));
     for (int v = 0; v < NV; ++v) f[v] = R(0.5) * (fL[v] + fR[v]) - R(0.5) * smax * (uR[v] - uL[v]);
     f[5] = fbn; f[8] = fpsi;
 }
@@ -284,8 +340,9 @@ static void llf_bound(const real* L, const real* Rq, real gamma, real ch, real f
     real fL[NV], fR[NV], uL[NV], uR[NV];
     o_phys_flux_x(L, fL, gamma); o_phys_flux_x(Rq, fR, gamma);
     o_prim2cons(L, uL, gamma); o_prim2cons(Rq, uR, gamma);
-    real smax = fmax(fmax(fabs(L[1]) + fast_speed_bound(L, gamma),
-                          fabs(Rq[1]) + fast_speed_bound(Rq, gamma)), ch);
+    real sa = fabs(L[1]) + fast_speed_bound(L, gamma), sb = fabs(Rq[1]) + fast_speed_bound(Rq, gamma);
+    real smax = fmax(fmax(sa, sb), ch);
+    O_BR(9, fmax(sa, sb) < ch ? 2 : (sa >= sb ? 0 : 1));
     for (int v = 0; v < NV; ++v) f[v] = R(0.5) * (fL[v] + fR[v]) - R(0.5) * smax * (uR[v] - uL[v]);
     f[5] = fbn; f[8] = fpsi;
 }
@@ -296,6 +353,8 @@ static void hll_x(const real* L, const real* Rq, real gamma, real bn, real fbn, 
     real cfL = o_fast_speed(L, gamma, bn), cfR = o_fast_speed(Rq, gamma, bn);
     real SL = fmin(fmin(L[1] - cfL, Rq[1] - cfR), R(0));
     real SR = fmax(fmax(L[1] + cfL, Rq[1] + cfR), R(0));
+    O_BR(10, fmin(L[1] - cfL, Rq[1] - cfR) > R(0) ? 2 : (L[1] - cfL <= Rq[1] - cfR ? 0 : 1));
+    O_BR(11, fmax(L[1] + cfL, Rq[1] + cfR) < R(0) ? 2 : (L[1] + cfL >= Rq[1] + cfR ? 0 : 1));
     real fL[NV], fR[NV], uL[NV], uR[NV];
     o_phys_flux_x(L, fL, gamma); o_phys_flux_x(Rq, fR, gamma);
     o_prim2cons(L, uL, gamma); o_prim2cons(Rq, uR, gamma);
@@ -309,7 +368,8 @@ static void hlld_star(real d, real u, real v, real w, real by, real bz, real S, 
                       real* vy, real* vz, real* byS, real* bzS)
 {
     real denK = d * (S - u) * (S - SM) - bn * bn;
-    if (fabs(denK) < R(1e-12)) { *vy = v; *vz = w; *byS = by; *bzS = bz; }
+    O_BR(18, fabs(denK) < R(1e-12));
+    if (fabs(denK) < R(1e-12)) { O_HIT(0); *vy = v; *vz = w; *byS = by; *bzS = bz; }
     else {
         real fac = bn * (SM - u) / denK;
         *vy = v - by * fac;
@@ -334,6 +394,10 @@ void o_hlld_x(const real* L, const real* Rq, real gamma, real bn, real fbn, real
     real cfL = o_fast_speed(L, gamma, bn), cfR = o_fast_speed(Rq, gamma, bn);
     real SL = fmin(fmin(uL, uR) - fmax(cfL, cfR), R(0));
     real SR = fmax(fmax(uL, uR) + fmax(cfL, cfR), R(0));
+    O_BR(12, fmin(uL, uR) - fmax(cfL, cfR) > R(0));
+    O_BR(13, fmax(uL, uR) + fmax(cfL, cfR) < R(0));
+    O_BR(14, uL <= uR);
+    O_BR(15, cfL >= cfR);
     real UL[NV] = {dL, dL * uL, dL * vL, dL * wL, EL, bn, byL, bzL, R(0)};
     real UR[NV] = {dR, dR * uR, dR * vR, dR * wR, ER, bn, byR, bzR, R(0)};
     real FL[NV], FR[NV];
@@ -344,6 +408,8 @@ void o_hlld_x(const real* L, const real* Rq, real gamma, real bn, real fbn, real
     real pts = ((SR - uR) * dR * ptL - (SL - uL) * dL * ptR + dL * dR * (SR - uR) * (SL - uL) * (uR - uL)) / den;
     real dLs = fmax(dL * (SL - uL) / (SL - SM), R(1e-12));
     real dRs = fmax(dR * (SR - uR) / (SR - SM), R(1e-12));
+    O_BR(16, dL * (SL - uL) / (SL - SM) < R(1e-12));
+    O_BR(17, dR * (SR - uR) / (SR - SM) < R(1e-12));
     real sqdLs = sqrt(dLs), sqdRs = sqrt(dRs);
     real SLs = SM - fabs(bn) / sqdLs;
     real SRs = SM + fabs(bn) / sqdRs;
@@ -359,6 +425,8 @@ void o_hlld_x(const real* L, const real* Rq, real gamma, real bn, real fbn, real
     real ULs[NV] = {dLs, dLs * SM, dLs * vyLs, dLs * vzLs, ELs, bn, byLs, bzLs, R(0)};
     real URs[NV] = {dRs, dRs * SM, dRs * vyRs, dRs * vzRs, ERs, bn, byRs, bzRs, R(0)};
     real sgn = bn >= R(0) ? R(1) : R(-1);
+    O_BR(20, bn >= R(0));
+    O_BR(19, SLs >= R(0) ? 0 : (SM >= R(0) ? 1 : (SRs >= R(0) ? 2 : 3)));
     real invsum = R(1) / (sqdLs + sqdRs);
     real vyss = (sqdLs * vyLs + sqdRs * vyRs + (byRs - byLs) * sgn) * invsum;
     real vzss = (sqdLs * vzLs + sqdRs * vzRs + (bzRs - bzLs) * sgn) * invsum;
@@ -390,6 +458,10 @@ void o_riemann(const real* Lq, const real* Rq, int dir, int kind, real gamma, re
     real L[NV], Rr[NV], f[NV];
     o_rot_to(Lq, L, dir);
     o_rot_to(Rq, Rr, dir);
+    if (L[0] < smallr) O_BR(21, 0);
+    if (L[4] < pfl) O_BR(21, 1);
+    if (Rr[0] < smallr) O_BR(21, 2);
+    if (Rr[4] < pfl) O_BR(21, 3);
     L[0] = fmax(L[0], smallr); L[4] = fmax(L[4], pfl);
     Rr[0] = fmax(Rr[0], smallr); Rr[4] = fmax(Rr[4], pfl);
     real bns, psis;
@@ -403,6 +475,8 @@ void o_riemann(const real* Lq, const real* Rq, int dir, int kind, real gamma, re
     } else {
         int uself = (llf_dmin > R(0) && fmin(L[0], Rr[0]) < llf_dmin) ||
                     (llf_pmin > R(0) && fmin(L[4], Rr[4]) < llf_pmin) || kind == 0;
+        if (uself && kind == 3) O_HIT(4);
+        if (kind == 3) O_BR(22, uself);
         if (uself) llf_x(L, Rr, gamma, ch, fbn, fpsi, f);
         else o_hlld_x(L, Rr, gamma, bns, fbn, fpsi, f);
     }
@@ -476,12 +550,14 @@ static void recon_cell(const real* prim, size_t n, int i, int j, int k, int nx, 
         real uh[NV], q[NV];
         o_hancock(m0, s[0], s[1], s[2], dtdx, gamma, uh);
         o_cons2prim(uh, q, gamma, smallr, pfl);
-        if (sc->mh_revert && (q[0] <= smallr || q[4] <= pfl)) memcpy(q, m0, sizeof q);
+        if (sc->mh_revert) O_BR(23, q[0] <= smallr || q[4] <= pfl);
+        if (sc->mh_revert && (q[0] <= smallr || q[4] <= pfl)) { O_HIT(1); memcpy(q, m0, sizeof q); }
         for (int d = 0; d < 3; ++d) memcpy(mh[d], q, sizeof q);
     } else {
         for (int d = 0; d < 3; ++d) {
             o_hanc1d(m0, s[d], d, dtdx, gamma, smallr, pfl, mh[d]);
-            if (sc->mh_revert && (mh[d][0] <= smallr || mh[d][4] <= pfl)) memcpy(mh[d], m0, sizeof m0);
+            if (sc->mh_revert) O_BR(23, mh[d][0] <= smallr || mh[d][4] <= pfl);
+            if (sc->mh_revert && (mh[d][0] <= smallr || mh[d][4] <= pfl)) { O_HIT(1); memcpy(mh[d], m0, sizeof m0); }
         }
     }
     int up = 1;
@@ -506,20 +582,36 @@ static void recon_cell(const real* prim, size_t n, int i, int j, int k, int nx, 
                 el[v] = o_ppm1_spike(nb[2 * d][v], m0[v], nb[2 * d + 1][v], R(-1));
                 eh[v] = o_ppm1_spike(nb[2 * d][v], m0[v], nb[2 * d + 1][v], R(1));
             }
+            if (el[0] < smallr || el[4] < pfl || eh[0] < smallr || eh[4] < pfl) O_BR(30, 0);
             el[0] = fmax(el[0], smallr); el[4] = fmax(el[4], pfl);
             eh[0] = fmax(eh[0], smallr); eh[4] = fmax(eh[4], pfl);
             for (int v = 0; v < NV; ++v) { lo[v] = el[v] + mh[d][v] - m0[v]; hi[v] = eh[v] + mh[d][v] - m0[v]; }
+            if (lo[0] < smallr || lo[4] < pfl || hi[0] < smallr || hi[4] < pfl) O_BR(30, 1);
             lo[0] = fmax(lo[0], smallr); lo[4] = fmax(lo[4], pfl);
             hi[0] = fmax(hi[0], smallr); hi[4] = fmax(hi[4], pfl);
         }
         if (sc->face_guard == 1) {
-            if (lo[0] < smallr || lo[4] < pfl) memcpy(lo, m0, sizeof m0);
-            if (hi[0] < smallr || hi[4] < pfl) memcpy(hi, m0, sizeof m0);
+            O_BR(24, lo[0] < smallr || lo[4] < pfl);
+            O_BR(24, hi[0] < smallr || hi[4] < pfl);
+            if (lo[0] < smallr || lo[4] < pfl) { O_HIT(2); memcpy(lo, m0, sizeof m0); }
+            if (hi[0] < smallr || hi[4] < pfl) { O_HIT(2); memcpy(hi, m0, sizeof m0); }
         } else if (sc->face_guard == 2) {
+            if (lo[0] < smallr) O_BR(25, 0);
+            if (lo[4] < pfl) O_BR(25, 1);
+            if (hi[0] < smallr) O_BR(25, 2);
+            if (hi[4] < pfl) O_BR(25, 3);
             lo[0] = fmax(lo[0], smallr); lo[4] = fmax(lo[4], pfl);
             hi[0] = fmax(hi[0], smallr); hi[4] = fmax(hi[4], pfl);
         }
     }
+}
+
+/* The face states of cell (i,j,k) from a primitive state (9 planes): fm/fp[d*9+v] on its -d/+d
+ * face.  Exposes recon_cell to the tests. */
+void o_recon_cell_faces(const real* prim, int nx, int ny, int nz, int i, int j, int k, const o_scheme* sc,
+                        real* fm, real* fp)
+{
+    recon_cell(prim, (size_t)nx * ny * nz, i, j, k, nx, ny, nz, sc, fm, fp);
 }
 
 /* Fused update epilogue of one cell: glmmhd_turb.jl:440-476 (update_kernel!).
@@ -535,26 +627,32 @@ void o_update_cell(real* r, const real* fxl, const real* fxh, const real* fyl, c
         a = a + (fzl[v] - fzh[v]);
         r[v] = r[v] + dtdx * a;
     }
+    if (sc->update_floor) O_BR(26, r[0] < smallr);
     if (sc->update_floor && r[0] < smallr) {
+        O_HIT(3);
         real emag = R(0.5) * (r[5] * r[5] + r[6] * r[6] + r[7] * r[7]);
         r[0] = smallr; r[1] = R(0); r[2] = R(0); r[3] = R(0);
         r[4] = pfl / (gamma - R(1)) + emag;
     }
     r[8] = r[8] * R(sc->glm_fac);
+    if (sc->do_turb && afield) O_BR(27, r[0] >= R(sc->turb_min_rho));
     if (sc->do_turb && afield && r[0] >= R(sc->turb_min_rho)) {
         real dx = R(sc->dx), acc[3];
         o_turb_interp(afield, sc->turb_gs, i, j, k, dx, R(nx) * dx, R(ny) * dx, R(nz) * dx, R(sc->ramp), acc);
         real rho = r[0], rhom = fmax(rho, smallr), sc2 = smallc * smallc;
         real e = r[4], dt = R(sc->dt);
-        e = fmax(e - R(0.5) * r[1] * r[1] / rhom, rho * sc2);
-        e = fmax(e - R(0.5) * r[2] * r[2] / rhom, rho * sc2);
-        e = fmax(e - R(0.5) * r[3] * r[3] / rhom, rho * sc2);
+        for (int a = 1; a <= 3; ++a) {
+            if (e - R(0.5) * r[a] * r[a] / rhom < rho * sc2) O_BR(28, a);
+            e = fmax(e - R(0.5) * r[a] * r[a] / rhom, rho * sc2);
+        }
         real m2 = r[1] + rhom * acc[0] * dt;
         real m3 = r[2] + rhom * acc[1] * dt;
         real m4 = r[3] + rhom * acc[2] * dt;
-        e = fmax(e + R(0.5) * m2 * m2 / rhom, rho * sc2);
-        e = fmax(e + R(0.5) * m3 * m3 / rhom, rho * sc2);
-        e = fmax(e + R(0.5) * m4 * m4 / rhom, rho * sc2);
+        const real mm[3] = {m2, m3, m4};
+        for (int a = 0; a < 3; ++a) {
+            if (e + R(0.5) * mm[a] * mm[a] / rhom < rho * sc2) O_BR(29, a + 1);
+            e = fmax(e + R(0.5) * mm[a] * mm[a] / rhom, rho * sc2);
+        }
         r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
     }
 }
@@ -647,6 +745,24 @@ double o_ctot_max(const real* u, int nx, int ny, int nz, double gamma_, double s
         real ct = (fabs(q[1]) + o_fast_speed(q, gamma, q[5])) + (fabs(q[2]) + o_fast_speed(q, gamma, q[6])) +
                   (fabs(q[3]) + o_fast_speed(q, gamma, q[7]));
         if ((double)ct > m) m = (double)ct;
+    }
+    return m;
+}
+
+/* Largest fast magnetosonic speed of the grid, max over cells and axes of c_f(B_d) (the
+ * "ch = max fast speed" Dedner cleaning speed; glmmhd_turb.jl:89-95 per axis). */
+double o_fast_max(const real* u, int nx, int ny, int nz, double gamma_, double smallr_, double pfl_)
+{
+    size_t n = (size_t)nx * ny * nz;
+    real gamma = R(gamma_), smallr = R(smallr_), pfl = R(pfl_);
+    double m = 0.0;
+#pragma omp parallel for reduction(max : m) schedule(static)
+    for (long long c = 0; c < (long long)n; ++c) {
+        real cc[NV], q[NV];
+        load9(u, n, (size_t)c, cc);
+        o_cons2prim(cc, q, gamma, smallr, pfl);
+        real f = fmax(o_fast_speed(q, gamma, q[5]), fmax(o_fast_speed(q, gamma, q[6]), o_fast_speed(q, gamma, q[7])));
+        if ((double)f > m) m = (double)f;
     }
     return m;
 }

@@ -139,12 +139,13 @@ class Oracle:
     """One precision of the oracle: dtype float64 ('f64') or float32 ('f32')."""
 
     def __init__(self, prec: str = "f64"):
+        """prec: 'f64', 'f32', or 'trace' (FP64 with the branch trace of o_trace_get)."""
         path = os.path.join(BUILD, f"liboracle_{prec}.so")
         if not os.path.exists(path):
             build()
         self.lib = C.CDLL(path)
-        self.dtype = np.float64 if prec == "f64" else np.float32
-        self.real = C.c_double if prec == "f64" else C.c_float
+        self.dtype = np.float32 if prec == "f32" else np.float64
+        self.real = C.c_float if prec == "f32" else C.c_double
         L, rl, vp, i = self.lib, self.real, C.c_void_p, C.c_int
         sig = {
             "o_cons2prim": (None, [vp, vp, rl, rl, rl]),
@@ -166,9 +167,15 @@ class Oracle:
             "o_riemann": (None, [vp, vp, i, i, rl, rl, rl, rl, rl, rl, vp]),
             "o_turb_interp": (None, [vp, i, i, i, i, rl, rl, rl, rl, rl, vp]),
             "o_update_cell": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, i, i]),
+            "o_recon_cell_faces": (None, [vp, i, i, i, i, i, i, vp, vp, vp]),
+            "o_branch_hits": (C.c_longlong, [i]),
+            "o_branch_hits_reset": (None, []),
+            "o_trace_get": (None, [vp]),
+            "o_trace_reset": (None, []),
             "o_step": (i, [vp, vp, i, i, i, vp, vp, vp]),
             "o_step_scratch_reals": (C.c_size_t, [i, i, i]),
             "o_ctot_max": (C.c_double, [vp, i, i, i, C.c_double, C.c_double, C.c_double]),
+            "o_fast_max": (C.c_double, [vp, i, i, i, C.c_double, C.c_double, C.c_double]),
             "o_diagnostics": (None, [vp, i, i, i, C.c_double, C.c_double, vp]),
             "o_ic_ot": (None, [vp, i, i, i, C.c_double]),
             "o_ic_uniform": (None, [vp, i, i, i] + [C.c_double] * 6),
@@ -289,6 +296,44 @@ class Oracle:
         self.lib.o_riemann(_ptr(L), _ptr(R_), d, k, gamma, ch, smallr, pfl, llf_dmin, llf_pmin, _ptr(f))
         return f
 
+    def update_cell(self, r, fxl, fxh, fyl, fyh, fzl, fzh, sc: "Scheme", afield=None, ijk=(0, 0, 0), nxyz=(1, 1, 1)):
+        """o_update_cell: the fused update epilogue of one cell (returns the new state)."""
+        r = self.arr(r, NV).copy()
+        fl = [self.arr(f, NV) for f in (fxl, fxh, fyl, fyh, fzl, fzh)]
+        af = None if afield is None else self.arr(afield)
+        c = sc.to_c()
+        self.lib.o_update_cell(_ptr(r), *[_ptr(f) for f in fl], C.byref(c), None if af is None else _ptr(af),
+                               *ijk, *nxyz)
+        return r
+
+    def recon_faces(self, prim, i, j, k, sc: "Scheme"):
+        """o_recon_cell_faces: (fm, fp), each (3, 9), the face states of cell (i, j, k)."""
+        prim = self.arr(prim)
+        nv, nz, ny, nx = prim.shape
+        fm = self.vec(3 * NV); fp = self.vec(3 * NV)
+        c = sc.to_c()
+        self.lib.o_recon_cell_faces(_ptr(prim), nx, ny, nz, i, j, k, C.byref(c), _ptr(fm), _ptr(fp))
+        return fm.reshape(3, NV), fp.reshape(3, NV)
+
+    BRANCHES = ("hlld_degenerate", "mh_revert", "iflo_revert", "update_floor", "llf_switch")
+
+    def branch_hits(self, reset=False):
+        """How often each guarded branch fired since the last reset (name -> count)."""
+        h = {b: int(self.lib.o_branch_hits(w)) for w, b in enumerate(self.BRANCHES)}
+        if reset:
+            self.lib.o_branch_hits_reset()
+        return h
+
+    TRACE_SITES = 32
+
+    def branch_trace(self, reset=False):
+        """(32, 4) counts of each outcome of each decision site (liboracle_trace.so only)."""
+        out = np.zeros(self.TRACE_SITES * 4, dtype=np.int64)
+        self.lib.o_trace_get(_ptr(out))
+        if reset:
+            self.lib.o_trace_reset()
+        return out.reshape(self.TRACE_SITES, 4)
+
     def turb_interp(self, afield, gs, i, j, k, dx, lx, ly, lz, ramp):
         a = self.arr(afield, 3 * gs ** 3); out = self.vec(3)
         self.lib.o_turb_interp(_ptr(a), gs, i, j, k, dx, lx, ly, lz, ramp, _ptr(out))
@@ -311,6 +356,10 @@ class Oracle:
     def ctot_max(self, u, gamma, smallr, pfl):
         nv, nz, ny, nx = u.shape
         return self.lib.o_ctot_max(_ptr(self.arr(u)), nx, ny, nz, gamma, smallr, pfl)
+
+    def fast_max(self, u, gamma, smallr, pfl):
+        nv, nz, ny, nx = u.shape
+        return self.lib.o_fast_max(_ptr(self.arr(u)), nx, ny, nz, gamma, smallr, pfl)
 
     def diagnostics(self, u, dx, smallr=1e-6):
         nv, nz, ny, nx = u.shape

@@ -453,7 +453,7 @@ def test_twiddle_table():
 # ----------------------------------------------------------------------------------------------
 def test_ctot_and_diagnostics_brute_force(orc):
     u = smooth(orc, 6, 5, 4, seed=7)
-    best, sums = 0.0, dict(mass=[], ekin=[], emag=[], hc=[])
+    best, fbest, sums = 0.0, 0.0, dict(mass=[], ekin=[], emag=[], hc=[])
     for idx in np.ndindex(4, 5, 6):
         c = u[(slice(None),) + idx]
         q = orc.cons2prim(c, G, SMALLR, PFL)
@@ -463,10 +463,12 @@ def test_ctot_and_diagnostics_brute_force(orc):
             a2 = G * q[4] / q[0]
             cf = math.sqrt(0.5 * (a2 + b2 / q[0]) + math.sqrt(max((0.5 * (a2 + b2 / q[0])) ** 2 - a2 * q[5 + d] ** 2 / q[0], 0)))
             tot += abs(q[1 + d]) + cf
+            fbest = max(fbest, cf)
         best = max(best, tot)
         sums["mass"].append(c[0]); sums["ekin"].append(0.5 * (c[1:4] @ c[1:4]) / c[0])
         sums["emag"].append(0.5 * (c[5:8] @ c[5:8])); sums["hc"].append((c[1:4] @ c[5:8]) / c[0])
     assert orc.ctot_max(u, G, SMALLR, PFL) == pytest.approx(best, rel=1e-14)
+    assert orc.fast_max(u, G, SMALLR, PFL) == pytest.approx(fbest, rel=1e-14)
     d = orc.diagnostics(u, 0.25, SMALLR)
     assert d["mass"] == pytest.approx(math.fsum(sums["mass"]), rel=1e-15)
     assert d["ekin"] == pytest.approx(math.fsum(sums["ekin"]), rel=1e-14)

@@ -26,7 +26,7 @@ EXPORTS = (
     "march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_glm", "march_set_gamma", "march_run_dev",
     "march_set_grid", "march_run_dev_f64", "ct_run", "ct_regs", "ct_shmem",
     "glm_params_default", "glm_create", "glm_destroy", "glm_set_stream", "glm_step", "glm_cfl_max",
-    "glm_ctot_max", "glm_diagnostics", "glm_ic", "glm_ou_setup", "glm_ou_advance", "glm_ou_afield",
+    "glm_ctot_max", "glm_fast_max", "glm_diagnostics", "glm_ic", "glm_ou_setup", "glm_ou_advance", "glm_ou_afield",
     "glm_ou_steps", "glm_ou_copy_afield", "glm_ct_step", "glm_ct_step_passes", "glm_ct_ctot_max", "glm_ct_divb_max",
     "glm_hydro_step", "glm_hydro_step_scalars", "glm_last_error", "glm_error_string",
     "glm_kernel_info",
@@ -86,6 +86,7 @@ def load(path: str | None = None):
         "glm_create": (vp, [i, i, i, i]), "glm_destroy": (None, [vp]), "glm_set_stream": (i, [vp, vp]),
         "glm_step": (i, [vp, pp, pp, C.POINTER(GLMParams)]), "glm_cfl_max": (i, [vp, C.POINTER(d)]),
         "glm_ctot_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
+        "glm_fast_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_diagnostics": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_ic": (i, [vp, pp, i, C.POINTER(d)]),
         "glm_ou_setup": (i, [vp, i, d, d, d, i, d, d, C.c_ulonglong]),

@@ -17,9 +17,11 @@ hipError_t launch_clear_u64(unsigned long long* p, int n, hipStream_t st)
 }
 
 template <typename T>
-hipError_t launch_ctot(Ptr9<const T> q9dev, size_t n, const Phys<T>& s, unsigned long long* out, hipStream_t st)
+hipError_t launch_ctot(Ptr9<const T> q9dev, size_t n, const Phys<T>& s, unsigned long long* out, hipStream_t st,
+                       bool fast)
 {
-    hipLaunchKernelGGL((ctot_kernel<T>), dim3(grid_for(n, AUX_NT, 4096)), dim3(AUX_NT), 0, st, q9dev, n, s, out);
+    if (fast) hipLaunchKernelGGL((ctot_kernel<T, true>), dim3(grid_for(n, AUX_NT, 4096)), dim3(AUX_NT), 0, st, q9dev, n, s, out);
+    else hipLaunchKernelGGL((ctot_kernel<T, false>), dim3(grid_for(n, AUX_NT, 4096)), dim3(AUX_NT), 0, st, q9dev, n, s, out);
     return hipGetLastError();
 }
 
@@ -53,7 +55,7 @@ hipError_t launch_ic(Ptr9<T> q9dev, int nx, int ny, int nz, int kind, const doub
 }
 
 #define GLM_INST(T)                                                                                                   \
-    template hipError_t launch_ctot<T>(Ptr9<const T>, size_t, const Phys<T>&, unsigned long long*, hipStream_t);     \
+    template hipError_t launch_ctot<T>(Ptr9<const T>, size_t, const Phys<T>&, unsigned long long*, hipStream_t, bool); \
     template hipError_t launch_diag<T>(Ptr9<const T>, int, int, int, double, double, void*, double*, hipStream_t);  \
     template hipError_t launch_ic<T>(Ptr9<T>, int, int, int, int, const double*, hipStream_t);
 GLM_INST(double)

@@ -13,10 +13,11 @@ GLM_DEV int awrap(int i, int n)
 }
 
 // ---------------------------------------------------------------------------------------------
-// CFL: max over cells of sum_d(|v_d| + c_f(B_d))  (glmmhd_turb.jl:1963-1978).  One atomic per
-// wavefront on the bit pattern of a non-negative double (NaN compares largest).
+// CFL: max over cells of sum_d(|v_d| + c_f(B_d))  (glmmhd_turb.jl:1963-1978), or with FAST the max
+// over cells of max_d c_f(B_d) (the "max fast speed" cleaning speed).  One atomic per wavefront on
+// the bit pattern of a non-negative double (NaN compares largest).
 // ---------------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool FAST>
 __global__ void __launch_bounds__(AUX_NT) ctot_kernel(Ptr9<const T> q9, size_t n, Phys<T> s, unsigned long long* out)
 {
     double cmax = 0.0;
@@ -25,7 +26,7 @@ __global__ void __launch_bounds__(AUX_NT) ctot_kernel(Ptr9<const T> q9, size_t n
 #pragma unroll
         for (int v = 0; v < 9; ++v) u[v] = q9[v][c];
         cons2prim(u, p, s);
-        double ct = cfl_speed(p, s);
+        double ct = FAST ? fast_max_speed(p, s) : cfl_speed(p, s);
         if (!(ct <= cmax)) cmax = ct;
     }
     unsigned long long bits = (unsigned long long)__double_as_longlong(cmax);

@@ -49,6 +49,16 @@ int hip_err(hipError_t e, const char* where)
 }
 }  // namespace glm
 
+// True when any plane of a is one of the planes of b.  A step that writes o while other
+// workgroups still read q (halo planes, the update base) races if the two share a plane.
+static bool planes_alias(void* const* a, int na, void* const* b, int nb)
+{
+    for (int v = 0; v < na; ++v)
+        for (int w = 0; w < nb; ++w)
+            if (a[v] == b[w]) return true;
+    return false;
+}
+
 // ------------------------------------------------------------------------------------------
 // context
 // ------------------------------------------------------------------------------------------
@@ -167,10 +177,10 @@ static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_param
     memset(&a, 0, sizeof a);
     for (int v = 0; v < 9; ++v) {
         if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_step: null plane pointer");
-        if (q[v] == o[v]) return set_err(GLM_ERR_ARG, "glm_step: input and output planes must differ");
         a.q[v] = (const T*)q[v];
         a.o[v] = (T*)o[v];
     }
+    if (planes_alias(q, 9, o, 9)) return set_err(GLM_ERR_ARG, "glm_step: input and output planes must differ");
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
     const int key = 1 + (((p->predictor * 4 + p->riemann) * 4 + p->recon) * 4 + p->slope);
     if (c->slots_key != key) {
@@ -255,20 +265,40 @@ static Ptr9<const T> cptr9(void* const* q)
 
 extern "C" {
 
-int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
+}  // extern "C"
+
+static int ctot_impl(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax, bool fast, const char* who)
 {
-    if (!ctx || !cmax) return set_err(GLM_ERR_ARG, "glm_ctot_max: null argument");
-    int e = check_planes(q, 9, "glm_ctot_max: null plane pointer");
+    char msg[128];
+    if (!ctx || !cmax) {
+        snprintf(msg, sizeof msg, "%s: null argument", who);
+        return set_err(GLM_ERR_ARG, msg);
+    }
+    snprintf(msg, sizeof msg, "%s: null plane pointer", who);
+    int e = check_planes(q, 9, msg);
     if (!e) e = check_params(p);
     if (e) return e;
     HIPCHK(launch_clear_u64(ctx->d_cmax + 3, 1, ctx->stream));
-    if (ctx->prec == 8) HIPCHK(launch_ctot<double>(cptr9<double>(q), ctx->n, make_phys<double>(p), ctx->d_cmax + 3, ctx->stream));
-    else HIPCHK(launch_ctot<float>(cptr9<float>(q), ctx->n, make_phys<float>(p), ctx->d_cmax + 3, ctx->stream));
+    if (ctx->prec == 8)
+        HIPCHK(launch_ctot<double>(cptr9<double>(q), ctx->n, make_phys<double>(p), ctx->d_cmax + 3, ctx->stream, fast));
+    else HIPCHK(launch_ctot<float>(cptr9<float>(q), ctx->n, make_phys<float>(p), ctx->d_cmax + 3, ctx->stream, fast));
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, ctx->d_cmax + 3, sizeof bits, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     memcpy(cmax, &bits, sizeof bits);
     return GLM_OK;
+}
+
+extern "C" {
+
+int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
+{
+    return ctot_impl(ctx, q, p, cmax, false, "glm_ctot_max");
+}
+
+int glm_fast_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)
+{
+    return ctot_impl(ctx, q, p, cmax, true, "glm_fast_max");
 }
 
 int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* out)
@@ -355,8 +385,10 @@ static int step_adjoint_impl(glm_ctx* ctx, void* const* u0, void* const* lam1, v
     if (p->do_turb && (!afield || ctx->g < 2))
         return set_err(GLM_ERR_STATE, "glm_step_adjoint: do_turb needs the step's forcing field (and glm_ou_setup)");
     for (int v = 0; v < 9; ++v)
-        if (!u0[v] || !lam1[v] || !lam0[v] || lam0[v] == lam1[v] || lam0[v] == u0[v])
-            return set_err(GLM_ERR_ARG, "glm_step_adjoint: bad plane pointers (lam0 must not alias lam1 or u0)");
+        if (!u0[v] || !lam1[v] || !lam0[v])
+            return set_err(GLM_ERR_ARG, "glm_step_adjoint: bad plane pointers (null)");
+    if (planes_alias(lam0, 9, lam1, 9) || planes_alias(lam0, 9, u0, 9))
+        return set_err(GLM_ERR_ARG, "glm_step_adjoint: bad plane pointers (lam0 must not alias lam1 or u0)");
     if (!ctx->adj_scratch && hipMalloc(&ctx->adj_scratch, adj_scratch_bytes(ctx->n)) != hipSuccess) {
         ctx->adj_scratch = nullptr;
         return set_err(GLM_ERR_ALLOC, "glm_step_adjoint: scratch allocation failed");
@@ -425,7 +457,8 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
     if (p->riemann == GLM_RIEMANN_LLF_BOUND || p->recon == GLM_RECON_PPM)
         return set_err(GLM_ERR_ARG, "glm_ct_step: riemann must be LLF/HLL/HLLD and recon PLM/PPM_SPIKE");
     for (int v = 0; v < 8; ++v)
-        if (!q[v] || !o[v] || q[v] == o[v]) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers");
+        if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers");
+    if (planes_alias(q, 8, o, 8)) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers (input and output alias)");
     if (p->do_turb && !ctx->afield) return set_err(GLM_ERR_STATE, "glm_ct_step: do_turb without glm_ou_setup");
     if (!fused && !ctx->ct_scratch) {
         if (hipMalloc(&ctx->ct_scratch, ct_scratch_bytes(ctx->n, ctx->prec)) != hipSuccess)
@@ -529,12 +562,10 @@ static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const g
             snprintf(msg, sizeof msg, "%s: null plane pointer", who);
             return set_err(GLM_ERR_ARG, msg);
         }
-    for (int v = 0; v < nv; ++v)
-        for (int w = 0; w < nv; ++w)
-            if (q[v] == o[w]) {
-                snprintf(msg, sizeof msg, "%s: input and output planes alias", who);
-                return set_err(GLM_ERR_ARG, msg);
-            }
+    if (planes_alias(q, nv, o, nv)) {
+        snprintf(msg, sizeof msg, "%s: input and output planes alias", who);
+        return set_err(GLM_ERR_ARG, msg);
+    }
     HydroArgsHost h;
     h.nx = ctx->nx; h.ny = ctx->ny; h.nz = ctx->nz;
     h.gamma = p->gamma; h.dtdx = p->dt / p->dx; h.smallr = p->smallr;
@@ -612,7 +643,11 @@ int march_nv(void) { return GLM_LIB_NV; }
 int march_nx(void) { return g_nx; }
 int march_nsp(void) { return GLM_LIB_NV == 7 ? 2 : 0; }
 void march_set_dtdx(float v) { g_dtdx = v; }
+#if GLM_LIB_NV == 9
+// Only the GLM-MHD library has a cleaning speed: MarchBridge.set_glm decides by this symbol's
+// presence (MarchBridge.jl:37-38, :50-53).
 void march_set_glm(float ch, float fac) { g_ch = ch; g_glmfac = fac; }
+#endif
 void march_set_gamma(float g) { g_gamma = g; }
 
 int march_set_grid(int nx, int ny, int nz)
@@ -689,19 +724,24 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kin
 
 extern "C" {
 
-// A 5-variable library (make legacy NV=5, build_march.sh) runs the hydro march of spike_25d.cu, a
-// 7-variable one its -DSCALARS build (with -DCMA when built with CMA=1); the others run the GLM-MHD
-// march of spike_mhd.cu.
-constexpr int LEGACY_NV = GLM_LIB_NV == 5 || GLM_LIB_NV == 7 ? GLM_LIB_NV : 9;
+// Which legacy entry points a library exports follows the reference library of its plane count:
+//   NV 9 (libglmmhd.so, build_mhd.sh): the GLM march (spike_mhd.cu:319-341) and, in libglmmhd.so,
+//        also the CT entry points;
+//   NV 8 (build_ct.sh): the CT march only (spike_ctm.cu:299-329) -- no march_run_dev, which would
+//        read a ninth (psi) plane the caller's 8-entry arrays do not hold;
+//   NV 5 / 7 (build_march.sh [-DSCALARS]): the hydro march (spike_25d.cu:559-576), no CT entry points.
+#if GLM_LIB_NV == 9 || GLM_LIB_NV == 5 || GLM_LIB_NV == 7
 constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : GLM_LIB_NV == 7 ? 3 : 0;
 double march_run_dev(float* const* qv, float* const* ov, int nsteps)
 {
-    return legacy_run<float>(qv, ov, nsteps, LEGACY_NV, LEGACY_KIND);
+    return legacy_run<float>(qv, ov, nsteps, GLM_LIB_NV, LEGACY_KIND);
 }
 double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps)
 {
-    return legacy_run<double>(qv, ov, nsteps, LEGACY_NV, LEGACY_KIND);
+    return legacy_run<double>(qv, ov, nsteps, GLM_LIB_NV, LEGACY_KIND);
 }
+#endif
+#if GLM_LIB_NV == 9 || GLM_LIB_NV == 8
 double ct_run(float* const* qv, float* const* ov, float* const* cc, int nsteps)
 {
     (void)cc;
@@ -719,5 +759,6 @@ int ct_shmem(void)
     if (hipFuncGetAttributes(&fa, ct_kernel_ptr<float>()) != hipSuccess) return -1;
     return (int)fa.sharedSizeBytes;
 }
+#endif
 
 }  // extern "C"

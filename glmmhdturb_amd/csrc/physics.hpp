@@ -177,6 +177,22 @@ GLM_DEV double cfl_speed(const T q[9], const Phys<T>& s)
     return (double)((tabs(q[1]) + fx) + (tabs(q[2]) + fy) + (tabs(q[3]) + fz));
 }
 
+// Largest fast magnetosonic speed of a cell over the three grid directions, max_d c_f(B_d): the
+// Dedner cleaning speed "ch = max fast speed" is the max of this over the grid.
+template <typename T>
+GLM_DEV double fast_max_speed(const T q[9], const Phys<T>& s)
+{
+    T ir = trcp(q[0]);
+    T c2 = s.gamma * q[4] * ir;
+    T bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
+    T d2 = T(0.5) * (bb + c2);
+    T dd = d2 * d2, k = c2 * ir;
+    T fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], T(0))));
+    T fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], T(0))));
+    T fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], T(0))));
+    return (double)tmax(fx, tmax(fy, fz));
+}
+
 // Ideal-MHD flux of a primitive state in direction D (glmmhd_turb.jl:98-115 evaluated in the
 // rotated frame and rotated back, :161-162).  F[Bn] = F[psi] = 0.
 template <typename T, int D>

@@ -38,7 +38,7 @@ def open_lib(path: str) -> Lib:
     f_run = h.march_run_dev
     f_run.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
     f_run.restype = C.c_double
-    f_glm = getattr(h, "march_set_glm", None) if NV == 9 else None
+    f_glm = getattr(h, "march_set_glm", None)   # only the GLM-MHD library exports it
     if f_glm is not None:
         f_glm.argtypes = [C.c_float, C.c_float]
         f_glm.restype = None

@@ -48,6 +48,9 @@ class Params:
     smallc: float = 1e-4
     glm_ch_scale: float = 0.25
     glm_cp_coef: float = 0.18
+    # Dedner cleaning speed: "ref" = courant*dx/dt/3*glm_ch_scale (glmmhd_turb.jl:884), "maxfast" =
+    # the largest fast magnetosonic speed on the grid (max over cells and axes of c_f(B_d))
+    glm_ch_mode: str = "ref"
     switch_llf_dmin: float = 0.0
     switch_llf_pmin: float = 0.0
     rho0: float = 1.0
@@ -85,10 +88,12 @@ def pfloor(p: Params) -> float:
     return p.smallr * p.smallc ** 2 / p.gamma
 
 
-def glm_speed(p: Params, dt: float) -> tuple[float, float]:
-    """Dedner ch and psi decay factor of a step (glmmhd_turb.jl:884-885)."""
+def glm_speed(p: Params, dt: float, ch: float | None = None) -> tuple[float, float]:
+    """Dedner ch and psi decay factor of a step (glmmhd_turb.jl:884-885).  A given ch (e.g. the
+    max fast speed of the state, glm_ch_mode "maxfast") replaces the reference's dt rule."""
     dx = dxof(p)
-    ch = p.courant * dx / dt / NDIM * p.glm_ch_scale
+    if ch is None:
+        ch = p.courant * dx / dt / NDIM * p.glm_ch_scale
     fac = math.exp(-(ch * ch / (p.glm_cp_coef * p.boxlen * ch)) * dt) if p.glm_cp_coef > 0 else 1.0
     return ch, fac
 
@@ -110,8 +115,7 @@ def step_args(p: Params, dt: float, t: float, *, do_turb=False, riemann="hll", r
         sc["riemann"], sc["recon"], sc["slope"] = riemann, recon, slope
     if predictor is not None:
         sc["predictor"] = predictor
-    ch_, fac_ = glm_speed(p, dt)
-    ch = ch_ if ch is None else ch
+    ch, fac_ = glm_speed(p, dt, ch)
     glm_fac = fac_ if glm_fac is None else glm_fac
     a = GLMParams()
     a.gamma, a.smallr, a.smallc, a.pfl = p.gamma, p.smallr, p.smallc, pfloor(p)
@@ -252,12 +256,18 @@ class Solver:
         return {"zchunk": best, "ms": times[best], "auto_ms": times[0]}
 
     def step_plm(self, uold, unew, p: Params, dt: float, t: float, *, do_turb=False, riemann="hll",
-                 recon="plm", predictor=None, slope="moncen", scheme="cube", want_cfl=True, zchunk=0):
-        """step_plm! (glmmhd_turb.jl:878-908): one step uold -> unew."""
+                 recon="plm", predictor=None, slope="moncen", scheme="cube", want_cfl=True, zchunk=0,
+                 ch: float | None = None):
+        """step_plm! (glmmhd_turb.jl:878-908): one step uold -> unew.  With glm_ch_mode "maxfast"
+        the cleaning speed is the max fast speed of uold unless ch is given."""
         if do_turb and self.forcing is None:
             raise _lib.GLMError("do_turb requires setup_forcing()")
+        if ch is None and p.glm_ch_mode == "maxfast":
+            ch = self.fast_max(uold)
+        elif p.glm_ch_mode not in ("ref", "maxfast"):
+            raise ValueError(f"glm_ch_mode must be 'ref' or 'maxfast', not {p.glm_ch_mode!r}")
         a = step_args(p, dt, t, do_turb=do_turb, riemann=riemann, recon=recon, predictor=predictor,
-                      slope=slope, scheme=scheme, want_cfl=want_cfl, zchunk=zchunk)
+                      slope=slope, scheme=scheme, want_cfl=want_cfl, zchunk=zchunk, ch=ch)
         self.step(uold, unew, a)
 
     def cfl_max(self) -> float:
@@ -271,6 +281,14 @@ class Solver:
         out = C.c_double()
         a = step_args(self.p, 1.0, 0.0)
         check(self.lib.glm_ctot_max(self.ctx, pa, C.byref(a), C.byref(out)))
+        return out.value
+
+    def fast_max(self, u) -> float:
+        """Largest fast magnetosonic speed on the grid, max over cells and axes of c_f(B_d)."""
+        pa, keep = self._planes(u)
+        out = C.c_double()
+        a = step_args(self.p, 1.0, 0.0)
+        check(self.lib.glm_fast_max(self.ctx, pa, C.byref(a), C.byref(out)))
         return out.value
 
     def compute_dt(self, u, p: Params | None = None) -> float:
@@ -313,10 +331,20 @@ class Solver:
         torch.cuda.synchronize(self.device)
         return out
 
+    def _check_afield(self, field, who: str):
+        """A forcing field must be a contiguous (3, g, g, g) tensor of the state dtype on this
+        context's device: the library copies or reads 3 g^3 values from its address."""
+        g = self.p.turb_gs
+        if (getattr(field, "dtype", None) != self.dtype or tuple(field.shape) != (3, g, g, g)
+                or not field.is_contiguous() or field.device != self.device):
+            raise _lib.GLMError(f"{who}: the forcing field must be a contiguous {self.dtype} tensor of shape "
+                                f"{(3, g, g, g)} on {self.device}")
+
     def set_afield(self, field):
         """Replace the context's forcing field by `field` ((3, g, g, g) tensor, state dtype)."""
         if self.forcing is None:
             raise _lib.GLMError("set_afield requires setup_forcing()")
+        self._check_afield(field, "set_afield")
         check(self.lib.glm_ou_set_afield(self.ctx, C.c_void_p(field.data_ptr())))
 
     # ---- adjoint ----------------------------------------------------------------------------
@@ -327,6 +355,8 @@ class Solver:
         qa, k0 = self._planes(u0)
         la, k1 = self._planes(lam1)
         oa, k2 = self._planes(lam0)
+        if afield is not None:
+            self._check_afield(afield, "step_adjoint")
         af = None if afield is None else C.c_void_p(afield.data_ptr())
         fn = self.lib.glm_step_adjoint_passes if passes else self.lib.glm_step_adjoint
         check(fn(self.ctx, qa, la, oa, C.byref(args), af))
@@ -396,9 +426,11 @@ def ou_advance(f: Forcing) -> None:
 
 
 def run_ot(N: int = 128, t_end: float = 0.5, *, nz: int | None = None, riemann="hlld", recon="plm",
-           predictor="full", slope="moncen", scheme="multi", courant=0.7, precision="f64", verbose=True):
-    """glmmhd_turb.jl:2041-2068: Orszag-Tang to t_end with a CFL dt each step, no forcing."""
-    p = Params(N=N, gamma=5.0 / 3.0, cs0=1.0, courant=courant, nz=nz)
+           predictor="full", slope="moncen", scheme="multi", courant=0.7, precision="f64", verbose=True,
+           ch_mode: str = "ref"):
+    """glmmhd_turb.jl:2041-2068: Orszag-Tang to t_end with a CFL dt each step, no forcing.
+    ch_mode "maxfast" takes the cleaning speed from the max fast speed of each step's state."""
+    p = Params(N=N, gamma=5.0 / 3.0, cs0=1.0, courant=courant, nz=nz, glm_ch_mode=ch_mode)
     s = Solver(p, precision)
     u1, u2 = s.new_state(), s.new_state()
     s.ic_ot(u1, p.gamma)

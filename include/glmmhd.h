@@ -118,6 +118,10 @@ int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
 int glm_cfl_max(glm_ctx* ctx, double* cmax);
 /* Standalone CFL max of a state (ctot_kernel! + maximum, :1963-1976); synchronous. */
 int glm_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
+/* Max over cells of max_d c_f(B_d), the largest fast magnetosonic speed in a grid direction: the
+ * "ch = max fast speed" Dedner cleaning speed (the reference's run_ot / glm_ch_scale rule at
+ * glmmhd_turb.jl:884 is the other choice).  Synchronous. */
+int glm_fast_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
 
 #define GLM_NDIAG 10
 /* Diagnostics (:1996-2018 + energy budget), FP64 with compensated sums; synchronous.

@@ -212,3 +212,43 @@ def test_species_library_exports_the_bridge_symbols():
     assert os.path.exists(path), "build it with SCALARS=1 CMA=1 glmmhdturb_amd/build_march.sh"
     lib = C.CDLL(path)
     assert lib.march_nv() == 7 and lib.march_nsp() == 2 and lib.march_nx() == 480
+
+
+def _exported(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+
+
+def test_glm_bridge_library_exports():
+    """libmhd480.so (build_mhd.sh): the seven spike_mhd.cu symbols (spike_mhd.cu:321-340), march_nv = 9."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", "libmhd480.so")
+    assert os.path.exists(path), "build it with glmmhdturb_amd/build_mhd.sh"
+    exported = _exported(path)
+    for sym in ("march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_glm", "march_set_gamma",
+                "march_run_dev", "march_run_dev_f64"):
+        assert sym in exported, sym
+    lib = C.CDLL(path)
+    assert lib.march_nv() == 9 and lib.march_nx() == 480 and lib.march_nsp() == 0
+
+
+def test_ct_library_exports_only_the_ct_bridge():
+    """libct480.so (build_ct.sh): spike_ctm.cu's symbols (spike_ctm.cu:300-328) with march_nv = 8,
+    and neither march_run_dev (it would read a ninth plane from the caller's 8-entry arrays) nor
+    march_set_glm (MarchBridge.set_glm must raise for a non-GLM library)."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", "libct480.so")
+    assert os.path.exists(path), "build it with glmmhdturb_amd/build_ct.sh"
+    exported = _exported(path)
+    for sym in ("march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_gamma", "ct_run", "ct_regs",
+                "ct_shmem"):
+        assert sym in exported, sym
+    for sym in ("march_run_dev", "march_run_dev_f64", "march_set_glm"):
+        assert sym not in exported, sym
+    lib = C.CDLL(path)
+    assert lib.march_nv() == 8 and lib.march_nsp() == 0
+
+
+def test_hydro_libraries_have_no_glm_or_ct_entry_points():
+    for name in ("libmarch480.so", "libmarch480_cma.so"):
+        exported = _exported(os.path.join(ROOT, "glmmhdturb_amd", name))
+        for sym in ("march_set_glm", "ct_run", "ct_regs", "ct_shmem"):
+            assert sym not in exported, (name, sym)

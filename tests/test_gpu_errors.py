@@ -151,3 +151,51 @@ def test_python_layer_raises_with_the_library_message(ctx64):
     with pytest.raises(_lib.GLMError, match="do_turb requires setup_forcing"):
         s.step_plm(u, o, p, 1e-3, 0.0, do_turb=True)
     assert np.isfinite(s.ctot_max(u))
+
+
+def test_cross_plane_aliasing_is_rejected(ctx64):
+    """An output plane that is a different input plane (o[w] == q[v], w != v) races in the z-chunked
+    marches exactly as o[v] == q[v] does: every step entry point checks all pairs."""
+    p, s, u, o = ctx64
+    lib = s.lib
+    o.fill_(5.0)
+    a = step_args(p, 1e-3, 0.0, riemann="hlld", recon="ppm", want_cfl=False)
+    optrs = [o[v].data_ptr() for v in range(9)]
+    optrs[3] = u[5].data_ptr()
+    expect(lib, lib.glm_step(s.ctx, planes(u)[0], _lib.plane_array(optrs)[0], C.byref(a)), ARG, "must differ")
+    # adjoint: lam0 against every lam1 and u0 plane
+    lam1, lam0 = s.new_state(), s.new_state()
+    l0 = [lam0[v].data_ptr() for v in range(9)]
+    l0[2] = lam1[7].data_ptr()
+    expect(lib, lib.glm_step_adjoint(s.ctx, planes(u)[0], planes(lam1)[0], _lib.plane_array(l0)[0], C.byref(a), None),
+           ARG, "must not alias")
+    l0 = [lam0[v].data_ptr() for v in range(9)]
+    l0[1] = u[4].data_ptr()
+    expect(lib, lib.glm_step_adjoint(s.ctx, planes(u)[0], planes(lam1)[0], _lib.plane_array(l0)[0], C.byref(a), None),
+           ARG, "must not alias")
+    # CT: 8 planes
+    c8, d8 = s.new_state(8), s.new_state(8)
+    dp = [d8[v].data_ptr() for v in range(8)]
+    dp[6] = c8[0].data_ptr()
+    c = step_args(p, 1e-3, 0.0, riemann="hlld", recon="plm", want_cfl=False)
+    expect(lib, lib.glm_ct_step(s.ctx, planes(c8, 8)[0], _lib.plane_array(dp)[0], C.byref(c), 0), ARG, "alias")
+    torch.cuda.synchronize()
+    assert bool((o == 5.0).all())
+
+
+def test_forcing_field_is_validated_before_copying(ctx64):
+    """set_afield / step_adjoint(afield=...) hand the library a raw address it reads 3 g^3 values
+    from: a field of the wrong dtype, shape, layout or device is refused on the host."""
+    p, s, u, o = ctx64
+    s.setup_forcing(p, amp=1.0)
+    g = p.turb_gs
+    good = s.afield()
+    s.set_afield(good)
+    for bad in (good.float(), good[:, :-1].contiguous(), good.transpose(1, 3), good.cpu()):
+        with pytest.raises(_lib.GLMError, match="forcing field"):
+            s.set_afield(bad)
+    lam1, lam0 = s.new_state(), s.new_state()
+    a = step_args(p, 1e-3, 0.0, do_turb=True, riemann="hlld", recon="ppm", want_cfl=False)
+    with pytest.raises(_lib.GLMError, match="forcing field"):
+        s.step_adjoint(u, lam1, lam0, a, afield=torch.zeros((3, g, g, g - 1), dtype=torch.float64, device=u.device))
+    assert torch.equal(s.afield(), good)

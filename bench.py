@@ -285,22 +285,79 @@ def host_layout():
 
 
 def cpu_baseline(seconds, extra_seconds=5.0):
-    """The oracle (C restatement, OpenMP) on a bounded sample of the same workload: 128^3, same
-    IC, forcing and scheme, as many steps as fit in about `seconds`.  Two shorter legs beside it:
-    the same scheme with an FP32 state, and the literal cpu_simd.jl scheme (full Hancock, MonCen,
-    LLF; cpu_simd.jl:62-101) in FP64."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    """The reference's own CPU path, restated: cpu_simd.jl's chunked z-stream (cpu_simd.jl:440-601;
+    oracle/cpu_stream.cpp: OpenMP over k-chunks, 8/4 lanes across x, first-touch buffers) on the
+    256^3 grid, in a child process with the OpenMP binding set (OMP_PROC_BIND=close,
+    OMP_PLACES=cores, all host threads).  Main figure: the GPU's scheme (full Hancock + PPM +
+    HLLD, FP64); beside it the same in FP32 and the literal cpu_simd.jl scheme (PLM MonCen + LLF)
+    in FP32 and FP64, plus the hydro march's oracle.  Input: the gradient IC (spike_mhd.cu:304-317),
+    on which no limiter short-cuts; a fixed CFL dt; no OU kick (cpu_simd.jl has none)."""
+    import subprocess
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    main = _cpu_leg(seconds, "f64", "ppm", "hlld")
-    main["cores"] = threads
-    main["host"] = host_layout()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", json.dumps([seconds, extra_seconds])]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, check=True).stdout
+    res = json.loads(out.strip().splitlines()[-1])
+    res["host"] = host_layout()
+    res["host"].update(omp_num_threads=env["OMP_NUM_THREADS"], omp_proc_bind=env["OMP_PROC_BIND"],
+                       omp_places=env["OMP_PLACES"], cpu_model=_cpu_model())
+    return res
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_child(seconds, extra_seconds):
+    """Runs in the child process of cpu_baseline: the z-stream legs and the hydro oracle leg."""
+    import ctypes as C
+
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import BUILD, Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    lib = C.CDLL(os.path.join(BUILD, "libcpustream.so"))
+    lib.cs_run.restype = C.c_double
+    lib.cs_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 7
+    N = 256
+    p = make_params(N)
+    orc = Oracle("f64")
+    u0 = orc.ic_gradient(N, N, N, p.gamma)
+    pfl = p.smallr * p.smallc ** 2 / p.gamma
+    dx = 1.0 / N
+    dt = p.courant * dx / orc.ctot_max(u0, p.gamma, p.smallr, pfl)
+    ch = p.courant * dx / dt / 3 * p.glm_ch_scale
+    fac = math.exp(-ch * dt / (p.glm_cp_coef * p.boxlen))
+
+    def leg(budget, prec, scheme):
+        u = np.ascontiguousarray(u0.astype(np.float64 if prec == 8 else np.float32))
+        el = lib.cs_run(u.ctypes.data, N, 1, prec, scheme, 0, p.gamma, p.smallr, pfl, dt, dx, ch, fac)
+        steps = max(1, min(50, int(budget / max(el, 1e-3))))
+        u = np.ascontiguousarray(u0.astype(np.float64 if prec == 8 else np.float32))
+        el = lib.cs_run(u.ctypes.data, N, steps, prec, scheme, 0, p.gamma, p.smallr, pfl, dt, dx, ch, fac)
+        assert el > 0 and np.isfinite(u).all()
+        name = ("full Hancock + PPM + HLLD (the GPU benchmark's scheme)" if scheme == 1
+                else "cpu_simd.jl's scheme: full Hancock + PLM MonCen + LLF")
+        return dict(value=N ** 3 * steps / el, unit="cell-updates/s", cores=threads, kind="port",
+                    sample=f"cpu_simd.jl chunked z-stream restated (oracle/cpu_stream.cpp), {N}^3 gradient IC, "
+                           f"{'FP64' if prec == 8 else 'FP32'}, {name}, {steps} steps, {el:.1f} s, {threads} threads")
+
+    main = leg(seconds, 8, 1)
     main["other_legs"] = {
-        "fp32_same_scheme": _cpu_leg(extra_seconds, "f32", "ppm", "hlld"),
-        "fp64_cpu_simd_scheme": _cpu_leg(extra_seconds, "f64", "plm", "llf"),
+        "fp32_same_scheme": leg(extra_seconds, 4, 1),
+        "fp32_cpu_simd_scheme": leg(extra_seconds, 4, 0),
+        "fp64_cpu_simd_scheme": leg(extra_seconds, 8, 0),
         "hydro_march_fp32": _cpu_hydro_leg(extra_seconds),
+        "note": "fp32_cpu_simd_scheme is the configuration of the reference's published CPU figure "
+                "(119 Mcell/s at 256^3 on 32 EPYC threads, REPORT.md:99-105)",
     }
-    return main
+    print(json.dumps(main))
 
 
 def _cpu_hydro_leg(seconds, M=128):
@@ -323,39 +380,6 @@ def _cpu_hydro_leg(seconds, M=128):
     return dict(value=M ** 3 * steps / busy, unit="cell-updates/s", cores=threads, kind="port",
                 sample=f"hydro march oracle (FP32, OpenMP, {threads} threads), {M}^3 of the hydro leg's input, "
                        f"{steps} steps, {busy:.1f} s")
-
-
-def _cpu_leg(seconds, prec, recon, riemann):
-    from oracle import OUForcing, Oracle, Scheme
-    from glmmhdturb_amd import glm_speed, pfloor, ramp_of
-
-    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
-    M = 128
-    p = make_params(M)
-    orc = Oracle(prec)
-    u = orc.ic_uniform(M, M, M, 1.0, 1.0, (0.5, 0.0, 0.0), p.gamma)
-    fo = OUForcing(orc, g=64, kmin=1.0, kmax=3.0, T=0.5, ndt=100, comp_frac=0.0, amp=AMP, seed=1)
-    fo.advance()
-    t, steps, busy = 0.0, 0, 0.0
-    while busy < seconds or steps < 1:
-        while t >= fo.next_time:
-            fo.advance()
-        t0 = time.perf_counter()
-        cm = orc.ctot_max(u, p.gamma, p.smallr, pfloor(p))
-        dt = min(p.courant * (1.0 / M) / cm, fo.next_time - t)
-        ch, fac = glm_speed(p, dt)
-        sc = Scheme(gamma=p.gamma, smallr=p.smallr, smallc=p.smallc, dt=dt, dx=1.0 / M, ch=ch, glm_fac=fac,
-                    turb_min_rho=p.turb_min_rho, ramp=ramp_of(p, t), predictor="full", recon=recon,
-                    riemann=riemann, slope="moncen", mh_revert=True, face_guard="iflo", update_floor=True,
-                    do_turb=True, turb_gs=64)
-        u = orc.step(u, sc, afield=fo.afield)
-        busy += time.perf_counter() - t0
-        t += dt
-        steps += 1
-    scheme = "PPM + full Hancock + HLLD" if recon == "ppm" else "cpu_simd.jl scheme: PLM MonCen + full Hancock + LLF"
-    return dict(value=M ** 3 * steps / busy, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"oracle C restatement ({prec.upper()}, OpenMP, {threads} threads), 128^3 of the same "
-                       f"workload ({scheme}), {steps} steps incl. the CFL reduction, {busy:.1f} s")
 
 
 def pct(xs, q):
@@ -703,6 +727,9 @@ def note(msg):
 
 def main():
     global ARGS
+    if len(sys.argv) > 2 and sys.argv[1] == "--cpu-child":
+        cpu_child(*json.loads(sys.argv[2]))
+        return
     args = parse()
     ARGS = args
     if args.gpus != 1:

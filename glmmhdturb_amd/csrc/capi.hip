@@ -316,6 +316,7 @@ int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* o
     return GLM_OK;
 }
 
+
 int glm_ic(glm_ctx* ctx, void* const* q, int kind, const double* args)
 {
     if (!ctx || !args) return set_err(GLM_ERR_ARG, "glm_ic: null argument");
@@ -448,8 +449,11 @@ int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out)
     return GLM_OK;
 }
 
+// CT step paths: the multi-pass kernel sequence, the fused z-march, the staged 3-D tiles.
+enum { CT_PASSES = 0, CT_MARCH = 1, CT_TILES = 2 };
+
 static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope,
-                        bool fused, const char* who)
+                        int mode, const char* who)
 {
     if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, who);
     int e = check_params(p);
@@ -460,7 +464,7 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
         if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers");
     if (planes_alias(q, 8, o, 8)) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers (input and output alias)");
     if (p->do_turb && !ctx->afield) return set_err(GLM_ERR_STATE, "glm_ct_step: do_turb without glm_ou_setup");
-    if (!fused && !ctx->ct_scratch) {
+    if (mode == CT_PASSES && !ctx->ct_scratch) {
         if (hipMalloc(&ctx->ct_scratch, ct_scratch_bytes(ctx->n, ctx->prec)) != hipSuccess)
             return set_err(GLM_ERR_ALLOC, "glm_ct_step: scratch allocation failed");
     }
@@ -473,7 +477,10 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
     h.update_floor = p->update_floor; h.do_turb = p->do_turb; h.tgs = ctx->g; h.afield = ctx->afield;
     h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2; h.zchunk = p->zchunk;
     if (p->want_cfl) HIPCHK(launch_clear_u64(ctx->d_cmax + 2, 1, ctx->stream));
-    if (fused) {
+    if (mode == CT_TILES) {
+        if (ctx->prec == 8) HIPCHK(launch_ct_tiles<double>(q, o, h, ctx->stream));
+        else HIPCHK(launch_ct_tiles<float>(q, o, h, ctx->stream));
+    } else if (mode == CT_MARCH) {
         const int key = 1000 + p->riemann;
         if (ctx->ct_slots_key != key) {
             int dev = 0, cus = 0, occ = 0;
@@ -497,12 +504,17 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
 
 int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
 {
-    return ct_step_impl(ctx, q, o, p, ct_face_slope, true, "glm_ct_step: null argument");
+    return ct_step_impl(ctx, q, o, p, ct_face_slope, CT_MARCH, "glm_ct_step: null argument");
 }
 
 int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
 {
-    return ct_step_impl(ctx, q, o, p, ct_face_slope, false, "glm_ct_step_passes: null argument");
+    return ct_step_impl(ctx, q, o, p, ct_face_slope, CT_PASSES, "glm_ct_step_passes: null argument");
+}
+
+int glm_ct_step_tiles(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
+{
+    return ct_step_impl(ctx, q, o, p, ct_face_slope, CT_TILES, "glm_ct_step_tiles: null argument");
 }
 
 int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax)

@@ -1,6 +1,7 @@
 // ct.hip -- launchers of the constrained-transport kernels (ct_impl.hpp).
 #include "ct_impl.hpp"
 #include "ct_march.hpp"
+#include "ct_tile.hpp"
 #include "ct_launch.hpp"
 
 namespace glm {
@@ -16,9 +17,9 @@ static CTArgs<T> make_ct_args(void* const* q, void* const* o, const CTArgsHost& 
         a.q.p[v] = v < 8 ? (const T*)q[v] : nullptr;
         a.o.p[v] = v < 8 ? (T*)o[v] : nullptr;
     }
-    a.prim = (T*)scratch;
-    a.flx = a.prim + 8 * n;
-    a.emf = a.flx + 24 * n;
+    a.prim = (T*)scratch;                       // the multi-pass step's scratch; null for the one-kernel steps
+    a.flx = scratch ? a.prim + 8 * n : nullptr;
+    a.emf = scratch ? a.flx + 24 * n : nullptr;
     a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
     a.gamma = (T)h.gamma; a.igm1 = (T)(1.0 / (h.gamma - 1.0)); a.smallr = (T)h.smallr; a.smallp = (T)h.smallp;
     double dtdx = h.dt / h.dx;
@@ -49,6 +50,31 @@ hipError_t launch_ct_step(void* const* q, void* const* o, const CTArgsHost& h, v
     hipLaunchKernelGGL((ct_ctot_kernel<T>), dim3(2048), b, 0, st, o9, h.nx, h.ny, h.nz, (T)h.gamma, (T)h.smallr,
                        (T)h.smallp, h.cmax);
     return hipGetLastError();
+}
+
+// The CT step over staged 3-D tiles (ct_tile.hpp), then the next step's CFL max as the other paths.
+template <typename T>
+hipError_t launch_ct_tiles(void* const* q, void* const* o, const CTArgsHost& h, hipStream_t st)
+{
+    CTArgs<T> a = make_ct_args<T>(q, o, h, nullptr);
+    const int tx = (h.nx + CTT_OX - 1) / CTT_OX, ty = (h.ny + CTT_OY - 1) / CTT_OY;
+    const int tz = (h.nz + CTTile<T>::OZ - 1) / CTTile<T>::OZ;
+    const long long nt = (long long)tx * ty * tz;
+    if (nt > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((ct_tile_kernel<T>), dim3((unsigned)nt), dim3(CTT_NT), 0, st, a, tx, ty, tz);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !h.want_cfl) return e;
+    Ptr9<const T> o9;
+    for (int v = 0; v < 9; ++v) o9.p[v] = v < 8 ? (const T*)o[v] : nullptr;
+    hipLaunchKernelGGL((ct_ctot_kernel<T>), dim3(2048), dim3(CT_NT), 0, st, o9, h.nx, h.ny, h.nz, (T)h.gamma,
+                       (T)h.smallr, (T)h.smallp, h.cmax);
+    return hipGetLastError();
+}
+
+template <typename T>
+const void* ct_tile_ptr()
+{
+    return (const void*)&ct_tile_kernel<T>;
 }
 
 template <typename T>
@@ -125,7 +151,9 @@ const void* ct_kernel_ptr()
     template hipError_t launch_ct_divb<T>(void* const*, int, int, int, double, unsigned long long*, hipStream_t); \
     template const void* ct_kernel_ptr<T>();                                                                    \
     template const void* ct_march_ptr<T>(int);                                                                  \
-    template hipError_t launch_ct_march<T>(void* const*, void* const*, const CTArgsHost&, int, hipStream_t);
+    template hipError_t launch_ct_march<T>(void* const*, void* const*, const CTArgsHost&, int, hipStream_t);   \
+    template hipError_t launch_ct_tiles<T>(void* const*, void* const*, const CTArgsHost&, hipStream_t);        \
+    template const void* ct_tile_ptr<T>();
 GLM_CT_INST(double)
 GLM_CT_INST(float)
 #undef GLM_CT_INST

@@ -192,6 +192,49 @@ GLM_DEV void ct_face_flux(const T m2[8], const T m1[8], const T c0[8], const T p
     else ct_riemann<T, RIEM_HLL, D>(L, R, bn, a, f);
 }
 
+// Update epilogue of one cell (ct_mhd.jl has none; the GLM rules of glmmhd_turb.jl:448-475): the void
+// floor with the cell-centred field (bcx, bcy, bcz) and the OU kick with its energy floors.
+template <typename T>
+GLM_DEV void ct_epilogue(T r[5], T bcx, T bcy, T bcz, int i, int j, int k, const CTArgs<T>& a)
+{
+    if (a.update_floor && r[0] < a.smallr) {
+        T emag = T(0.5) * (bcx * bcx + bcy * bcy + bcz * bcz);
+        r[0] = a.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0); r[4] = a.smallp * a.igm1 + emag;
+    }
+    if (a.do_turb && r[0] >= a.turb_min_rho) {
+        const int g = a.tgs;
+        const T gg = T(g);
+        T cx = ((T(i) + T(0.5)) * a.dx) / a.lx * gg, cy = ((T(j) + T(0.5)) * a.dx) / a.ly * gg;
+        T cz = ((T(k) + T(0.5)) * a.dx) / a.lz * gg;
+        T fx = floor(cx), fy = floor(cy), fz = floor(cz);
+        int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+        T wx = cx - fx, wy = cy - fy, wz = cz - fz;
+        int i0w = cwrap(i0, g), i1w = cwrap(i0 + 1, g), j0w = cwrap(j0, g), j1w = cwrap(j0 + 1, g);
+        int k0w = cwrap(k0, g), k1w = cwrap(k0 + 1, g);
+        size_t g3 = (size_t)g * g * g;
+        T acc[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const T* af = a.afield + cc * g3;
+            auto at = [&](int x, int y, int z) { return af[(size_t)x + (size_t)g * ((size_t)y + (size_t)g * z)]; };
+            T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
+                    (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
+                   ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
+                    (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
+            acc[cc] = a.ramp * t;
+        }
+        T rho = r[0], rhom = tmax(rho, a.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc, e = r[4];
+        e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
+        e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
+        e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
+        T m2 = r[1] + rhom * acc[0] * a.dt, m3 = r[2] + rhom * acc[1] * a.dt, m4 = r[3] + rhom * acc[2] * a.dt;
+        e = tmax(e + T(0.5) * m2 * m2 * irm, fl);
+        e = tmax(e + T(0.5) * m3 * m3 * irm, fl);
+        e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
+        r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
+    }
+}
+
 // Pass 1: cell-centred B and primitives (ct_mhd.jl:85-96 bcc, :26-34 cellprim)
 template <typename T>
 __global__ void __launch_bounds__(CT_NT) ct_prim_kernel(CTArgs<T> a)
@@ -285,43 +328,7 @@ __global__ void __launch_bounds__(CT_NT) ct_update_kernel(CTArgs<T> a)
         T bxn = a.q[5][c] - a.dtdx * ((ez[jp] - ez[c]) - (ey[kp] - ey[c]));
         T byn = a.q[6][c] - a.dtdx * ((ex[kp] - ex[c]) - (ez[ip] - ez[c]));
         T bzn = a.q[7][c] - a.dtdx * ((ey[ip] - ey[c]) - (ex[jp] - ex[c]));
-        if (a.update_floor && r[0] < a.smallr) {
-            T bcx = a.prim[5 * n + c], bcy = a.prim[6 * n + c], bcz = a.prim[7 * n + c];
-            T emag = T(0.5) * (bcx * bcx + bcy * bcy + bcz * bcz);
-            r[0] = a.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0); r[4] = a.smallp * a.igm1 + emag;
-        }
-        if (a.do_turb && r[0] >= a.turb_min_rho) {
-            const int g = a.tgs;
-            const T gg = T(g);
-            T cx = ((T(i) + T(0.5)) * a.dx) / a.lx * gg, cy = ((T(j) + T(0.5)) * a.dx) / a.ly * gg;
-            T cz = ((T(k) + T(0.5)) * a.dx) / a.lz * gg;
-            T fx = floor(cx), fy = floor(cy), fz = floor(cz);
-            int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-            T wx = cx - fx, wy = cy - fy, wz = cz - fz;
-            int i0w = cwrap(i0, g), i1w = cwrap(i0 + 1, g), j0w = cwrap(j0, g), j1w = cwrap(j0 + 1, g);
-            int k0w = cwrap(k0, g), k1w = cwrap(k0 + 1, g);
-            size_t g3 = (size_t)g * g * g;
-            T acc[3];
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-                const T* af = a.afield + cc * g3;
-                auto at = [&](int x, int y, int z) { return af[(size_t)x + (size_t)g * ((size_t)y + (size_t)g * z)]; };
-                T t = (((at(i0w, j0w, k0w) * (T(1) - wx) + at(i1w, j0w, k0w) * wx) * (T(1) - wy) +
-                        (at(i0w, j1w, k0w) * (T(1) - wx) + at(i1w, j1w, k0w) * wx) * wy) * (T(1) - wz) +
-                       ((at(i0w, j0w, k1w) * (T(1) - wx) + at(i1w, j0w, k1w) * wx) * (T(1) - wy) +
-                        (at(i0w, j1w, k1w) * (T(1) - wx) + at(i1w, j1w, k1w) * wx) * wy) * wz);
-                acc[cc] = a.ramp * t;
-            }
-            T rho = r[0], rhom = tmax(rho, a.smallr), irm = trcp(rhom), fl = rho * a.smallc * a.smallc, e = r[4];
-            e = tmax(e - T(0.5) * r[1] * r[1] * irm, fl);
-            e = tmax(e - T(0.5) * r[2] * r[2] * irm, fl);
-            e = tmax(e - T(0.5) * r[3] * r[3] * irm, fl);
-            T m2 = r[1] + rhom * acc[0] * a.dt, m3 = r[2] + rhom * acc[1] * a.dt, m4 = r[3] + rhom * acc[2] * a.dt;
-            e = tmax(e + T(0.5) * m2 * m2 * irm, fl);
-            e = tmax(e + T(0.5) * m3 * m3 * irm, fl);
-            e = tmax(e + T(0.5) * m4 * m4 * irm, fl);
-            r[1] = m2; r[2] = m3; r[3] = m4; r[4] = e;
-        }
+        ct_epilogue(r, a.prim[5 * n + c], a.prim[6 * n + c], a.prim[7 * n + c], i, j, k, a);
 #pragma unroll
         for (int t = 0; t < 5; ++t) a.o[t][c] = r[t];
         a.o[5][c] = bxn; a.o[6][c] = byn; a.o[7][c] = bzn;

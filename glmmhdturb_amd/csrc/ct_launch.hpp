@@ -30,5 +30,10 @@ template <typename T>
 hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, int slots, hipStream_t st);
 template <typename T>
 const void* ct_march_ptr(int riemann);
+// The same step as one kernel over staged 3-D tiles (ct_tile.hpp) and that kernel (resource queries).
+template <typename T>
+hipError_t launch_ct_tiles(void* const* q, void* const* o, const CTArgsHost& h, hipStream_t st);
+template <typename T>
+const void* ct_tile_ptr();
 
 }  // namespace glm

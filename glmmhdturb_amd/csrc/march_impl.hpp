@@ -182,10 +182,10 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
     if (RECON == RECON_PPM) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
-            T el, eh;
-            ppm_edges(l[v], m0[v], r[v], el, eh);
-            lo[v] = mh[v] + el - m0[v];
-            hi[v] = mh[v] + eh - m0[v];
+            T dl, dr;
+            ppm_offsets(l[v], m0[v], r[v], dl, dr);
+            lo[v] = mh[v] - dl;
+            hi[v] = mh[v] + dr;
         }
         if (__builtin_expect(!up, 0)) {
 #pragma unroll

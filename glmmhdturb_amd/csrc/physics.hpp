@@ -97,6 +97,45 @@ GLM_HD double tsqrt(double a)
     return __builtin_sqrt(a);
 #endif
 }
+// Square root and reciprocal square root together (positive operands): the root as tsqrt, and
+// 1/sqrt(a) from the same Goldschmidt step (2h, about an ulp from the correctly rounded value).
+GLM_HD double tsqrt_rs(double a, double& rs)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    rs = h + h;
+    const double d = __builtin_fma(-g, g, a);
+    return __builtin_fma(d, h, g);
+#else
+    const double q = __builtin_sqrt(a);
+    rs = 1.0 / q;
+    return q;
+#endif
+}
+GLM_HD float tsqrt_rs(float a, float& rs)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float q = __builtin_amdgcn_sqrtf(a);
+    rs = __builtin_amdgcn_rsqf(a);
+    return q;
+#else
+    const float q = __builtin_sqrtf(a);
+    rs = 1.0f / q;
+    return q;
+#endif
+}
+// Any other scalar type (the dual numbers of the CPU checks): the root and its reciprocal.
+template <typename T>
+GLM_HD T tsqrt_rs(T a, T& rs)
+{
+    T q = tsqrt(a);
+    rs = trcp(q);
+    return q;
+}
 GLM_HD float tsqrt(float a)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -263,6 +302,35 @@ GLM_HD void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
     hi = flat ? q0 : (lo_os ? T(3) * q0 - T(2) * ql : qr);
 }
 
+// The CW84-limited parabola of ppm_edges as edge offsets dl = q0 - lo, dr = hi - q0, in min/max
+// form.  With a = slope - curve and b = slope + curve the unlimited offsets, ppm_edges' tests are
+// (flat) a*b <= 0 -> both 0; (diff > lim) <=> |a| > 2|b| -> a = 2b; (diff < -lim) <=> |b| > 2|a|
+// -> b = 2a (diff = (a^2 - b^2)/2 and lim = (a + b)^2/6, so diff > lim <=> (a + b)(a - 2b) > 0).
+// With s the sign of a + b that is a' = s max(0, min(s a, 2 s b)) and b' = s max(0, min(s b, 2 s a)):
+// the same limited parabola up to rounding, with no selects (the limiter is continuous at each
+// switch, so a switch decided differently by rounding moves the result by rounding only).
+GLM_HD double flip_sign(double x, double ref)
+{
+    const unsigned long long m = __builtin_bit_cast(unsigned long long, ref) & 0x8000000000000000ull;
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ m);
+}
+GLM_HD float flip_sign(float x, float ref)
+{
+    const unsigned m = __builtin_bit_cast(unsigned, ref) & 0x80000000u;
+    return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) ^ m);
+}
+template <typename T>
+GLM_HD void ppm_offsets(T qm, T q0, T qp, T& dl, T& dr)
+{
+    T slope = T(0.25) * (qp - qm);
+    T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
+    T a = slope - curve, b = slope + curve;
+    T sum = a + b;
+    T sa = flip_sign(a, sum), sb = flip_sign(b, sum);
+    dl = flip_sign(tmax(tmin(sa, sb + sb), T(0)), sum);
+    dr = flip_sign(tmax(tmin(sb, sa + sa), T(0)), sum);
+}
+
 // spike_mhd.cu:164-172 ppm1 (both edges).
 template <typename T>
 GLM_HD void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
@@ -359,33 +427,35 @@ GLM_HD void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     T iL = trcp(SL - SM), iR = trcp(SR - SM);
     T dLs = tmax(dL * aL * iL, T(1e-12));
     T dRs = tmax(dR * aR * iR, T(1e-12));
-    T sqdLs = tsqrt(dLs), sqdRs = tsqrt(dRs);
+    T rsL, rsR;
+    T sqdLs = tsqrt_rs(dLs, rsL), sqdRs = tsqrt_rs(dRs, rsR);
     T absbn = tabs(bn);
-    T SLs = SM - absbn * trcp(sqdLs);
-    T SRs = SM + absbn * trcp(sqdRs);
+    T SLs = SM - absbn * rsL;
+    T SRs = SM + absbn * rsR;
     // star states (glmmhd_turb.jl:227-239)
     T vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
     {
+        // the degenerate star (|denK| < 1e-12) keeps the side's state: fac = 0 and gb = 1 give it exactly
         T denK = dL * aL * (SL - SM) - bn * bn;
         bool sm = tabs(denK) < T(1e-12);
         T ik = trcp(denK);
-        T fac = bn * (SM - uL) * ik;
-        T gb = (dL * aL * aL - bn * bn) * ik;
-        vyLs = sm ? vL : vL - byL * fac;
-        vzLs = sm ? wL : wL - bzL * fac;
-        byLs = sm ? byL : byL * gb;
-        bzLs = sm ? bzL : bzL * gb;
+        T fac = sm ? T(0) : bn * (SM - uL) * ik;
+        T gb = sm ? T(1) : (dL * aL * aL - bn * bn) * ik;
+        vyLs = vL - byL * fac;
+        vzLs = wL - bzL * fac;
+        byLs = byL * gb;
+        bzLs = bzL * gb;
     }
     {
         T denK = dR * aR * (SR - SM) - bn * bn;
         bool sm = tabs(denK) < T(1e-12);
         T ik = trcp(denK);
-        T fac = bn * (SM - uR) * ik;
-        T gb = (dR * aR * aR - bn * bn) * ik;
-        vyRs = sm ? vR : vR - byR * fac;
-        vzRs = sm ? wR : wR - bzR * fac;
-        byRs = sm ? byR : byR * gb;
-        bzRs = sm ? bzR : bzR * gb;
+        T fac = sm ? T(0) : bn * (SM - uR) * ik;
+        T gb = sm ? T(1) : (dR * aR * aR - bn * bn) * ik;
+        vyRs = vR - byR * fac;
+        vzRs = wR - bzR * fac;
+        byRs = byR * gb;
+        bzRs = bzR * gb;
     }
     T sgn = bn >= T(0) ? T(1) : T(-1);
     T invsum = trcp(sqdLs + sqdRs);
@@ -416,11 +486,13 @@ GLM_HD void hlld_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     T U[9] = {d, d * u, d * v, d * w, E, bn, by, bz, T(0)};
     T Us[9] = {ds, ds * SM, ds * vys, ds * vzs, Es, bn, bys, bzs, T(0)};
     T Uss[9] = {ds, ds * SM, ds * vyss, ds * vzss, Ess, bn, byss, bzss, T(0)};
+    // F* = F + S (U* - U); F** = F + S** U** - (S** - S) U* - S U = F* + S** (U** - U*), and U** - U*
+    // vanishes in rho, m_n and Bn (glmmhd_turb.jl:262-271)
+    const T w2 = outer ? T(0) : Ss;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         T fo = F[k] + S * (Us[k] - U[k]);
-        T fi = F[k] + Ss * Uss[k] - (Ss - S) * Us[k] - S * U[k];
-        f[k] = outer ? fo : fi;
+        f[k] = (k == 0 || k == 1 || k == 5 || k == 8) ? fo : fo + w2 * (Uss[k] - Us[k]);
     }
 }
 

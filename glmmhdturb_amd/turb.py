@@ -368,16 +368,21 @@ class Solver:
         return out.value
 
     # ---- constrained transport ------------------------------------------------------------
-    def ct_step(self, qold, qnew, args: GLMParams, face_slope: int = 0, passes: bool = False):
-        """One CT step (ct_mhd.jl step!, :174-184): the fused z-march kernel, or with passes=True
-        the multi-pass kernel sequence (same results to rounding)."""
+    def ct_step(self, qold, qnew, args: GLMParams, face_slope: int = 0, passes: bool = False, path: str = "march"):
+        """One CT step (ct_mhd.jl step!, :174-184).  path "march": the fused z-march kernel;
+        "passes" (or passes=True): the multi-pass kernel sequence; "tiles": one kernel over staged
+        3-D tiles (spike_ct2.cu design).  All three give the same results to rounding."""
+        if passes:
+            path = "passes"
+        fns = {"march": self.lib.glm_ct_step, "passes": self.lib.glm_ct_step_passes, "tiles": self.lib.glm_ct_step_tiles}
+        if path not in fns:
+            raise ValueError(f"ct_step path must be one of {sorted(fns)}, not {path!r}")
         qa, kq = self._planes(qold, 8)
         oa, ko = self._planes(qnew, 8)
-        if args.zchunk == 0 and self.ct_zchunk and not passes:
+        if args.zchunk == 0 and self.ct_zchunk and path == "march":
             args = GLMParams.from_buffer_copy(args)
             args.zchunk = self.ct_zchunk
-        fn = self.lib.glm_ct_step_passes if passes else self.lib.glm_ct_step
-        check(fn(self.ctx, qa, oa, C.byref(args), int(face_slope)))
+        check(fns[path](self.ctx, qa, oa, C.byref(args), int(face_slope)))
 
     def hydro_step(self, qold, qnew, args: GLMParams, nsp: int = 0, cma: bool = False):
         """One step of the hydro march (spike_25d.cu march, build_march.sh scheme) on

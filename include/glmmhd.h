@@ -158,6 +158,11 @@ int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* 
  * primitives, face fluxes, edge EMFs, update, each its own pass over a 35-plane scratch.  Same
  * arguments and results (to rounding) as glm_ct_step. */
 int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope);
+/* The same CT step as one kernel over staged 3-D tiles (spike_ct2.cu:147-210 ct_step2 and
+ * spike_ct3.cu:148-190 ct_step3 design): an 8 x 8 x 4 (FP32) or 8 x 8 x 2 (FP64) tile of owned cells
+ * per workgroup, its primitives and every face flux it needs staged in LDS, then the hydro update
+ * and the face-B curl from edge EMFs.  Same arguments and results (to rounding) as glm_ct_step. */
+int glm_ct_step_tiles(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope);
 /* CFL max of a CT state (cell-centred B); synchronous. */
 int glm_ct_ctot_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax);
 /* max |div B| of a CT state from the face fields (ct_mhd.jl:191-194); synchronous. */

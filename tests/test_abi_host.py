@@ -103,6 +103,7 @@ def test_ens_create_rejects_bad_arguments_without_gpu_calls():
 
 def test_random_boxes_follow_the_workload_ranges():
     from glmmhdturb_amd import random_boxes
+    from glmmhdturb_amd.ensemble import AMP_MACH1_G64, MACH_AMP_EXPONENT
     bx = random_boxes(64, seed=7)
     assert len(bx) == 64 and {b.N for b in bx} <= {32, 48, 64, 96, 128}
     assert len({b.seed for b in bx}) == 64
@@ -111,7 +112,7 @@ def test_random_boxes_follow_the_workload_ranges():
         assert 0.5 <= beta <= 100.0
         g = min(64, b.N)
         assert b.turb_gs == g
-        mach = (b.amp / (1e5 * (g / 64) ** 3)) ** (2 / 3)
+        mach = (b.amp / (AMP_MACH1_G64 * (g / 64) ** 3)) ** MACH_AMP_EXPONENT
         assert 0.3 - 1e-12 <= mach <= 2.0 + 1e-12
     assert random_boxes(64, seed=7) == bx
 

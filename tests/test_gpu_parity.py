@@ -96,8 +96,7 @@ def grid3d(orc):
 
 
 COMBOS = [(pr, rc, rm, sl) for pr in ("full", "oned") for rc in ("plm", "ppm", "ppm_spike")
-          for rm in ("llf", "llf_bound", "hll", "hlld") for sl in ("moncen",)]
-COMBOS += [("full", "plm", "hlld", "minmod"), ("oned", "ppm", "hll", "minmod")]
+          for rm in ("llf", "llf_bound", "hll", "hlld") for sl in ("moncen", "minmod")]
 
 
 @pytest.mark.parametrize("pred,recon,riem,slope", COMBOS)

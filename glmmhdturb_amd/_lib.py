@@ -34,6 +34,7 @@ EXPORTS = (
     "glm_ens_sync", "glm_ens_sweeps", "glm_ens_current", "glm_ens_log", "glm_ens_timing", "glm_ens_configure",
     "glm_ens_tune",
     "glm_step_adjoint", "glm_step_adjoint_passes", "glm_adjoint_force_dot", "glm_ou_set_afield",
+    "glm_diagnostics_noalloc",
 )
 ENS_NLOG = 8
 ENS_LOG_LEN = 256
@@ -88,6 +89,7 @@ def load(path: str | None = None):
         "glm_ctot_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_fast_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_diagnostics": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
+        "glm_diagnostics_noalloc": (i, [pp, i, i, i, i, C.POINTER(GLMParams), vp, vp]),
         "glm_ic": (i, [vp, pp, i, C.POINTER(d)]),
         "glm_ou_setup": (i, [vp, i, d, d, d, i, d, d, C.c_ulonglong]),
         "glm_ou_advance": (i, [vp]), "glm_ou_afield": (vp, [vp]), "glm_ou_copy_afield": (i, [vp, vp]), "glm_ou_steps": (C.c_longlong, [vp]),

@@ -316,6 +316,66 @@ int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* o
     return GLM_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Diagnostics without allocation.  The per-workgroup partials of the diagnostics reduction live
+// in `scratch`, a file-scope device array (one copy per GPU, held until the process ends).  In
+// host code `scratch` names a host-side stand-in, so its device address is taken once per device
+// with hipGetSymbolAddress and only that address is handed to the kernels.
+// ------------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_MAX_PARTS = 8192;   // diagnostics workgroups of 16384 cells: grids up to 2^27 cells (512^3)
+constexpr size_t NOALLOC_PART_BYTES = 136;   // one workgroup's partial (7 compensated sums, divB, rho_min, finite)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_MAX_PARTS * NOALLOC_PART_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on the current device (looked up on the first call per device).
+static int noalloc_scratch(void** out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return set_err(GLM_ERR_STATE, "glm_diagnostics_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return set_err(GLM_ERR_STATE, "glm_diagnostics_noalloc: no device address for the static scratch");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return GLM_OK;
+}
+
+extern "C" {
+
+int glm_diagnostics_noalloc(void* const* q, int nx, int ny, int nz, int precision, const glm_params* p, void* stream,
+                            double* out)
+{
+    if (!out) return set_err(GLM_ERR_ARG, "glm_diagnostics_noalloc: null output");
+    if (nx < 1 || ny < 1 || nz < 1 || (precision != 8 && precision != 4))
+        return set_err(GLM_ERR_ARG, "glm_diagnostics_noalloc: grid must be >= 1 per axis and precision 8 or 4");
+    int e = check_planes(q, 9, "glm_diagnostics_noalloc: null plane pointer");
+    if (!e) e = check_params(p);
+    if (e) return e;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t np = diag_nparts(n);
+    if (np > NOALLOC_MAX_PARTS) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "glm_diagnostics_noalloc: %zu cells exceed the static scratch's maximum of %zu cells",
+                 n, NOALLOC_MAX_PARTS * DIAG_CELLS_PER_BLOCK);
+        return set_err(GLM_ERR_ARG, msg);
+    }
+    if (diag_part_bytes() > NOALLOC_PART_BYTES) return set_err(GLM_ERR_STATE, "glm_diagnostics_noalloc: partial larger than its slot");
+    void* parts = nullptr;
+    if ((e = noalloc_scratch(&parts))) return e;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == 8) HIPCHK(launch_diag<double>(cptr9<double>(q), nx, ny, nz, p->dx, p->smallr, parts, out, st));
+    else HIPCHK(launch_diag<float>(cptr9<float>(q), nx, ny, nz, p->dx, p->smallr, parts, out, st));
+    return GLM_OK;
+}
 
 int glm_ic(glm_ctx* ctx, void* const* q, int kind, const double* args)
 {

@@ -129,6 +129,18 @@ int glm_fast_max(glm_ctx* ctx, void* const* q, const glm_params* p, double* cmax
  *      sum m.B/rho (cross helicity), sum E, sum psi^2   (plain cell sums; times dV for integrals) */
 int glm_diagnostics(glm_ctx* ctx, void* const* q, const glm_params* p, double* out);
 
+/* No reference counterpart.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): glm_diagnostics without a
+ * context and without any allocation.  q: 9 device planes of an nx*ny*nz grid in the given
+ * precision (8 or 4); out: 10 doubles in DEVICE memory, written in stream order on `stream`
+ * (hipStream_t as void*, NULL = the null stream), same values and order as glm_diagnostics.
+ * Uses p->dx and p->smallr.  Asynchronous: no host synchronisation.  The reduction's partials live
+ * in a static device array of about 1 MiB (per GPU, held until the process ends), which limits the
+ * grid to 2^27 cells (512^3); larger grids are rejected with GLM_ERR_ARG.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two streams at once) race. */
+int glm_diagnostics_noalloc(void* const* q, int nx, int ny, int nz, int precision, const glm_params* p, void* stream,
+                            double* out);
+
 /* Initial conditions: GLM_IC_ORSZAG_TANG args {gamma} (:2021-2036); GLM_IC_UNIFORM args
  * {rho0, p0, b0x, b0y, b0z, gamma} (:2164-2172); GLM_IC_GRADIENT args {gamma}
  * (spike_mhd.cu:304-317).  Asynchronous. */

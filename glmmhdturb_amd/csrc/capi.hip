@@ -170,22 +170,24 @@ int glm_set_stream(glm_ctx* ctx, void* stream)
 
 }  // extern "C"
 
+// One march step over 9 + nsp planes (nsp = 0 or 2 passive species).
 template <typename T>
-static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_params* p)
+static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_params* p, int nsp = 0)
 {
     MarchArgs<T> a;
     memset(&a, 0, sizeof a);
-    for (int v = 0; v < 9; ++v) {
+    const int nv = 9 + nsp;
+    for (int v = 0; v < nv; ++v) {
         if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_step: null plane pointer");
         a.q[v] = (const T*)q[v];
         a.o[v] = (T*)o[v];
     }
-    if (planes_alias(q, 9, o, 9)) return set_err(GLM_ERR_ARG, "glm_step: input and output planes must differ");
+    if (planes_alias(q, nv, o, nv)) return set_err(GLM_ERR_ARG, "glm_step: input and output planes must differ");
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
-    const int key = 1 + (((p->predictor * 4 + p->riemann) * 4 + p->recon) * 4 + p->slope);
+    const int key = 1 + (((p->predictor * 4 + p->riemann) * 4 + p->recon) * 4 + p->slope) + 1000 * nsp;
     if (c->slots_key != key) {
         int dev = 0, cus = 0, occ = 0;
-        const void* fn = march_kernel_ptr<T>(p->predictor, p->riemann, p->recon, p->slope);
+        const void* fn = march_kernel_ptr<T>(p->predictor, p->riemann, p->recon, p->slope, nsp);
         if (!fn) return set_err(GLM_ERR_ARG, "glm_step: unknown scheme");
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -216,7 +218,7 @@ static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_param
     a.want_cfl = p->want_cfl;
     a.cmax = c->d_cmax + c->cfl_parity;
     a.cmax_next = c->d_cmax + (1 - c->cfl_parity);
-    HIPCHK(launch_march<T>(a, p->predictor, p->riemann, p->recon, p->slope, c->stream));
+    HIPCHK(launch_march<T>(a, p->predictor, p->riemann, p->recon, p->slope, c->stream, nsp));
     if (p->want_cfl) {
         c->cfl_slot = c->cfl_parity;
         c->cfl_parity = 1 - c->cfl_parity;
@@ -232,6 +234,15 @@ int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
     int e = check_params(p);
     if (e) return e;
     return ctx->prec == 8 ? step_impl<double>(ctx, q, o, p) : step_impl<float>(ctx, q, o, p);
+}
+
+int glm_step_species(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp)
+{
+    if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, "glm_step_species: null argument");
+    if (nsp != 0 && nsp != 2) return set_err(GLM_ERR_ARG, "glm_step_species: nsp must be 0 or 2");
+    int e = check_params(p);
+    if (e) return e;
+    return ctx->prec == 8 ? step_impl<double>(ctx, q, o, p, nsp) : step_impl<float>(ctx, q, o, p, nsp);
 }
 
 int glm_cfl_max(glm_ctx* ctx, double* cmax)
@@ -713,10 +724,10 @@ static glm_ctx* g_ctx64 = nullptr;
 
 int march_nv(void) { return GLM_LIB_NV; }
 int march_nx(void) { return g_nx; }
-int march_nsp(void) { return GLM_LIB_NV == 7 ? 2 : 0; }
+int march_nsp(void) { return GLM_LIB_NV == 7 || GLM_LIB_NV == 11 ? 2 : 0; }
 void march_set_dtdx(float v) { g_dtdx = v; }
-#if GLM_LIB_NV == 9
-// Only the GLM-MHD library has a cleaning speed: MarchBridge.set_glm decides by this symbol's
+#if GLM_LIB_NV == 9 || GLM_LIB_NV == 11
+// Only the GLM-MHD libraries have a cleaning speed: MarchBridge.set_glm decides by this symbol's
 // presence (MarchBridge.jl:37-38, :50-53).
 void march_set_glm(float ch, float fac) { g_ch = ch; g_glmfac = fac; }
 #endif
@@ -776,6 +787,7 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kin
         err = kind == 1   ? glm_ct_step(c, q.data(), o.data(), &p, 0)
               : kind == 2 ? glm_hydro_step(c, q.data(), o.data(), &p)
               : kind == 3 ? glm_hydro_step_scalars(c, q.data(), o.data(), &p, 2, GLM_LIB_CMA)
+              : kind == 4 ? glm_step_species(c, q.data(), o.data(), &p, 2)
                           : glm_step(c, q.data(), o.data(), &p);
         std::swap(q, o);
     }
@@ -801,9 +813,10 @@ extern "C" {
 //        also the CT entry points;
 //   NV 8 (build_ct.sh): the CT march only (spike_ctm.cu:299-329) -- no march_run_dev, which would
 //        read a ninth (psi) plane the caller's 8-entry arrays do not hold;
-//   NV 5 / 7 (build_march.sh [-DSCALARS]): the hydro march (spike_25d.cu:559-576), no CT entry points.
-#if GLM_LIB_NV == 9 || GLM_LIB_NV == 5 || GLM_LIB_NV == 7
-constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : GLM_LIB_NV == 7 ? 3 : 0;
+//   NV 5 / 7 (build_march.sh [-DSCALARS]): the hydro march (spike_25d.cu:559-576), no CT entry points;
+//   NV 11 (NSP=2 build_mhd.sh): the GLM march with two passive species (spike_mhd.cu -DNSP=2).
+#if GLM_LIB_NV == 9 || GLM_LIB_NV == 5 || GLM_LIB_NV == 7 || GLM_LIB_NV == 11
+constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : GLM_LIB_NV == 7 ? 3 : GLM_LIB_NV == 11 ? 4 : 0;
 double march_run_dev(float* const* qv, float* const* ov, int nsteps)
 {
     return legacy_run<float>(qv, ov, nsteps, GLM_LIB_NV, LEGACY_KIND);

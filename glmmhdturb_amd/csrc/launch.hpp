@@ -13,15 +13,22 @@ hipError_t launch_march_pr(const MarchArgs<T>& a, int recon, int slope, hipStrea
 template <typename T, int PRED, int RIEM>
 const void* march_kernel_ptr_pr(int recon, int slope);
 template <typename T, int PRED, int RIEM>
+hipError_t launch_march_pr_sp(const MarchArgs<T>& a, int recon, int slope, hipStream_t st);
+template <typename T, int PRED, int RIEM>
+const void* march_kernel_ptr_pr_sp(int recon, int slope);
+template <typename T, int PRED, int RIEM>
 hipError_t launch_ens_pr(const EnsLaunch<T>& e, int recon, int slope, int grid, hipStream_t st);
 template <typename T, int PRED, int RIEM>
 const void* ens_kernel_ptr_pr(int recon, int slope);
 
+// nsp = 0: the 9-plane march; nsp = 2: with two passive species (11 planes).
 template <typename T>
-inline hipError_t launch_march(const MarchArgs<T>& a, int pred, int riem, int recon, int slope, hipStream_t st)
+inline hipError_t launch_march(const MarchArgs<T>& a, int pred, int riem, int recon, int slope, hipStream_t st,
+                               int nsp = 0)
 {
 #define GLM_CASE(P, R) \
-    if (pred == P && riem == R) return launch_march_pr<T, P, R>(a, recon, slope, st);
+    if (pred == P && riem == R) \
+        return nsp ? launch_march_pr_sp<T, P, R>(a, recon, slope, st) : launch_march_pr<T, P, R>(a, recon, slope, st);
     GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
     GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
 #undef GLM_CASE
@@ -30,10 +37,11 @@ inline hipError_t launch_march(const MarchArgs<T>& a, int pred, int riem, int re
 
 // Kernel symbol of one march variant (for resource queries); nullptr if unknown.
 template <typename T>
-inline const void* march_kernel_ptr(int pred, int riem, int recon, int slope)
+inline const void* march_kernel_ptr(int pred, int riem, int recon, int slope, int nsp = 0)
 {
 #define GLM_CASE(P, R) \
-    if (pred == P && riem == R) return march_kernel_ptr_pr<T, P, R>(recon, slope);
+    if (pred == P && riem == R) \
+        return nsp ? march_kernel_ptr_pr_sp<T, P, R>(recon, slope) : march_kernel_ptr_pr<T, P, R>(recon, slope);
     GLM_CASE(PRED_FULL, RIEM_LLF) GLM_CASE(PRED_FULL, RIEM_LLFB) GLM_CASE(PRED_FULL, RIEM_HLL) GLM_CASE(PRED_FULL, RIEM_HLLD)
     GLM_CASE(PRED_ONED, RIEM_LLF) GLM_CASE(PRED_ONED, RIEM_LLFB) GLM_CASE(PRED_ONED, RIEM_HLL) GLM_CASE(PRED_ONED, RIEM_HLLD)
 #undef GLM_CASE

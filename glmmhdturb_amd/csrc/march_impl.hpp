@@ -65,11 +65,16 @@ static_assert(NT <= 256 && NT >= NHELP && NO % 64 == 0, "tile shape");
 #define GLM_WAVES_F32 2
 #endif
 #define GLM_WAVES_PER_EU(T) (sizeof(T) == 8 ? GLM_WAVES_F64 : GLM_WAVES_F32)
+// With species the FP64 workgroup's 97 KB of LDS leaves one workgroup per CU: 512 registers per lane.
+#define GLM_WAVES_PER_EU_NS(T, NS) ((NS) > 0 && sizeof(T) == 8 ? 1 : GLM_WAVES_PER_EU(T))
+
+// Planes of a state: the 9 GLM-MHD planes and up to two passive species (rho X_s) after them.
+constexpr int NVMAX = 11;
 
 template <typename T>
 struct MarchArgs {
-    const T* q[9];
-    T* o[9];
+    const T* q[NVMAX];
+    T* o[NVMAX];
     int nx, ny, nz;
     int tiles_x, tiles_y, nzc, zchunk;
     Phys<T> ph;
@@ -86,8 +91,8 @@ struct MarchArgs {
 // kernel entry so they do not hold scalar registers across the march loop.
 template <typename T>
 struct ColdArgs {
-    const T* q[9];
-    T* o[9];
+    const T* q[NVMAX];
+    T* o[NVMAX];
     const T* afield;
     T dtdx, glm_fac, smallc, turb_min_rho, ramp, dt, dx, lx, ly, lz;
     int update_floor, do_turb, tgs, pad_;
@@ -115,21 +120,42 @@ GLM_DEV void helper_cell(int h, int& px, int& py)
     else { px = PX - 1; py = 2 + r - 2 * OX - OY; }
 }
 
-template <typename T>
-GLM_DEV void load9(const ColdArgs<T>& a, size_t off, T c[9])
+template <int NV, typename T>
+GLM_DEV void loadv(const ColdArgs<T>& a, size_t off, T c[NV])
 {
 #pragma unroll
-    for (int v = 0; v < 9; ++v) c[v] = gload(a.q[v] + off);
+    for (int v = 0; v < NV; ++v) c[v] = gload(a.q[v] + off);
+}
+
+// cons2prim plus the species fractions X = (rho X) / max(rho, smallr) (spike_mhd.cu:243-244).
+template <int NV, typename T>
+GLM_DEV void c2p(const T c[NV], T q[NV], const Phys<T>& s)
+{
+    cons2prim(c, q, s);
+    if (NV > 9) {
+        const T ir = trcp(q[0]);
+#pragma unroll
+        for (int v = 9; v < NV; ++v) q[v] = c[v] * ir;
+    }
+}
+
+// Species fluxes riding the face's mass flux (CMA, spike_mhd.cu:218-221): F_s = F_rho X_L if
+// F_rho >= 0, else F_rho X_R, with L and R the face states in the grid frame.
+template <int NV, typename T>
+GLM_DEV void species_flux(const T L[NV], const T R[NV], T f[NV])
+{
+#pragma unroll
+    for (int v = 9; v < NV; ++v) f[v] = f[0] >= T(0) ? f[0] * L[v] : f[0] * R[v];
 }
 
 // Face guard on one face state (glmmhd_turb.jl:618 iflo; spike_mhd.cu:145-147 clamp).
-template <typename T>
-GLM_DEV void guard(T f[9], const T m0[9], int g, const Phys<T>& s)
+template <int NV, typename T>
+GLM_DEV void guard(T f[NV], const T m0[NV], int g, const Phys<T>& s)
 {
     if (g == GUARD_IFLO) {
         if (__builtin_expect(f[0] < s.smallr || f[4] < s.pfl, 0)) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) f[v] = m0[v];
+            for (int v = 0; v < NV; ++v) f[v] = m0[v];
         }
     } else if (g == GUARD_CLAMP) {
         f[0] = tmax(f[0], s.smallr);
@@ -139,8 +165,8 @@ GLM_DEV void guard(T f[9], const T m0[9], int g, const Phys<T>& s)
 
 // Normal-only Hancock half step of the cell centre in direction D (spike_mhd.cu:149-161); hs is
 // the half slope.  Returns the primitive state.
-template <typename T, int D>
-GLM_DEV void hanc1d(const T c0[9], const T hs[9], T mh[9], T half_dtdx, const Phys<T>& s)
+template <typename T, int D, int NV>
+GLM_DEV void hanc1d(const T c0[NV], const T hs[NV], T mh[NV], T half_dtdx, const Phys<T>& s)
 {
     T t[9], fa[9], fb[9], u[9];
 #pragma unroll
@@ -153,35 +179,38 @@ GLM_DEV void hanc1d(const T c0[9], const T hs[9], T mh[9], T half_dtdx, const Ph
 #pragma unroll
     for (int v = 0; v < 9; ++v) u[v] -= half_dtdx * (fb[v] - fa[v]);
     cons2prim(u, mh, s);
+    // species: the advective half step X - dt/dx u_D hs (spike_mhd.cu:155-159, hs the half slope)
+#pragma unroll
+    for (int v = 9; v < NV; ++v) mh[v] = c0[v] - (T(2) * half_dtdx * c0[1 + D]) * hs[v];
 }
 
 // Face states of one cell in direction D from its left/right neighbours l, r, the cell m0 and
 // the predicted mh (glmmhd_turb.jl:378-403, :955-973; spike_mhd.cu:262-275).
-template <typename T, int PRED, int RECON, int SLOPE, int D>
-GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9], bool up,
-                       const MarchArgs<T>& a, T lo[9], T hi[9])
+template <typename T, int PRED, int RECON, int SLOPE, int D, int NV>
+GLM_DEV void faces_dir(const T l[NV], const T m0[NV], const T r[NV], const T mhf[NV], bool up,
+                       const MarchArgs<T>& a, T lo[NV], T hi[NV])
 {
     const Phys<T>& s = a.ph;
     // half slopes: the PLM faces, the normal-only predictor, and the PPM shock fallback
-    T hs[9];
+    T hs[NV];
     if (RECON != RECON_PPM || PRED == PRED_ONED) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(l[v], m0[v], r[v]);
+        for (int v = 0; v < NV; ++v) hs[v] = half_slope<T, SLOPE>(l[v], m0[v], r[v]);
     }
-    T mh[9];
+    T mh[NV];
     if (PRED == PRED_ONED) {
-        hanc1d<T, D>(m0, hs, mh, a.half_dtdx, s);
+        hanc1d<T, D, NV>(m0, hs, mh, a.half_dtdx, s);
         if (a.mh_revert && __builtin_expect(mh[0] <= s.smallr || mh[4] <= s.pfl, 0)) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) mh[v] = m0[v];
+            for (int v = 0; v < NV; ++v) mh[v] = m0[v];
         }
     } else {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) mh[v] = mhf[v];
+        for (int v = 0; v < NV; ++v) mh[v] = mhf[v];
     }
     if (RECON == RECON_PPM) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) {
+        for (int v = 0; v < NV; ++v) {
             T dl, dr;
             ppm_offsets(l[v], m0[v], r[v], dl, dr);
             lo[v] = mh[v] - dl;
@@ -189,7 +218,7 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
         }
         if (__builtin_expect(!up, 0)) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 T h = PRED == PRED_ONED ? hs[v] : half_slope<T, SLOPE>(l[v], m0[v], r[v]);
                 lo[v] = mh[v] - h;
                 hi[v] = mh[v] + h;
@@ -197,33 +226,41 @@ GLM_DEV void faces_dir(const T l[9], const T m0[9], const T r[9], const T mhf[9]
         }
     } else if (RECON == RECON_PPM_SPIKE) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) ppm_edges_spike(l[v], m0[v], r[v], lo[v], hi[v]);
+        for (int v = 0; v < NV; ++v) ppm_edges_spike(l[v], m0[v], r[v], lo[v], hi[v]);
         lo[0] = tmax(lo[0], s.smallr); lo[4] = tmax(lo[4], s.pfl);
         hi[0] = tmax(hi[0], s.smallr); hi[4] = tmax(hi[4], s.pfl);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) { lo[v] = lo[v] + mh[v] - m0[v]; hi[v] = hi[v] + mh[v] - m0[v]; }
+        for (int v = 0; v < NV; ++v) { lo[v] = lo[v] + mh[v] - m0[v]; hi[v] = hi[v] + mh[v] - m0[v]; }
         lo[0] = tmax(lo[0], s.smallr); lo[4] = tmax(lo[4], s.pfl);
         hi[0] = tmax(hi[0], s.smallr); hi[4] = tmax(hi[4], s.pfl);
     } else {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) { lo[v] = mh[v] - hs[v]; hi[v] = mh[v] + hs[v]; }
+        for (int v = 0; v < NV; ++v) { lo[v] = mh[v] - hs[v]; hi[v] = mh[v] + hs[v]; }
     }
-    guard(lo, m0, a.face_guard, s);
-    guard(hi, m0, a.face_guard, s);
+    guard<NV>(lo, m0, a.face_guard, s);
+    guard<NV>(hi, m0, a.face_guard, s);
 }
 
 // Update epilogue of one cell (glmmhd_turb.jl:440-476), CFL speed of the result
 // (glmmhd_turb.jl:1963-1971).  acc = (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) + (Fz_lo - Fz_hi).
-template <typename T>
-GLM_DEV double update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl,
+template <int NV, typename T>
+GLM_DEV double update_cell(T r[NV], const T acc[NV], const ColdArgs<T>& a, const Phys<T>& s, bool want_cfl,
                            ForceCache<T, NT>& fc, int gx, int gy, int gz)
 {
+    T xold[NV > 9 ? NV - 9 : 1];   // species fractions at the start of the step (for the floor)
+    if (NV > 9) {
+        const T ir = trcp(tmax(r[0], s.smallr));
 #pragma unroll
-    for (int v = 0; v < 9; ++v) r[v] = r[v] + a.dtdx * acc[v];
+        for (int v = 9; v < NV; ++v) xold[v - 9] = r[v] * ir;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) r[v] = r[v] + a.dtdx * acc[v];
     if (a.update_floor && r[0] < s.smallr) {
         T emag = T(0.5) * (r[5] * r[5] + r[6] * r[6] + r[7] * r[7]);
         r[0] = s.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0);
         r[4] = s.pfl * s.igm1 + emag;
+#pragma unroll
+        for (int v = 9; v < NV; ++v) r[v] = s.smallr * xold[v - 9];
     }
     r[8] = r[8] * a.glm_fac;
     if (a.do_turb && r[0] >= a.turb_min_rho) {
@@ -248,12 +285,12 @@ GLM_DEV double update_cell(T r[9], const T acc[9], const ColdArgs<T>& a, const P
     return cfl_speed(q, s);
 }
 
-// LDS of one march workgroup.
-template <typename T>
+// LDS of one march workgroup (NV planes: 9, or 11 with two species).
+template <typename T, int NV = 9>
 struct MarchLDS {
-    T P[9][NPL];    // primitives of the current plane over the tile grown by two cells
-    T XB[9][NPL];   // +x face states, then x fluxes, at the LDS plane position of the cell
-    T YB[9][NPL];
+    T P[NV][NPL];    // primitives of the current plane over the tile grown by two cells
+    T XB[NV][NPL];   // +x face states, then x fluxes, at the LDS plane position of the cell
+    T YB[NV][NPL];
     T FCB[6][NT];   // forcing interpolation cache (ForceCache)
     ColdArgs<T> ca;
 };
@@ -261,7 +298,7 @@ struct MarchLDS {
 template <typename T>
 GLM_DEV void stage_cold(const MarchArgs<T>& a, ColdArgs<T>& ca)
 {
-    for (int v = 0; v < 9; ++v) { ca.q[v] = a.q[v]; ca.o[v] = a.o[v]; }
+    for (int v = 0; v < NVMAX; ++v) { ca.q[v] = a.q[v]; ca.o[v] = a.o[v]; }
     ca.afield = a.afield;
     ca.dtdx = a.dtdx; ca.glm_fac = a.glm_fac; ca.smallc = a.smallc; ca.turb_min_rho = a.turb_min_rho;
     ca.ramp = a.ramp; ca.dt = a.dt; ca.dx = a.dx; ca.lx = a.lx; ca.ly = a.ly; ca.lz = a.lz;
@@ -272,12 +309,13 @@ GLM_DEV void stage_cold(const MarchArgs<T>& a, ColdArgs<T>& ca)
 // z-chunk zc, tile = tile_x + tiles_x * (tile_y + tiles_y * zc), and write the updated planes.
 // L.ca must hold `a`'s cold arguments, published by a barrier.  Returns the lane's max CFL speed
 // over the cells it updated.
-template <typename T, int PRED, int RIEM, int RECON, int SLOPE, typename A = MarchArgs<T>>
-GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE, int NS = 0, typename A = MarchArgs<T>>
+GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 {
-    T (&P)[9][NPL] = L.P;
-    T (&XB)[9][NPL] = L.XB;
-    T (&YB)[9][NPL] = L.YB;
+    constexpr int NV = 9 + NS;   // the GLM-MHD planes and NS passive species
+    T (&P)[NV][NPL] = L.P;
+    T (&XB)[NV][NPL] = L.XB;
+    T (&YB)[NV][NPL] = L.YB;
     const ColdArgs<T>& ca = L.ca;
     const Phys<T>& s = a.ph;
     const int tid = threadIdx.x;
@@ -304,28 +342,28 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
 
 
     // column window: primitives of planes q-1, q, q+1
-    T pm[9], pp[9];
+    T pm[NV], pp[NV];
     {
-        T c[9], p0[9];
-        load9(ca, col + plane * wrapi(zs - 2, nz), c); cons2prim(c, pm, s);
-        load9(ca, col + plane * wrapi(zs - 1, nz), c); cons2prim(c, p0, s);
-        load9(ca, col + plane * wrapi(zs, nz), c); cons2prim(c, pp, s);
+        T c[NV], p0[NV];
+        loadv<NV>(ca, col + plane * wrapi(zs - 2, nz), c); c2p<NV>(c, pm, s);
+        loadv<NV>(ca, col + plane * wrapi(zs - 1, nz), c); c2p<NV>(c, p0, s);
+        loadv<NV>(ca, col + plane * wrapi(zs, nz), c); c2p<NV>(c, pp, s);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) P[v][lo_own] = p0[v];
+        for (int v = 0; v < NV; ++v) P[v][lo_own] = p0[v];
         if (helper) {
-            T h[9];
-            load9(ca, hcol + plane * wrapi(zs - 1, nz), c); cons2prim(c, h, s);
+            T h[NV];
+            loadv<NV>(ca, hcol + plane * wrapi(zs - 1, nz), c); c2p<NV>(c, h, s);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) P[v][lo_ring] = h[v];
+            for (int v = 0; v < NV; ++v) P[v][lo_ring] = h[v];
         }
     }
     __syncthreads();
 
     // zprev: +z face state of plane q-1.  pend: (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) of plane q-1 plus
     // Fz(q-3/2), so the update of q-1 is U + dt/dx * (pend - Fz(q-1/2)).
-    T zprev[9], pend[9];
+    T zprev[NV], pend[NV];
 #pragma unroll
-    for (int v = 0; v < 9; ++v) { zprev[v] = T(0); pend[v] = T(0); }
+    for (int v = 0; v < NV; ++v) { zprev[v] = T(0); pend[v] = T(0); }
     double cmax = 0.0;
     ForceCache<T, NT> fc;
     fc.k0 = -0x7fffffff;
@@ -336,10 +374,10 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
         const bool inplane = q >= zs && q < ze;
         const bool do_upd = owned && q >= zs + 1;
 
-        T u0[9];
-        T p0[9];   // this column's cell of plane q, read back from the LDS plane (not carried in registers)
+        T u0[NV];
+        T p0[NV];   // this column's cell of plane q, read back from the LDS plane (not carried in registers)
 #pragma unroll
-        for (int v = 0; v < 9; ++v) p0[v] = P[v][lo_own];
+        for (int v = 0; v < NV; ++v) p0[v] = P[v][lo_own];
 
         // ---- reconstruction of this column's cell in plane q --------------------------------
         bool up = true;   // PPM strong-pressure-jump fallback flag (glmmhd_turb.jl:959)
@@ -347,12 +385,15 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
             up = !(spj(P[4][lo_own - 1], p0[4], P[4][lo_own + 1]) || spj(P[4][lo_own - PX], p0[4], P[4][lo_own + PX]) ||
                    spj(pm[4], p0[4], pp[4]));
         }
-        T mh[9];
+        T mh[NV];
         if (PRED == PRED_FULL) {
-            // glmmhd_turb.jl:165-172 hancock, :391-395 revert: U0 - dt/(2dx) sum_d [F_d(m0+s_d) - F_d(m0-s_d)]
-            T acc[9], t[9], f[9], hs[9];
+            // glmmhd_turb.jl:165-172 hancock, :391-395 revert: U0 - dt/(2dx) sum_d [F_d(m0+s_d) - F_d(m0-s_d)];
+            // species: the advective half step X - dt/dx (u hs_x + v hs_y + w hs_z)
+            T acc[9], t[9], f[9], hs[NV], sadv[NS > 0 ? NS : 1];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(P[v][lo_own - 1], p0[v], P[v][lo_own + 1]);
+            for (int v = 0; v < NV; ++v) hs[v] = half_slope<T, SLOPE>(P[v][lo_own - 1], p0[v], P[v][lo_own + 1]);
+#pragma unroll
+            for (int v = 9; v < NV; ++v) sadv[v - 9] = p0[1] * hs[v];
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
             flux_dir<T, 0>(t, acc, s);
@@ -362,7 +403,9 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
 #pragma unroll
             for (int v = 0; v < 9; ++v) acc[v] -= f[v];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(P[v][lo_own - PX], p0[v], P[v][lo_own + PX]);
+            for (int v = 0; v < NV; ++v) hs[v] = half_slope<T, SLOPE>(P[v][lo_own - PX], p0[v], P[v][lo_own + PX]);
+#pragma unroll
+            for (int v = 9; v < NV; ++v) sadv[v - 9] = sadv[v - 9] + p0[2] * hs[v];
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
             flux_dir<T, 1>(t, f, s);
@@ -374,7 +417,9 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
 #pragma unroll
             for (int v = 0; v < 9; ++v) acc[v] -= f[v];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) hs[v] = half_slope<T, SLOPE>(pm[v], p0[v], pp[v]);
+            for (int v = 0; v < NV; ++v) hs[v] = half_slope<T, SLOPE>(pm[v], p0[v], pp[v]);
+#pragma unroll
+            for (int v = 9; v < NV; ++v) sadv[v - 9] = sadv[v - 9] + p0[3] * hs[v];
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = p0[v] + hs[v];
             flux_dir<T, 2>(t, f, s);
@@ -389,108 +434,115 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
 #pragma unroll
             for (int v = 0; v < 9; ++v) t[v] = t[v] - a.half_dtdx * acc[v];
             cons2prim(t, mh, s);
+#pragma unroll
+            for (int v = 9; v < NV; ++v) mh[v] = p0[v] - a.dtdx * sadv[v - 9];
             if (a.mh_revert && __builtin_expect(mh[0] <= s.smallr || mh[4] <= s.pfl, 0)) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) mh[v] = p0[v];
+                for (int v = 0; v < NV; ++v) mh[v] = p0[v];
             }
         }
 
         // ---- z face q-1/2 and the update of plane q-1 ---------------------------------------
-        T fz[9];
+        T fz[NV];
         if (owned) {
-            T zlo[9], zhi[9];
-            faces_dir<T, PRED, RECON, SLOPE, 2>(pm, p0, pp, mh, up, a, zlo, zhi);
-            if (q >= zs) riemann<T, RIEM, 2>(zprev, zlo, fz, s);
+            T zlo[NV], zhi[NV];
+            faces_dir<T, PRED, RECON, SLOPE, 2, NV>(pm, p0, pp, mh, up, a, zlo, zhi);
+            if (q >= zs) {
+                riemann<T, RIEM, 2>(zprev, zlo, fz, s);
+                species_flux<NV>(zprev, zlo, fz);
+            }
 #pragma unroll
-            for (int v = 0; v < 9; ++v) zprev[v] = zhi[v];
+            for (int v = 0; v < NV; ++v) zprev[v] = zhi[v];
         }
         if (PRED == PRED_FULL && inplane) {
             // park the predicted state in this lane's own +x slot (free until its x face state is
             // stored) across the z solve and the update
 #pragma unroll
-            for (int v = 0; v < 9; ++v) XB[v][lo_own] = mh[v];
+            for (int v = 0; v < NV; ++v) XB[v][lo_own] = mh[v];
         }
         if (do_upd) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v) u0[v] = gload(ca.q[v] + col + plane * wrapi(q - 1, nz));
-            T acc[9];
+            for (int v = 0; v < NV; ++v) u0[v] = gload(ca.q[v] + col + plane * wrapi(q - 1, nz));
+            T acc[NV];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] = pend[v] - fz[v];
+            for (int v = 0; v < NV; ++v) acc[v] = pend[v] - fz[v];
             const int zq = wrapi(q - 1, nz);
-            double ct = update_cell(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
+            double ct = update_cell<NV>(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
             if (!(ct <= cmax)) cmax = ct;
             const size_t off = col + plane * zq;
 #pragma unroll
-            for (int v = 0; v < 9; ++v) gstore_nt(u0[v], ca.o[v] + off);
+            for (int v = 0; v < NV; ++v) gstore_nt(u0[v], ca.o[v] + off);
         }
         if (last) break;
 
 
         // ---- x and y faces of plane q (only for planes this chunk updates) ---------------------
         if (inplane) {
-            T xlo[9], ylo[9];
+            T xlo[NV], ylo[NV];
             __asm__ volatile("" ::: "memory");   // re-read p0 here instead of keeping it live
 #pragma unroll
-            for (int v = 0; v < 9; ++v) p0[v] = P[v][lo_own];
+            for (int v = 0; v < NV; ++v) p0[v] = P[v][lo_own];
             if (PRED == PRED_FULL) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) mh[v] = XB[v][lo_own];
+                for (int v = 0; v < NV; ++v) mh[v] = XB[v][lo_own];
             }
             {
-                T l[9], r[9], hi[9];
+                T l[NV], r[NV], hi[NV];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - 1]; r[v] = P[v][lo_own + 1]; }
-                faces_dir<T, PRED, RECON, SLOPE, 0>(l, p0, r, mh, up, a, xlo, hi);
+                for (int v = 0; v < NV; ++v) { l[v] = P[v][lo_own - 1]; r[v] = P[v][lo_own + 1]; }
+                faces_dir<T, PRED, RECON, SLOPE, 0, NV>(l, p0, r, mh, up, a, xlo, hi);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) XB[v][lo_own] = hi[v];
+                for (int v = 0; v < NV; ++v) XB[v][lo_own] = hi[v];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) { l[v] = P[v][lo_own - PX]; r[v] = P[v][lo_own + PX]; }
-                faces_dir<T, PRED, RECON, SLOPE, 1>(l, p0, r, mh, up, a, ylo, hi);
+                for (int v = 0; v < NV; ++v) { l[v] = P[v][lo_own - PX]; r[v] = P[v][lo_own + PX]; }
+                faces_dir<T, PRED, RECON, SLOPE, 1, NV>(l, p0, r, mh, up, a, ylo, hi);
 #pragma unroll
-                for (int v = 0; v < 9; ++v) YB[v][lo_own] = hi[v];
+                for (int v = 0; v < NV; ++v) YB[v][lo_own] = hi[v];
             }
             __syncthreads();
-            T fx[9], fy[9];
+            T fx[NV], fy[NV];
 #pragma unroll
-            for (int v = 0; v < 9; ++v) { fx[v] = T(0); fy[v] = T(0); }
+            for (int v = 0; v < NV; ++v) { fx[v] = T(0); fy[v] = T(0); }
             if (need_x) {
-                T L[9];
+                T L[NV];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) L[v] = XB[v][lo_own - 1];
+                for (int v = 0; v < NV; ++v) L[v] = XB[v][lo_own - 1];
                 riemann<T, RIEM, 0>(L, xlo, fx, s);
+                species_flux<NV>(L, xlo, fx);
             }
             if (need_y) {
-                T L[9];
+                T L[NV];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) L[v] = YB[v][lo_own - PX];
+                for (int v = 0; v < NV; ++v) L[v] = YB[v][lo_own - PX];
                 riemann<T, RIEM, 1>(L, ylo, fy, s);
+                species_flux<NV>(L, ylo, fy);
             }
             __syncthreads();
 #pragma unroll
-            for (int v = 0; v < 9; ++v) { XB[v][lo_own] = fx[v]; YB[v][lo_own] = fy[v]; }
+            for (int v = 0; v < NV; ++v) { XB[v][lo_own] = fx[v]; YB[v][lo_own] = fy[v]; }
             __syncthreads();
             if (owned) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v)
+                for (int v = 0; v < NV; ++v)
                     pend[v] = ((fx[v] - XB[v][lo_own + 1]) + (fy[v] - YB[v][lo_own + PX])) + fz[v];
             }
         }
 
-        T cn[9], hr[9];
-        load9(ca, col + plane * wrapi(q + 2, nz), cn);
-        if (helper) load9(ca, hcol + plane * wrapi(q + 1, nz), hr);
+        T cn[NV], hr[NV];
+        loadv<NV>(ca, col + plane * wrapi(q + 2, nz), cn);
+        if (helper) loadv<NV>(ca, hcol + plane * wrapi(q + 1, nz), hr);
         // ---- advance the column window and publish plane q+1 to LDS ------------------------------
         // The last reads of plane q's LDS cells precede the face-state barrier when this plane's
         // faces were solved; otherwise (the lead-in plane) they are the predictor's.
         if (!inplane) __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 9; ++v) { pm[v] = P[v][lo_own]; P[v][lo_own] = pp[v]; }
-        cons2prim(cn, pp, s);
+        for (int v = 0; v < NV; ++v) { pm[v] = P[v][lo_own]; P[v][lo_own] = pp[v]; }
+        c2p<NV>(cn, pp, s);
         if (helper) {
-            T h[9];
-            cons2prim(hr, h, s);
+            T h[NV];
+            c2p<NV>(hr, h, s);
 #pragma unroll
-            for (int v = 0; v < 9; ++v) P[v][lo_ring] = h[v];
+            for (int v = 0; v < NV; ++v) P[v][lo_ring] = h[v];
         }
         __syncthreads();
     }
@@ -500,8 +552,8 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T>& L, int tile)
 
 // Workgroup max of the lanes' CFL speeds through LDS (the last wavefront may be partial), then one
 // atomic on the bits of a non-negative double (NaN compares largest).
-template <typename T>
-GLM_DEV void cfl_merge(MarchLDS<T>& L, double cmax, unsigned long long* target)
+template <typename LDS>
+GLM_DEV void cfl_merge(LDS& L, double cmax, unsigned long long* target)
 {
     const int tid = threadIdx.x;
     __syncthreads();
@@ -521,13 +573,14 @@ GLM_DEV void cfl_merge(MarchLDS<T>& L, double cmax, unsigned long long* target)
 }
 
 // One step of one grid: workgroup w marches tile xcd_block(w) (x fastest, then y, then z-chunk).
-template <typename T, int PRED, int RIEM, int RECON, int SLOPE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) march_kernel(MarchArgs<T> a)
+// NS = 0 or 2 passive species after the 9 GLM-MHD planes.
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE, int NS = 0>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU_NS(T, NS)))) march_kernel(MarchArgs<T> a)
 {
-    __shared__ MarchLDS<T> L;
+    __shared__ MarchLDS<T, 9 + NS> L;
     if (threadIdx.x == 0) stage_cold(a, L.ca);
     __syncthreads();
-    const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE>(a, L, xcd_block((int)blockIdx.x, (int)gridDim.x));
+    const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE, NS>(a, L, xcd_block((int)blockIdx.x, (int)gridDim.x));
     if (a.want_cfl) {
         cfl_merge(L, cmax, a.cmax);
         if (blockIdx.x == 0 && threadIdx.x == 0) *a.cmax_next = 0ull;

@@ -35,6 +35,33 @@ const void* march_kernel_ptr_pr<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
     return nullptr;
 }
 
+// The same family with two passive species (11 planes).
+template <>
+hipError_t launch_march_pr_sp<GLM_T, GLM_PRED, GLM_RIEM>(const MarchArgs<GLM_T>& a, int recon, int slope, hipStream_t st)
+{
+    dim3 grid((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), block(NT);
+#define GLM_CASE(RC, SL)                                                                                \
+    if (recon == RC && slope == SL) {                                                                   \
+        hipLaunchKernelGGL((march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL, 2>), grid, block, 0, st, a); \
+        return hipGetLastError();                                                                       \
+    }
+    GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
+    GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
+#undef GLM_CASE
+    return hipErrorInvalidValue;
+}
+
+template <>
+const void* march_kernel_ptr_pr_sp<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
+{
+#define GLM_CASE(RC, SL) \
+    if (recon == RC && slope == SL) return (const void*)&march_kernel<GLM_T, GLM_PRED, GLM_RIEM, RC, SL, 2>;
+    GLM_CASE(RECON_PLM, SLOPE_MONCEN) GLM_CASE(RECON_PPM, SLOPE_MONCEN) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MONCEN)
+    GLM_CASE(RECON_PLM, SLOPE_MINMOD) GLM_CASE(RECON_PPM, SLOPE_MINMOD) GLM_CASE(RECON_PPM_SPIKE, SLOPE_MINMOD)
+#undef GLM_CASE
+    return nullptr;
+}
+
 template <>
 hipError_t launch_ens_pr<GLM_T, GLM_PRED, GLM_RIEM>(const EnsLaunch<GLM_T>& e, int recon, int slope, int grid,
                                                     hipStream_t st)

@@ -222,6 +222,16 @@ class Solver:
             args.zchunk = self.zchunk
         check(self.lib.glm_step(self.ctx, qa, oa, C.byref(args)))
 
+    def step_species(self, uold, unew, args: GLMParams, nsp: int = 2):
+        """step() on (9 + nsp, nz, ny, nx) states whose last nsp planes are the conserved rho*X_s of
+        passive species riding the mass flux (spike_mhd.cu -DNSP, :55-61, :218-220, :278-296)."""
+        qa, kq = self._planes(uold, NV + nsp)
+        oa, ko = self._planes(unew, NV + nsp)
+        if args.zchunk == 0 and self.zchunk:
+            args = GLMParams.from_buffer_copy(args)
+            args.zchunk = self.zchunk
+        check(self.lib.glm_step_species(self.ctx, qa, oa, C.byref(args), int(nsp)))
+
     def tune_zchunk(self, uold, unew, args: GLMParams, reps: int = 3, ct: bool = False, face_slope: int = 0) -> dict:
         """Time the march (ct=True: the CT march) for a set of z-chunkings of this grid and keep the
         fastest for later steps whose args leave zchunk at 0.  Every chunking gives the same bits

@@ -29,12 +29,13 @@ extern "C" {
 /*    GLM cleaning and psi damping, smallr = smallp = 1e-6 (spike_mhd.cu:52, :257-288).    */
 /* ===================================================================================== */
 
-/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library), 8 (CT library, spike_ctm.cu:301), 5
- * (hydro library, spike_25d.cu:560) or 7 (its -DSCALARS build, spike_25d.cu:115-119) */
+/* spike_mhd.cu:321  number of SoA planes: 9 (GLM library), 11 (its -DNSP=2 build, spike_mhd.cu:30-33),
+ * 8 (CT library, spike_ctm.cu:301), 5 (hydro library, spike_25d.cu:560) or 7 (its -DSCALARS build,
+ * spike_25d.cu:115-119) */
 int march_nv(void);
 /* spike_mhd.cu:322  grid size N (compile-time NX in the reference; runtime here, see march_set_grid) */
 int march_nx(void);
-/* spike_mhd.cu:323  passive species count (2 in the 7-plane hydro library, else 0) */
+/* spike_mhd.cu:324  passive species count (2 in the 11-plane GLM and the 7-plane hydro libraries, else 0) */
 int march_nsp(void);
 /* spike_mhd.cu:324  fixed dt/dx of subsequent march_run_dev / ct_run calls */
 void march_set_dtdx(float v);
@@ -113,6 +114,14 @@ int glm_set_stream(glm_ctx* ctx, void* stream);
 /* One step q -> o (step_plm!, glmmhd_turb.jl:878-908), asynchronous on the context stream.
  * q, o: 9 device plane pointers each (distinct buffers).  Returns 0 or an error code. */
 int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
+/* glm_step with nsp passive species (spike_mhd.cu -DNSP, :55-61, :128-188, :218-220, :278-296):
+ * q, o: 9 + nsp device planes, the 9 GLM-MHD planes followed by the conserved rho*X_s.  The
+ * fractions X_s = rho*X_s / max(rho, smallr) are reconstructed and predicted like the primitives
+ * (advected: X - dt/dx u_n dX_n for the one-dimensional predictor, summed over the three
+ * directions for the full one), and each
+ * face carries F_s = F_rho * X_s of the upwind side of F_rho.  The MHD planes come out bit-identical
+ * to glm_step's.  nsp: 0 (same as glm_step) or 2. */
+int glm_step_species(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp);
 /* Max over cells of sum_d(|v_d| + c_f) of the state written by the last glm_step that had
  * want_cfl set (synchronises the stream).  dt = courant*dx/cmax (compute_dt, :1973-1978). */
 int glm_cfl_max(glm_ctx* ctx, double* cmax);

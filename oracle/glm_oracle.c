@@ -533,7 +533,7 @@ static void store9(real* a, size_t n, size_t c, const real* q)
  * Full predictor: glmmhd_turb.jl:378-403 (recon_kernel!), :955-973 (PPM with the spj fallback),
  * :618 (iflo).  Normal-only predictor: spike_mhd.cu:262-275 (edge/hanc1d, PPM variant). */
 static void recon_cell(const real* prim, size_t n, int i, int j, int k, int nx, int ny, int nz,
-                       const o_scheme* sc, real* fm /*3*NV*/, real* fp /*3*NV*/)
+                       const o_scheme* sc, real* fm /*3*NV*/, real* fp /*3*NV*/, int* rec /*10 or NULL*/)
 {
     real gamma = R(sc->gamma), smallr = R(sc->smallr), pfl = R(sc->pfl);
     real dtdx = R(sc->dt) / R(sc->dx);
@@ -551,18 +551,23 @@ static void recon_cell(const real* prim, size_t n, int i, int j, int k, int nx, 
         o_hancock(m0, s[0], s[1], s[2], dtdx, gamma, uh);
         o_cons2prim(uh, q, gamma, smallr, pfl);
         if (sc->mh_revert) O_BR(23, q[0] <= smallr || q[4] <= pfl);
-        if (sc->mh_revert && (q[0] <= smallr || q[4] <= pfl)) { O_HIT(1); memcpy(q, m0, sizeof q); }
+        const int rv = sc->mh_revert && (q[0] <= smallr || q[4] <= pfl);
+        if (rv) { O_HIT(1); memcpy(q, m0, sizeof q); }
         for (int d = 0; d < 3; ++d) memcpy(mh[d], q, sizeof q);
+        if (rec) rec[0] = rec[1] = rec[2] = rv;
     } else {
         for (int d = 0; d < 3; ++d) {
             o_hanc1d(m0, s[d], d, dtdx, gamma, smallr, pfl, mh[d]);
             if (sc->mh_revert) O_BR(23, mh[d][0] <= smallr || mh[d][4] <= pfl);
-            if (sc->mh_revert && (mh[d][0] <= smallr || mh[d][4] <= pfl)) { O_HIT(1); memcpy(mh[d], m0, sizeof m0); }
+            const int rv = sc->mh_revert && (mh[d][0] <= smallr || mh[d][4] <= pfl);
+            if (rv) { O_HIT(1); memcpy(mh[d], m0, sizeof m0); }
+            if (rec) rec[d] = rv;
         }
     }
     int up = 1;
     if (sc->recon == 1)
         up = !(o_spj(nb[0][4], m0[4], nb[1][4]) || o_spj(nb[2][4], m0[4], nb[3][4]) || o_spj(nb[4][4], m0[4], nb[5][4]));
+    if (rec) rec[9] = up;
     for (int d = 0; d < 3; ++d) {
         real* lo = fm + d * NV;
         real* hi = fp + d * NV;
@@ -593,9 +598,14 @@ static void recon_cell(const real* prim, size_t n, int i, int j, int k, int nx, 
         if (sc->face_guard == 1) {
             O_BR(24, lo[0] < smallr || lo[4] < pfl);
             O_BR(24, hi[0] < smallr || hi[4] < pfl);
-            if (lo[0] < smallr || lo[4] < pfl) { O_HIT(2); memcpy(lo, m0, sizeof m0); }
-            if (hi[0] < smallr || hi[4] < pfl) { O_HIT(2); memcpy(hi, m0, sizeof m0); }
-        } else if (sc->face_guard == 2) {
+            const int rl = lo[0] < smallr || lo[4] < pfl, rh = hi[0] < smallr || hi[4] < pfl;
+            if (rl) { O_HIT(2); memcpy(lo, m0, sizeof m0); }
+            if (rh) { O_HIT(2); memcpy(hi, m0, sizeof m0); }
+            if (rec) { rec[3 + d] = rl; rec[6 + d] = rh; }
+        } else if (rec) {
+            rec[3 + d] = rec[6 + d] = 0;
+        }
+        if (sc->face_guard == 2) {
             if (lo[0] < smallr) O_BR(25, 0);
             if (lo[4] < pfl) O_BR(25, 1);
             if (hi[0] < smallr) O_BR(25, 2);
@@ -611,7 +621,7 @@ static void recon_cell(const real* prim, size_t n, int i, int j, int k, int nx, 
 void o_recon_cell_faces(const real* prim, int nx, int ny, int nz, int i, int j, int k, const o_scheme* sc,
                         real* fm, real* fp)
 {
-    recon_cell(prim, (size_t)nx * ny * nz, i, j, k, nx, ny, nz, sc, fm, fp);
+    recon_cell(prim, (size_t)nx * ny * nz, i, j, k, nx, ny, nz, sc, fm, fp, NULL);
 }
 
 /* Fused update epilogue of one cell: glmmhd_turb.jl:440-476 (update_kernel!).
@@ -685,7 +695,7 @@ int o_step(const real* uold, real* unew, int nx, int ny, int nz, const o_scheme*
             for (int i = 0; i < nx; ++i) {
                 size_t c = cidx(i, j, k, nx, ny);
                 real a[3 * NV], b[3 * NV];
-                recon_cell(prim, n, i, j, k, nx, ny, nz, sc, a, b);
+                recon_cell(prim, n, i, j, k, nx, ny, nz, sc, a, b, NULL);
                 for (int w = 0; w < 3 * NV; ++w) { fm[(size_t)w * n + c] = a[w]; fp[(size_t)w * n + c] = b[w]; }
             }
 #pragma omp parallel for schedule(static)
@@ -726,6 +736,138 @@ int o_step(const real* uold, real* unew, int nx, int ny, int nz, const o_scheme*
 }
 
 size_t o_step_scratch_reals(int nx, int ny, int nz) { return (size_t)99 * nx * ny * nz; }
+
+/* ------------------------------------------------------------------------- */
+/* Passive species riding the mass flux (march_bridge/cu/spike_mhd.cu -DNSP, :30-33, :55-61,   */
+/* :128-131, :155-159, :177-188, :218-221, :278-288).                                          */
+/* ------------------------------------------------------------------------- */
+#define O_NSPMAX 2
+
+/* One step of a state with nsp passive species: planes 0..8 the GLM-MHD state, planes 9..8+nsp the
+ * conserved rho*X_s.  The MHD planes are o_step's result (the species do not feed back).  The
+ * species, per cell:
+ *   X = (rho X) / max(rho, smallr), with the cell's primitive density (spike_mhd.cu:243-244);
+ *   half slopes with the scheme's limiter (:128-131, as every primitive);
+ *   predictor: the advective half step X - dt/dx u_d hs_d (:155-159, hs the half slope) in each
+ *     direction for the normal-only predictor, and X - dt/dx (u hs_x + v hs_y + w hs_z) for the
+ *     full-transverse one (the same advection with all three transverse terms); X itself where
+ *     the cell's MHD predictor reverts to m0;
+ *   faces: the scheme's reconstruction of X around that half step (PLM, or PPM and its PLM
+ *     fallback, or the spike PPM edges, :177-188), X itself on a face that the iflo guard reverts;
+ *   flux: F_s = F_rho X_L if the face's mass flux F_rho >= 0, else F_rho X_R (CMA, :218-221);
+ *   update: rho X += dt/dx sum_d (F_lo - F_hi) in o_update_cell's order (:278-288); where the MHD
+ *     update floors the density (rho < smallr, glmmhd_turb.jl:448-451) rho X = smallr X_old.
+ * scratch: o_step_sp_scratch_reals(nx, ny, nz, nsp) reals. */
+size_t o_step_sp_scratch_reals(int nx, int ny, int nz, int nsp)
+{
+    return (size_t)(99 + 10 * nsp) * nx * ny * nz;
+}
+
+int o_step_sp(const real* uold, real* unew, int nx, int ny, int nz, int nsp, const o_scheme* sc,
+              const real* afield, real* scratch)
+{
+    if (nsp < 0 || nsp > O_NSPMAX) return 1;
+    int rc = o_step(uold, unew, nx, ny, nz, sc, afield, scratch);
+    if (rc || nsp == 0) return rc;
+    const size_t n = (size_t)nx * ny * nz;
+    const real smallr = R(sc->smallr), dtdx = R(sc->dt) / R(sc->dx);
+    const real* prim = scratch;                      /* o_step's primitives (9n) */
+    const real* flx = scratch + (size_t)63 * n;      /* o_step's fluxes at the +d face (27n, after prim and fm/fp) */
+    real* X = scratch + (size_t)99 * n;              /* nsp*n fractions */
+    real* FS = X + (size_t)nsp * n;                  /* [d][lo,hi][s] face states: 6*nsp*n */
+    real* FF = FS + (size_t)6 * nsp * n;             /* [d][s] species flux at the +d face: 3*nsp*n */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = cidx(i, j, k, nx, ny);
+                for (int sp = 0; sp < nsp; ++sp) X[(size_t)sp * n + c] = uold[(size_t)(9 + sp) * n + c] / prim[c];
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = cidx(i, j, k, nx, ny);
+                real fm[3 * NV], fp[3 * NV];
+                int rec[10];
+                recon_cell(prim, n, i, j, k, nx, ny, nz, sc, fm, fp, rec);
+                size_t nb[6] = {cidx(wrapi(i - 1, nx), j, k, nx, ny), cidx(wrapi(i + 1, nx), j, k, nx, ny),
+                                cidx(i, wrapi(j - 1, ny), k, nx, ny), cidx(i, wrapi(j + 1, ny), k, nx, ny),
+                                cidx(i, j, wrapi(k - 1, nz), nx, ny), cidx(i, j, wrapi(k + 1, nz), nx, ny)};
+                const real u3[3] = {prim[n + c], prim[2 * n + c], prim[3 * n + c]};
+                for (int sp = 0; sp < nsp; ++sp) {
+                    const real* Xs = X + (size_t)sp * n;
+                    const real x0 = Xs[c];
+                    real hs[3], xh[3];
+                    for (int d = 0; d < 3; ++d) {
+                        real dl = x0 - Xs[nb[2 * d]], dr = Xs[nb[2 * d + 1]] - x0;
+                        hs[d] = R(0.5) * (sc->slope == 1 ? o_minmod(dl, dr) : o_moncen(dl, dr));
+                    }
+                    if (sc->predictor == 0) {
+                        real t = x0 - dtdx * ((u3[0] * hs[0] + u3[1] * hs[1]) + u3[2] * hs[2]);
+                        for (int d = 0; d < 3; ++d) xh[d] = rec[d] ? x0 : t;
+                    } else {
+                        for (int d = 0; d < 3; ++d) xh[d] = rec[d] ? x0 : x0 - (dtdx * u3[d]) * hs[d];
+                    }
+                    for (int d = 0; d < 3; ++d) {
+                        const real xl = Xs[nb[2 * d]], xr = Xs[nb[2 * d + 1]];
+                        real lo, hi;
+                        if (sc->recon == 0 || (sc->recon == 1 && !rec[9])) {
+                            lo = xh[d] - hs[d]; hi = xh[d] + hs[d];
+                        } else if (sc->recon == 1) {
+                            real el, eh;
+                            o_ppm_edges(xl, x0, xr, &el, &eh);
+                            lo = xh[d] + el - x0; hi = xh[d] + eh - x0;
+                        } else {
+                            real el = o_ppm1_spike(xl, x0, xr, R(-1)), eh = o_ppm1_spike(xl, x0, xr, R(1));
+                            lo = el + xh[d] - x0; hi = eh + xh[d] - x0;
+                        }
+                        if (rec[3 + d]) lo = x0;
+                        if (rec[6 + d]) hi = x0;
+                        FS[((size_t)(d * 2 + 0) * nsp + sp) * n + c] = lo;
+                        FS[((size_t)(d * 2 + 1) * nsp + sp) * n + c] = hi;
+                    }
+                }
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = cidx(i, j, k, nx, ny);
+                size_t up[3] = {cidx(wrapi(i + 1, nx), j, k, nx, ny), cidx(i, wrapi(j + 1, ny), k, nx, ny),
+                                cidx(i, j, wrapi(k + 1, nz), nx, ny)};
+                for (int d = 0; d < 3; ++d) {
+                    const real fr = flx[(size_t)(d * NV) * n + c];
+                    for (int sp = 0; sp < nsp; ++sp) {
+                        const real xL = FS[((size_t)(d * 2 + 1) * nsp + sp) * n + c];
+                        const real xR = FS[((size_t)(d * 2 + 0) * nsp + sp) * n + up[d]];
+                        FF[((size_t)d * nsp + sp) * n + c] = fr >= R(0) ? fr * xL : fr * xR;
+                    }
+                }
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = cidx(i, j, k, nx, ny);
+                size_t lo[3] = {cidx(wrapi(i - 1, nx), j, k, nx, ny), cidx(i, wrapi(j - 1, ny), k, nx, ny),
+                                cidx(i, j, wrapi(k - 1, nz), nx, ny)};
+                /* the MHD update's density before its floor (o_update_cell's arithmetic) */
+                real a0 = (flx[lo[0]] - flx[c]) + (flx[(size_t)NV * n + lo[1]] - flx[(size_t)NV * n + c]);
+                a0 = a0 + (flx[(size_t)2 * NV * n + lo[2]] - flx[(size_t)2 * NV * n + c]);
+                const int floored = sc->update_floor && uold[c] + dtdx * a0 < smallr;
+                for (int sp = 0; sp < nsp; ++sp) {
+                    const real* F = FF + (size_t)sp * n;
+                    const size_t st = (size_t)nsp * n;
+                    real a = (F[lo[0]] - F[c]) + (F[st + lo[1]] - F[st + c]);
+                    a = a + (F[2 * st + lo[2]] - F[2 * st + c]);
+                    real r = uold[(size_t)(9 + sp) * n + c] + dtdx * a;
+                    if (floored) r = smallr * X[(size_t)sp * n + c];
+                    unew[(size_t)(9 + sp) * n + c] = r;
+                }
+            }
+    return 0;
+}
 
 /* ------------------------------------------------------------------------- */
 /* CFL, diagnostics, initial conditions                                        */

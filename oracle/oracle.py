@@ -174,6 +174,8 @@ class Oracle:
             "o_trace_reset": (None, []),
             "o_step": (i, [vp, vp, i, i, i, vp, vp, vp]),
             "o_step_scratch_reals": (C.c_size_t, [i, i, i]),
+            "o_step_sp": (i, [vp, vp, i, i, i, i, vp, vp, vp]),
+            "o_step_sp_scratch_reals": (C.c_size_t, [i, i, i, i]),
             "o_ctot_max": (C.c_double, [vp, i, i, i, C.c_double, C.c_double, C.c_double]),
             "o_fast_max": (C.c_double, [vp, i, i, i, C.c_double, C.c_double, C.c_double]),
             "o_diagnostics": (None, [vp, i, i, i, C.c_double, C.c_double, vp]),
@@ -351,6 +353,22 @@ class Oracle:
         cs = sc.to_c()
         af = None if afield is None else self.arr(afield)
         self.lib.o_step(_ptr(u), _ptr(out), nx, ny, nz, C.byref(cs), None if af is None else _ptr(af), _ptr(scratch))
+        return out
+
+    def step_sp(self, u, sc: Scheme, nsp: int, afield=None):
+        """One step of a (9 + nsp)-plane state: the GLM-MHD planes and nsp passive species rho X
+        (o_step_sp)."""
+        nv, nz, ny, nx = u.shape
+        assert nv == 9 + nsp
+        u = self.arr(u)
+        out = self.new_state(nx, ny, nz, nv)
+        scratch = np.empty(self.lib.o_step_sp_scratch_reals(nx, ny, nz, nsp), dtype=self.dtype)
+        cs = sc.to_c()
+        af = None if afield is None else self.arr(afield)
+        rc = self.lib.o_step_sp(_ptr(u), _ptr(out), nx, ny, nz, nsp, C.byref(cs), None if af is None else _ptr(af),
+                                _ptr(scratch))
+        if rc:
+            raise ValueError(f"o_step_sp: nsp must be 0..2, not {nsp}")
         return out
 
     def ctot_max(self, u, gamma, smallr, pfl):

@@ -232,6 +232,21 @@ def test_glm_bridge_library_exports():
     assert lib.march_nv() == 9 and lib.march_nx() == 480 and lib.march_nsp() == 0
 
 
+def test_glm_species_library_exports():
+    """libmhd480_nsp2.so (NSP=2 build_mhd.sh; spike_mhd.cu built with -DNSP=2): the same symbols,
+    march_nv = 11, march_nsp = 2 (spike_mhd.cu:30-33, :321-324)."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", "libmhd480_nsp2.so")
+    assert os.path.exists(path), "build it with NSP=2 glmmhdturb_amd/build_mhd.sh"
+    exported = _exported(path)
+    for sym in ("march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_glm", "march_set_gamma",
+                "march_run_dev", "march_run_dev_f64"):
+        assert sym in exported, sym
+    for sym in ("ct_run", "ct_regs", "ct_shmem"):
+        assert sym not in exported, sym
+    lib = C.CDLL(path)
+    assert lib.march_nv() == 11 and lib.march_nx() == 480 and lib.march_nsp() == 2
+
+
 def test_ct_library_exports_only_the_ct_bridge():
     """libct480.so (build_ct.sh): spike_ctm.cu's symbols (spike_ctm.cu:300-328) with march_nv = 8,
     and neither march_run_dev (it would read a ninth plane from the caller's 8-entry arrays) nor

@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--no-ct", action="store_true")
     ap.add_argument("--no-hydro", action="store_true")
+    ap.add_argument("--no-matrix", action="store_true")
     ap.add_argument("--hydro-N", type=int, default=480)
     ap.add_argument("--ct-N", type=int, default=512)
     ap.add_argument("--ct-steps", type=int, default=200)
@@ -747,6 +748,103 @@ def run_hydro(N=480, iters=30, warmup=3, dtdx=0.05):
     return out
 
 
+# The reference's march matrix: Mcell/s at 480^3 on an RTX A6000, gradient IC, FP32 state with
+# an f16 tile (BENCHMARKS.md:189-195).  Keys: (march, reconstruction, species).
+REF_MATRIX_A6000 = {("glm", "plm", 0): 3175, ("glm", "plm", 2): 2733, ("glm", "ppm", 0): 2064, ("glm", "ppm", 2): 1410,
+                    ("ct", "plm", 0): 1255, ("ct", "plm", 2): 1011, ("ct", "ppm", 0): 752, ("ct", "ppm", 2): 558,
+                    ("hydro", "plm", 0): 6865, ("hydro", "plm", 2): 5082}
+
+
+def run_reference_matrix(N=480, iters=30, warmup=3):
+    """The reference's march matrix on its own protocol (spike_mhd.cu:343-370): N^3 periodic, the
+    gradient IC ph = 0.001 (i mod 911) (spike_mhd.cu:304-317; species rho X = rho (0.2 + 0.3 ph),
+    :313-315), fixed dt/dx 0.02, CH = GLMFAC = 1, 3 untimed and 30 timed steps, FP32 state.  The
+    GLM and CT marches run the MarchBridge scheme (normal-only Hancock, MonCen or the parabolic-edge
+    PPM, HLL, clamped faces; march_args); the hydro march the build_march.sh scheme at dt/dx 0.05
+    (spike_25d.cu:180).  Each configuration takes the fastest z-chunk of a short sweep (the
+    reference reports its best tile per configuration).  Species in the CT march are not built."""
+    import torch
+    from glmmhdturb_amd import Params, Solver, hydro_args, march_args
+    out = {"workload": f"reference march matrix {N}^3, gradient IC, FP32, {warmup} + {iters} steps per configuration",
+           "unit": "Mcell/s", "reference": "RTX A6000, BENCHMARKS.md:189-195"}
+    s = Solver(Params(N=N, gamma=1.4), "f32")
+    st = torch.cuda.current_stream()
+
+    def timed(step, zc):
+        for _ in range(warmup):
+            step(zc)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(iters):
+            step(zc)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / iters
+
+    rows = {}
+    for march, rc, nsp in sorted(REF_MATRIX_A6000):
+        if march == "ct" and nsp:
+            continue
+        if march == "glm":
+            nv = 9 + nsp
+            q, o = s.new_state(nv), s.new_state(nv)
+            s.ic_gradient(q[:9], 1.4)
+        elif march == "ct":
+            nv = 8
+            q, o, g = s.new_state(8), s.new_state(8), s.new_state(9)
+            s.ic_gradient(g, 5.0 / 3.0)
+            q.copy_(g[:8])
+            del g
+        else:
+            nv = 5 + nsp
+            q, o = s.new_state(nv), s.new_state(nv)
+        i = torch.arange(N ** 3, device=s.device, dtype=torch.int64)
+        ph = (0.001 * (i % 911).to(torch.float64)).to(s.dtype)
+        del i
+        if march == "hydro":
+            for v, (c0, c1) in enumerate(((1.0, 0.3), (0.0, 0.1), (0.0, -0.1), (0.0, 0.05), (2.5, 0.5))):
+                q[v].view(-1).copy_(c0 + c1 * ph)
+        for j in range(nsp):
+            q[nv - nsp + j].view(-1).copy_(q[0].view(-1) * (0.2 + 0.3 * ph))
+        del ph
+        recon = "plm" if rc == "plm" else "ppm_spike"
+        state = {"q": q, "o": o}
+
+        def step(zc):
+            if march == "glm":
+                a = march_args(0.02, 1.4, recon=recon, zchunk=zc)
+                if nsp:
+                    s.step_species(state["q"], state["o"], a, nsp)
+                else:
+                    s.step(state["q"], state["o"], a)
+            elif march == "ct":
+                s.ct_step(state["q"], state["o"], march_args(0.02, 5.0 / 3.0, recon=recon, zchunk=zc))
+            else:
+                s.hydro_step(state["q"], state["o"], hydro_args(0.05, 1.4, zchunk=zc), nsp=nsp, cma=bool(nsp))
+            state["q"], state["o"] = state["o"], state["q"]
+
+        best = None
+        for zc in (0, 16, 32, 60, 120):
+            ms = timed(step, zc)
+            if best is None or ms < best[1]:
+                best = (zc, ms)
+        mcs = N ** 3 / (best[1] * 1e-3) / 1e6
+        ref = REF_MATRIX_A6000[(march, rc, nsp)]
+        key = f"{march}_{rc}_{nsp}sp"
+        rows[key] = {"value": mcs, "ms_per_step": best[1], "zchunk": best[0], "reference_a6000": ref,
+                     "ratio_vs_reference": mcs / ref, "bytes_per_cell": 2 * 4 * nv,
+                     "hbm_frac": 2 * 4 * nv * N ** 3 / (best[1] * 1e-3) / HBM_PEAK,
+                     "finite": bool(torch.isfinite(state["q"]).all())}
+        note(f"matrix {key}: {mcs:.0f} Mcell/s (reference A6000 {ref})")
+        del q, o, state
+        torch.cuda.empty_cache()
+    s.close()
+    out["rows"] = rows
+    out["not_built"] = ["ct_plm_2sp", "ct_ppm_2sp", "hydro_ppm_0sp", "hydro_ppm_2sp"]
+    return out
+
+
 def note(msg):
     """A progress line on stderr (the JSON record is the only line on stdout)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -823,6 +921,9 @@ def main():
     if not args.no_hydro:
         note("hydro march next")
         rec["hydro"] = run_hydro(args.hydro_N)
+    if not args.no_matrix:
+        note("reference march matrix next")
+        rec["reference_matrix"] = run_reference_matrix(args.hydro_N)
     if not args.no_adjoint:
         note("adjoint next")
         rec["adjoint"] = run_adjoint(args.adj_N, args.adj_steps)

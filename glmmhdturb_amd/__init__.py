@@ -11,9 +11,9 @@ the host-side mirror of the reference's Julia interfaces:
 """
 from . import _lib  # noqa: F401
 from .turb import (Params, Solver, Forcing, dxof, pfloor, glm_speed, ramp_of, step_args, run_ot,  # noqa: F401
-                   run_turb, SCHEMES, hydro_args)
+                   run_turb, SCHEMES, hydro_args, march_args)
 
 from .ensemble import BoxSpec, Ensemble, random_boxes, single_box_run  # noqa: F401
 
 __all__ = ["Params", "Solver", "Forcing", "dxof", "pfloor", "glm_speed", "ramp_of", "step_args", "run_ot",
-           "run_turb", "SCHEMES", "hydro_args", "BoxSpec", "Ensemble", "random_boxes", "single_box_run"]
+           "run_turb", "SCHEMES", "hydro_args", "march_args", "BoxSpec", "Ensemble", "random_boxes", "single_box_run"]

@@ -148,6 +148,22 @@ def hydro_args(dtdx: float, gamma: float = 1.4, *, smallr: float = 1e-6, smallp:
     return a
 
 
+def march_args(dtdx: float, gamma: float = 1.4, *, ch: float = 1.0, glm_fac: float = 1.0, recon: str = "plm",
+               zchunk: int = 0) -> GLMParams:
+    """Parameters of the line march that the MarchBridge libraries run (spike_mhd.cu:52, :257-288;
+    spike_ctm.cu:58): normal-only Hancock, MonCen, HLL, the GLM pair with CH and psi *= GLMFAC,
+    faces clamped at SMALLR = SMALLP = 1e-6, no predictor revert, no update floor, fixed dt/dx.
+    recon "ppm_spike" is the -DPPM build's parabolic-edge reconstruction (spike_mhd.cu:164-188)."""
+    a = GLMParams()
+    _lib.load().glm_params_default(C.byref(a))
+    a.gamma, a.dt, a.dx, a.smallr, a.pfl, a.ch, a.glm_fac = gamma, dtdx, 1.0, 1e-6, 1e-6, ch, glm_fac
+    a.predictor, a.recon, a.slope, a.riemann = (_lib.PREDICTOR["oned"], _lib.RECON[recon], _lib.SLOPE["moncen"],
+                                                _lib.RIEMANN["hll"])
+    a.mh_revert, a.face_guard, a.update_floor = 0, _lib.GUARD["clamp"], 0
+    a.zchunk, a.want_cfl, a.do_turb = int(zchunk), 0, 0
+    return a
+
+
 def _torch():
     import torch
     return torch

@@ -85,8 +85,20 @@ void o_branch_hits_reset(void) { for (int w = 0; w < O_NHITS; ++w) o_hits[w] = 0
 #define O_NSITES 32
 #define O_NOUT 4
 #ifdef O_BRANCH_TRACE
-static long long o_trace[O_NSITES][O_NOUT];
-#define O_BR(site, k) do { _Pragma("omp atomic") o_trace[site][k]++; } while (0)
+/* One row of counters per OpenMP thread (summed by o_trace_get), so no atomics are needed; threads
+ * past the last private row share it with atomic increments. */
+#define O_TRACE_THREADS 256
+static long long o_trace[O_TRACE_THREADS][O_NSITES][O_NOUT];
+static inline void o_br(int site, int k)
+{
+    const int t = omp_get_thread_num();
+    if (t < O_TRACE_THREADS - 1) {
+        o_trace[t][site][k]++;
+    } else {
+        _Pragma("omp atomic") o_trace[O_TRACE_THREADS - 1][site][k]++;
+    }
+}
+#define O_BR(site, k) o_br((site), (k))
 #else
 #define O_BR(site, k) ((void)0)
 #endif
@@ -94,7 +106,9 @@ void o_trace_get(long long* out)
 {
     for (int a = 0; a < O_NSITES * O_NOUT; ++a) {
 #ifdef O_BRANCH_TRACE
-        out[a] = o_trace[a / O_NOUT][a % O_NOUT];
+        long long s = 0;
+        for (int t = 0; t < O_TRACE_THREADS; ++t) s += o_trace[t][a / O_NOUT][a % O_NOUT];
+        out[a] = s;
 #else
         out[a] = -1;
 #endif

@@ -41,6 +41,17 @@ def test_libmhd_through_march_bridge(orc, orc32):
         uo = orc32.step(uo, sc)
     got = np.stack([x.cpu().numpy() for x in q])
     assert np.all(rel_l1(got, uo) <= 2e-6)
+    # march_args is the same scheme on the context API: the same bits as the bridge library's steps
+    from glmmhdturb_amd import Params, Solver, march_args
+    s = Solver(Params(N=N, gamma=5.0 / 3.0), "f32")
+    a = march_args(float(np.float32(dtdx)), float(np.float32(5.0 / 3.0)), ch=float(np.float32(ch)),
+                   glm_fac=float(np.float32(fac)))
+    u, un = torch.from_numpy(u0).cuda().contiguous(), s.new_state()
+    for _ in range(3):
+        s.step(u, un, a)
+        u, un = un, u
+    assert torch.equal(u, torch.stack(q))
+    s.close()
 
 
 def test_libct_ct_run(orc, orc32):

@@ -27,7 +27,7 @@ EXPORTS = (
     "march_set_grid", "march_run_dev_f64", "ct_run", "ct_regs", "ct_shmem",
     "glm_params_default", "glm_create", "glm_destroy", "glm_set_stream", "glm_step", "glm_step_species", "glm_cfl_max",
     "glm_ctot_max", "glm_fast_max", "glm_diagnostics", "glm_ic", "glm_ou_setup", "glm_ou_advance", "glm_ou_afield",
-    "glm_ou_steps", "glm_ou_copy_afield", "glm_ct_step", "glm_ct_step_passes", "glm_ct_step_tiles", "glm_ct_ctot_max", "glm_ct_divb_max",
+    "glm_ou_steps", "glm_ou_copy_afield", "glm_ct_step", "glm_ct_step_species", "glm_ct_step_passes", "glm_ct_step_tiles", "glm_ct_ctot_max", "glm_ct_divb_max",
     "glm_hydro_step", "glm_hydro_step_scalars", "glm_last_error", "glm_error_string",
     "glm_kernel_info",
     "glm_ens_create", "glm_ens_destroy", "glm_ens_stream", "glm_ens_reset", "glm_ens_step", "glm_ens_graph",
@@ -95,6 +95,7 @@ def load(path: str | None = None):
         "glm_ou_setup": (i, [vp, i, d, d, d, i, d, d, C.c_ulonglong]),
         "glm_ou_advance": (i, [vp]), "glm_ou_afield": (vp, [vp]), "glm_ou_copy_afield": (i, [vp, vp]), "glm_ou_steps": (C.c_longlong, [vp]),
         "glm_ct_step": (i, [vp, pp, pp, C.POINTER(GLMParams), i]),
+        "glm_ct_step_species": (i, [vp, pp, pp, C.POINTER(GLMParams), i, i]),
         "glm_ct_step_passes": (i, [vp, pp, pp, C.POINTER(GLMParams), i]),
         "glm_ct_step_tiles": (i, [vp, pp, pp, C.POINTER(GLMParams), i]),
         "glm_ct_ctot_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),

@@ -524,16 +524,19 @@ int glm_adjoint_force_dot(glm_ctx* ctx, int reset, double* out)
 enum { CT_PASSES = 0, CT_MARCH = 1, CT_TILES = 2 };
 
 static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope,
-                        int mode, const char* who)
+                        int mode, const char* who, int nsp = 0)
 {
     if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, who);
     int e = check_params(p);
     if (e) return e;
     if (p->riemann == GLM_RIEMANN_LLF_BOUND || p->recon == GLM_RECON_PPM)
         return set_err(GLM_ERR_ARG, "glm_ct_step: riemann must be LLF/HLL/HLLD and recon PLM/PPM_SPIKE");
-    for (int v = 0; v < 8; ++v)
+    if (nsp != 0 && (nsp != 2 || mode != CT_MARCH))
+        return set_err(GLM_ERR_ARG, "glm_ct_step_species: nsp must be 0 or 2 (the fused march)");
+    const int nv = 8 + nsp;
+    for (int v = 0; v < nv; ++v)
         if (!q[v] || !o[v]) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers");
-    if (planes_alias(q, 8, o, 8)) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers (input and output alias)");
+    if (planes_alias(q, nv, o, nv)) return set_err(GLM_ERR_ARG, "glm_ct_step: bad plane pointers (input and output alias)");
     if (p->do_turb && !ctx->afield) return set_err(GLM_ERR_STATE, "glm_ct_step: do_turb without glm_ou_setup");
     if (mode == CT_PASSES && !ctx->ct_scratch) {
         if (hipMalloc(&ctx->ct_scratch, ct_scratch_bytes(ctx->n, ctx->prec)) != hipSuccess)
@@ -546,16 +549,16 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
     h.dt = p->dt; h.dx = p->dx; h.smallc = p->smallc; h.turb_min_rho = p->turb_min_rho; h.ramp = p->ramp;
     h.face_slope = ct_face_slope; h.riemann = p->riemann; h.recon = p->recon;
     h.update_floor = p->update_floor; h.do_turb = p->do_turb; h.tgs = ctx->g; h.afield = ctx->afield;
-    h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2; h.zchunk = p->zchunk;
+    h.want_cfl = p->want_cfl; h.cmax = ctx->d_cmax + 2; h.zchunk = p->zchunk; h.nsp = nsp;
     if (p->want_cfl) HIPCHK(launch_clear_u64(ctx->d_cmax + 2, 1, ctx->stream));
     if (mode == CT_TILES) {
         if (ctx->prec == 8) HIPCHK(launch_ct_tiles<double>(q, o, h, ctx->stream));
         else HIPCHK(launch_ct_tiles<float>(q, o, h, ctx->stream));
     } else if (mode == CT_MARCH) {
-        const int key = 1000 + p->riemann;
+        const int key = 1000 + p->riemann + 10 * nsp;
         if (ctx->ct_slots_key != key) {
             int dev = 0, cus = 0, occ = 0;
-            const void* fn = ctx->prec == 8 ? ct_march_ptr<double>(p->riemann) : ct_march_ptr<float>(p->riemann);
+            const void* fn = ctx->prec == 8 ? ct_march_ptr<double>(p->riemann, nsp) : ct_march_ptr<float>(p->riemann, nsp);
             if (!fn) return set_err(GLM_ERR_ARG, "glm_ct_step: unknown Riemann solver");
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -576,6 +579,11 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
 int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
 {
     return ct_step_impl(ctx, q, o, p, ct_face_slope, CT_MARCH, "glm_ct_step: null argument");
+}
+
+int glm_ct_step_species(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope, int nsp)
+{
+    return ct_step_impl(ctx, q, o, p, ct_face_slope, CT_MARCH, "glm_ct_step_species: null argument", nsp);
 }
 
 int glm_ct_step_passes(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope)
@@ -724,7 +732,7 @@ static glm_ctx* g_ctx64 = nullptr;
 
 int march_nv(void) { return GLM_LIB_NV; }
 int march_nx(void) { return g_nx; }
-int march_nsp(void) { return GLM_LIB_NV == 7 || GLM_LIB_NV == 11 ? 2 : 0; }
+int march_nsp(void) { return GLM_LIB_NV == 7 || GLM_LIB_NV == 10 || GLM_LIB_NV == 11 ? 2 : 0; }
 void march_set_dtdx(float v) { g_dtdx = v; }
 #if GLM_LIB_NV == 9 || GLM_LIB_NV == 11
 // Only the GLM-MHD libraries have a cleaning speed: MarchBridge.set_glm decides by this symbol's
@@ -785,6 +793,7 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kin
     int err = GLM_OK;
     for (int it = 0; it < nsteps && !err; ++it) {
         err = kind == 1   ? glm_ct_step(c, q.data(), o.data(), &p, 0)
+              : kind == 5 ? glm_ct_step_species(c, q.data(), o.data(), &p, 0, 2)
               : kind == 2 ? glm_hydro_step(c, q.data(), o.data(), &p)
               : kind == 3 ? glm_hydro_step_scalars(c, q.data(), o.data(), &p, 2, GLM_LIB_CMA)
               : kind == 4 ? glm_step_species(c, q.data(), o.data(), &p, 2)
@@ -814,7 +823,8 @@ extern "C" {
 //   NV 8 (build_ct.sh): the CT march only (spike_ctm.cu:299-329) -- no march_run_dev, which would
 //        read a ninth (psi) plane the caller's 8-entry arrays do not hold;
 //   NV 5 / 7 (build_march.sh [-DSCALARS]): the hydro march (spike_25d.cu:559-576), no CT entry points;
-//   NV 11 (NSP=2 build_mhd.sh): the GLM march with two passive species (spike_mhd.cu -DNSP=2).
+//   NV 11 (NSP=2 build_mhd.sh): the GLM march with two passive species (spike_mhd.cu -DNSP=2);
+//   NV 10 (NSP=2 build_ct.sh): the CT march with two passive species (spike_ctm.cu -DNSP=2).
 #if GLM_LIB_NV == 9 || GLM_LIB_NV == 5 || GLM_LIB_NV == 7 || GLM_LIB_NV == 11
 constexpr int LEGACY_KIND = GLM_LIB_NV == 5 ? 2 : GLM_LIB_NV == 7 ? 3 : GLM_LIB_NV == 11 ? 4 : 0;
 double march_run_dev(float* const* qv, float* const* ov, int nsteps)
@@ -826,11 +836,12 @@ double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps)
     return legacy_run<double>(qv, ov, nsteps, GLM_LIB_NV, LEGACY_KIND);
 }
 #endif
-#if GLM_LIB_NV == 9 || GLM_LIB_NV == 8
+#if GLM_LIB_NV == 9 || GLM_LIB_NV == 8 || GLM_LIB_NV == 10
+// qv/ov: the 8 CT planes, then (NV 10) the 2 species planes rho X_s (spike_ctm.cu:304-311)
 double ct_run(float* const* qv, float* const* ov, float* const* cc, int nsteps)
 {
     (void)cc;
-    return legacy_run<float>(qv, ov, nsteps, 8, 1);
+    return GLM_LIB_NV == 10 ? legacy_run<float>(qv, ov, nsteps, 10, 5) : legacy_run<float>(qv, ov, nsteps, 8, 1);
 }
 int ct_regs(void)
 {

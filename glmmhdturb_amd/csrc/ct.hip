@@ -78,8 +78,15 @@ const void* ct_tile_ptr()
 }
 
 template <typename T>
-const void* ct_march_ptr(int riemann)
+const void* ct_march_ptr(int riemann, int nsp)
 {
+    if (nsp == 2) {
+        if (riemann == RIEM_HLLD) return (const void*)&ct_march_kernel<T, RIEM_HLLD, 2>;
+        if (riemann == RIEM_HLL) return (const void*)&ct_march_kernel<T, RIEM_HLL, 2>;
+        if (riemann == RIEM_LLF) return (const void*)&ct_march_kernel<T, RIEM_LLF, 2>;
+        return nullptr;
+    }
+    if (nsp != 0) return nullptr;
     if (riemann == RIEM_HLLD) return (const void*)&ct_march_kernel<T, RIEM_HLLD>;
     if (riemann == RIEM_HLL) return (const void*)&ct_march_kernel<T, RIEM_HLL>;
     if (riemann == RIEM_LLF) return (const void*)&ct_march_kernel<T, RIEM_LLF>;
@@ -89,10 +96,11 @@ const void* ct_march_ptr(int riemann)
 template <typename T>
 hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, int slots, hipStream_t st)
 {
+    if (h.nsp != 0 && h.nsp != 2) return hipErrorInvalidValue;
     CTMArgs<T> a;
-    for (int v = 0; v < 8; ++v) {
-        a.q[v] = (const T*)q[v];
-        a.o[v] = (T*)o[v];
+    for (int v = 0; v < 10; ++v) {
+        a.q[v] = v < 8 + h.nsp ? (const T*)q[v] : nullptr;
+        a.o[v] = v < 8 + h.nsp ? (T*)o[v] : nullptr;
     }
     a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
     ct_march_tiling(h.nx, h.ny, h.nz, h.zchunk, slots, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
@@ -105,7 +113,12 @@ hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, 
     a.do_turb = h.do_turb; a.tgs = h.tgs;
     a.afield = (const T*)h.afield;
     dim3 g((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), b(CM_NT);
-    if (h.riemann == RIEM_HLLD) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLLD>), g, b, 0, st, a);
+    if (h.nsp == 2) {
+        if (h.riemann == RIEM_HLLD) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLLD, 2>), g, b, 0, st, a);
+        else if (h.riemann == RIEM_HLL) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLL, 2>), g, b, 0, st, a);
+        else if (h.riemann == RIEM_LLF) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_LLF, 2>), g, b, 0, st, a);
+        else return hipErrorInvalidValue;
+    } else if (h.riemann == RIEM_HLLD) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLLD>), g, b, 0, st, a);
     else if (h.riemann == RIEM_HLL) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLL>), g, b, 0, st, a);
     else if (h.riemann == RIEM_LLF) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_LLF>), g, b, 0, st, a);
     else return hipErrorInvalidValue;
@@ -150,7 +163,7 @@ const void* ct_kernel_ptr()
                                           unsigned long long*, hipStream_t);                                    \
     template hipError_t launch_ct_divb<T>(void* const*, int, int, int, double, unsigned long long*, hipStream_t); \
     template const void* ct_kernel_ptr<T>();                                                                    \
-    template const void* ct_march_ptr<T>(int);                                                                  \
+    template const void* ct_march_ptr<T>(int, int);                                                             \
     template hipError_t launch_ct_march<T>(void* const*, void* const*, const CTArgsHost&, int, hipStream_t);   \
     template hipError_t launch_ct_tiles<T>(void* const*, void* const*, const CTArgsHost&, hipStream_t);        \
     template const void* ct_tile_ptr<T>();

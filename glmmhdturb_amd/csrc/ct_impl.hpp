@@ -113,17 +113,23 @@ GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const A& a)
 // step, then PLM (mh -/+ hs) or the spike PPM edges (edge + mh - c), floored.  lo is the state on
 // the cell's lower D face, hi on its upper one; a face flux pairs hi of the lower cell with lo of
 // the upper cell.
-template <typename T, int D, typename A>
-GLM_DEV void ct_cell_states(const T l[8], const T c[8], const T r[8], const A& a, T lo[8], T hi[8])
+//
+// NV = 8 + passive species fractions X_s (spike_ctm.cu -DNSP): their slopes and PPM edges are the
+// primitives', and the normal-only half step advects them, X - (dt/dx / 2) un s with s = 2 hs the
+// slope and un the cell's normal velocity (spike_ctm.cu:178-181).
+template <typename T, int D, int NV = 8, typename A>
+GLM_DEV void ct_cell_states(const T l[NV], const T c[NV], const T r[NV], const A& a, T lo[NV], T hi[NV])
 {
-    T hs[8], mh[8];
+    T hs[NV], mh[NV];
     const T fac = a.face_slope == 1 ? T(0.25) : T(0.5);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) hs[k] = fac * moncen(c[k] - l[k], r[k] - c[k]);
+    for (int k = 0; k < NV; ++k) hs[k] = fac * moncen(c[k] - l[k], r[k] - c[k]);
     ct_hanc1d<T, D>(c, hs, mh, a);
+#pragma unroll
+    for (int k = 8; k < NV; ++k) mh[k] = c[k] - a.half_dtdx * c[1 + D] * (T(2) * hs[k]);
     if (a.recon == RECON_PPM_SPIKE) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < NV; ++k) {
             T el, eh;
             ppm_edges_spike(l[k], c[k], r[k], el, eh);
             hi[k] = eh + mh[k] - c[k];
@@ -131,7 +137,7 @@ GLM_DEV void ct_cell_states(const T l[8], const T c[8], const T r[8], const A& a
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) { hi[k] = mh[k] + hs[k]; lo[k] = mh[k] - hs[k]; }
+        for (int k = 0; k < NV; ++k) { hi[k] = mh[k] + hs[k]; lo[k] = mh[k] - hs[k]; }
     }
     hi[0] = tmax(hi[0], a.smallr); hi[4] = tmax(hi[4], a.smallp);
     lo[0] = tmax(lo[0], a.smallr); lo[4] = tmax(lo[4], a.smallp);
@@ -140,7 +146,7 @@ GLM_DEV void ct_cell_states(const T l[8], const T c[8], const T r[8], const A& a
 // Flux through a D face from the floored face states Lq (lower cell's hi) and Rq (upper cell's
 // lo) with the face-normal field bn (ct_mhd.jl:114-122, spike_ctm.cu:235-245): F[Bn] = 0.
 template <typename T, int RIEM, int D, typename A>
-GLM_DEV void ct_riemann(const T Lq[8], const T Rq[8], T bn, const A& a, T f[8])
+GLM_DEV void ct_riemann8(const T Lq[8], const T Rq[8], T bn, const A& a, T f[8])
 {
     T L[8], R[8];
 #pragma unroll
@@ -178,6 +184,16 @@ GLM_DEV void ct_riemann(const T Lq[8], const T Rq[8], T bn, const A& a, T f[8])
 #pragma unroll
         for (int k = 0; k < 8; ++k) f[k] = (SR * FL[k] - SL * FR[k] + SL * SR * (UR[k] - UL[k])) * ih;
     }
+}
+
+// ct_riemann8 plus the species fluxes of an NV-value state: F_s = F_rho X_s of the upwind face
+// state (the consistent multi-fluid advection of spike_ctm.cu:240-243).
+template <typename T, int RIEM, int D, int NV = 8, typename A>
+GLM_DEV void ct_riemann(const T Lq[NV], const T Rq[NV], T bn, const A& a, T f[NV])
+{
+    ct_riemann8<T, RIEM, D>(Lq, Rq, bn, a, f);
+#pragma unroll
+    for (int k = 8; k < NV; ++k) f[k] = f[0] * (f[0] >= T(0) ? Lq[k] : Rq[k]);
 }
 
 // Flux at the lower-D face from the stencil m2, m1, c0, p1 (ct_mhd.jl:99-122 / spike_ctm.cu:223-245).

@@ -10,6 +10,7 @@ struct CTArgsHost {
     double gamma, smallr, smallp, dt, dx, smallc, turb_min_rho, ramp;
     int face_slope, riemann, recon, update_floor, do_turb, tgs, want_cfl;
     int zchunk;                    // z planes per march tile (0: the occupancy model)
+    int nsp;                       // passive species planes after the 8 CT planes (0 or 2; fused march only)
     const void* afield;
     unsigned long long* cmax;
 };
@@ -29,7 +30,7 @@ const void* ct_kernel_ptr();
 template <typename T>
 hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, int slots, hipStream_t st);
 template <typename T>
-const void* ct_march_ptr(int riemann);
+const void* ct_march_ptr(int riemann, int nsp = 0);
 // The same step as one kernel over staged 3-D tiles (ct_tile.hpp) and that kernel (resource queries).
 template <typename T>
 hipError_t launch_ct_tiles(void* const* q, void* const* o, const CTArgsHost& h, hipStream_t st);

@@ -49,11 +49,13 @@ static_assert(CM_NT == CT_MARCH_THREADS, "launch width of the CT march");
 #define GLM_CT_WAVES_F32 3
 #endif
 #define GLM_CT_WAVES_PER_EU(T) (sizeof(T) == 8 ? GLM_CT_WAVES_F64 : GLM_CT_WAVES_F32)
+// With species the FP64 LDS (93.5 KB) admits one workgroup per CU, which may then use every register.
+#define GLM_CT_WAVES_PER_EU_NS(T, NS) (sizeof(T) == 8 && (NS) > 0 ? 1 : GLM_CT_WAVES_PER_EU(T))
 
 template <typename T>
 struct CTMArgs {
-    const T* q[8];
-    T* o[8];
+    const T* q[10];        // the 8 CT planes, then up to 2 species rho X_s
+    T* o[10];
     int nx, ny, nz;
     int tiles_x, tiles_y, nzc, zchunk;
     T gamma, igm1, smallr, smallp, dtdx, half_dtdx, dt, dx, lx, ly, lz, smallc, turb_min_rho, ramp;
@@ -83,24 +85,25 @@ GLM_DEV void cm_helper_cell(int h, int& px, int& py)
 
 // The exchange buffers XB/YB hold one slot per lane (the tile grown by one cell, row CM_LX); the
 // primitive plane and the EMF ring cover the tile grown by two cells (row CM_PX).
-template <typename T>
+template <typename T, int NV = 8>
 struct CTMarchLDS {
-    T P[8][CM_NPL];        // primitives of plane q (two-cell ring)
-    T XB[8][CM_NT];        // +x face states, then x fluxes (0..4, By in 6) and the z flux's Bx (5), By (7)
-    T YB[8][CM_NT];        // +y face states, then y fluxes (0..4, Bx in 5)
+    T P[NV][CM_NPL];       // primitives (and species fractions) of plane q (two-cell ring)
+    T XB[NV][CM_NT];       // +x face states, then x fluxes (0..4 and species 8.., By in 6) and the z flux's Bx (5), By (7)
+    T YB[NV][CM_NT];       // +y face states, then y fluxes (0..4 and species 8.., Bx in 5)
     T E[2][3][CM_NPL];     // edge EMFs ex, ey, ez of the last two planes (plane k in slot k & 1)
     T FCB[6][CM_NT];       // forcing interpolation cache (ForceCache)
 };
 
-// Primitives with cell-centred B of cell (x, y, z) (ct_mhd.jl:26-34, :85-96).
-template <typename T>
-GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[8])
+// Primitives with cell-centred B of cell (x, y, z) (ct_mhd.jl:26-34, :85-96), then the species
+// fractions X_s = rho X_s / max(rho, smallr) (spike_ctm.cu:260-263).
+template <typename T, int NV = 8>
+GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[NV])
 {
     const size_t pl = (size_t)a.nx * a.ny;
     const size_t c = (size_t)x + (size_t)a.nx * y + pl * z;
-    T u[8];
+    T u[NV];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) u[v] = gload(a.q[v] + c);
+    for (int v = 0; v < NV; ++v) u[v] = gload(a.q[v] + c);
     const T bx1 = gload(a.q[5] + ((size_t)cwrap(x + 1, a.nx) + (size_t)a.nx * y + pl * z));
     const T by1 = gload(a.q[6] + ((size_t)x + (size_t)a.nx * cwrap(y + 1, a.ny) + pl * z));
     const T bz1 = gload(a.q[7] + ((size_t)x + (size_t)a.nx * y + pl * cwrap(z + 1, a.nz)));
@@ -109,6 +112,8 @@ GLM_DEV void cm_prim(const CTMArgs<T>& a, int x, int y, int z, T p[8])
     T vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
     T pr = ct_pressure(u[4], rho, vx, vy, vz, bx, by, bz, a.gamma, a.smallp);
     p[0] = rho; p[1] = vx; p[2] = vy; p[3] = vz; p[4] = pr; p[5] = bx; p[6] = by; p[7] = bz;
+#pragma unroll
+    for (int v = 8; v < NV; ++v) p[v] = u[v] * ir;
 }
 
 // Tiles and z-chunks of the CT march on `slots` concurrent workgroups: rounds x (chunk + 2 lead-in
@@ -138,15 +143,17 @@ inline void ct_march_tiling(int nx, int ny, int nz, int zchunk_req, int slots, i
     nzc = (nz + zchunk - 1) / zchunk;
 }
 
-// One step of the CT state: workgroup w marches tile xcd_block(w).
-template <typename T, int RIEM>
-__global__ void __launch_bounds__(CM_NT) __attribute__((amdgpu_waves_per_eu(GLM_CT_WAVES_PER_EU(T))))
+// One step of the CT state with NS passive species (spike_ctm.cu -DNSP: their fluxes ride the
+// mass flux and are updated with the hydro planes): workgroup w marches tile xcd_block(w).
+template <typename T, int RIEM, int NS = 0>
+__global__ void __launch_bounds__(CM_NT) __attribute__((amdgpu_waves_per_eu(GLM_CT_WAVES_PER_EU_NS(T, NS))))
 ct_march_kernel(CTMArgs<T> a)
 {
-    __shared__ CTMarchLDS<T> L;
-    T (&P)[8][CM_NPL] = L.P;
-    T (&XB)[8][CM_NT] = L.XB;
-    T (&YB)[8][CM_NT] = L.YB;
+    constexpr int NV = 8 + NS, NH = 5 + NS;   // values per cell; hydro-updated values (species at 5..)
+    __shared__ CTMarchLDS<T, NV> L;
+    T (&P)[NV][CM_NPL] = L.P;
+    T (&XB)[NV][CM_NT] = L.XB;
+    T (&YB)[NV][CM_NT] = L.YB;
     const int tid = threadIdx.x;
     int tile = xcd_block((int)blockIdx.x, (int)gridDim.x);
     int tx, ty;
@@ -177,31 +184,31 @@ ct_march_kernel(CTMArgs<T> a)
     const int lo_ring = hy * CM_PX + hx;
 
     // column window: primitives of planes q-1 and q+1 (plane q is read back from LDS)
-    T pm[8], pp[8];
+    T pm[NV], pp[NV];
     {
-        T p0[8];
-        cm_prim(a, gx, gy, cwrap(zs - 2, nz), pm);
-        cm_prim(a, gx, gy, cwrap(zs - 1, nz), p0);
-        cm_prim(a, gx, gy, cwrap(zs, nz), pp);
+        T p0[NV];
+        cm_prim<T, NV>(a, gx, gy, cwrap(zs - 2, nz), pm);
+        cm_prim<T, NV>(a, gx, gy, cwrap(zs - 1, nz), p0);
+        cm_prim<T, NV>(a, gx, gy, cwrap(zs, nz), pp);
 #pragma unroll
-        for (int v = 0; v < 8; ++v) P[v][lo] = p0[v];
+        for (int v = 0; v < NV; ++v) P[v][lo] = p0[v];
         if (helper) {
-            T h[8];
-            cm_prim(a, hgx, hgy, cwrap(zs - 1, nz), h);
+            T h[NV];
+            cm_prim<T, NV>(a, hgx, hgy, cwrap(zs - 1, nz), h);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) P[v][lo_ring] = h[v];
+            for (int v = 0; v < NV; ++v) P[v][lo_ring] = h[v];
         }
     }
     __syncthreads();
 
-    T zprev[8];                 // +z face state of plane q-1
-    T fzp[5];                   // hydro part of the z flux below plane q-1
-    T pend[5];                  // (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) of plane q-1
+    T zprev[NV];                // +z face state of plane q-1
+    T fzp[NH];                  // hydro (and species) part of the z flux below plane q-1
+    T pend[NH];                 // (Fx_lo - Fx_hi) + (Fy_lo - Fy_hi) of plane q-1
     T fxbz = T(0), fybz = T(0); // Fx[Bz], Fy[Bz] of this cell in plane q-1
 #pragma unroll
-    for (int v = 0; v < 8; ++v) zprev[v] = T(0);
+    for (int v = 0; v < NV; ++v) zprev[v] = T(0);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { fzp[v] = T(0); pend[v] = T(0); }
+    for (int v = 0; v < NH; ++v) { fzp[v] = T(0); pend[v] = T(0); }
     ForceCache<T, CM_NT> fc;
     fc.k0 = -0x7fffffff;
     fc.b = &L.FCB[0][tid];
@@ -215,59 +222,61 @@ ct_march_kernel(CTMArgs<T> a)
         const T bnx = gload(a.q[5] + off), bny = gload(a.q[6] + off), bnz = gload(a.q[7] + off);
 
         // ---- x and y face states of this lane's cell in plane q -----------------------------------
-        T xlo[8], ylo[8];
+        T xlo[NV], ylo[NV];
         {
-            T l[8], c0[8], r[8], hi[8];
+            T l[NV], c0[NV], r[NV], hi[NV];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) { l[v] = P[v][lo - 1]; c0[v] = P[v][lo]; r[v] = P[v][lo + 1]; }
-            ct_cell_states<T, 0>(l, c0, r, a, xlo, hi);
+            for (int v = 0; v < NV; ++v) { l[v] = P[v][lo - 1]; c0[v] = P[v][lo]; r[v] = P[v][lo + 1]; }
+            ct_cell_states<T, 0, NV>(l, c0, r, a, xlo, hi);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) XB[v][li] = hi[v];
+            for (int v = 0; v < NV; ++v) XB[v][li] = hi[v];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) { l[v] = P[v][lo - CM_PX]; r[v] = P[v][lo + CM_PX]; }
-            ct_cell_states<T, 1>(l, c0, r, a, ylo, hi);
+            for (int v = 0; v < NV; ++v) { l[v] = P[v][lo - CM_PX]; r[v] = P[v][lo + CM_PX]; }
+            ct_cell_states<T, 1, NV>(l, c0, r, a, ylo, hi);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) YB[v][li] = hi[v];
+            for (int v = 0; v < NV; ++v) YB[v][li] = hi[v];
         }
         __syncthreads();
 
         // ---- x and y faces ------------------------------------------------------------------------
-        T fx[8], fy[8];
+        T fx[NV], fy[NV];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) { fx[v] = T(0); fy[v] = T(0); }
+        for (int v = 0; v < NV; ++v) { fx[v] = T(0); fy[v] = T(0); }
         if (need_x) {
-            T Lx[8];
+            T Lx[NV];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) Lx[v] = XB[v][li - 1];
-            ct_riemann<T, RIEM, 0>(Lx, xlo, bnx, a, fx);
+            for (int v = 0; v < NV; ++v) Lx[v] = XB[v][li - 1];
+            ct_riemann<T, RIEM, 0, NV>(Lx, xlo, bnx, a, fx);
         }
         if (need_y) {
-            T Ly[8];
+            T Ly[NV];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) Ly[v] = YB[v][li - CM_LX];
-            ct_riemann<T, RIEM, 1>(Ly, ylo, bny, a, fy);
+            for (int v = 0; v < NV; ++v) Ly[v] = YB[v][li - CM_LX];
+            ct_riemann<T, RIEM, 1, NV>(Ly, ylo, bny, a, fy);
         }
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < 5; ++v) { XB[v][li] = fx[v]; YB[v][li] = fy[v]; }
+#pragma unroll
+        for (int v = 8; v < NV; ++v) { XB[v][li] = fx[v]; YB[v][li] = fy[v]; }
         XB[6][li] = fx[6];
         YB[5][li] = fy[5];
         const T fx7 = fx[7], fy7 = fy[7];
         __asm__ volatile("" ::: "memory");   // the other flux components are read back from LDS below
 
         // ---- z face q-1/2 (the plane's LDS cells are untouched until the publish) -----------------
-        T fz[8];
+        T fz[NV];
 #pragma unroll
-        for (int v = 0; v < 8; ++v) fz[v] = T(0);
+        for (int v = 0; v < NV; ++v) fz[v] = T(0);
         {
-            T c0[8];
+            T c0[NV];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) c0[v] = P[v][lo];
-            T zlo[8], zhi[8];
-            ct_cell_states<T, 2>(pm, c0, pp, a, zlo, zhi);
-            if (need_z && !lead) ct_riemann<T, RIEM, 2>(zprev, zlo, bnz, a, fz);
+            for (int v = 0; v < NV; ++v) c0[v] = P[v][lo];
+            T zlo[NV], zhi[NV];
+            ct_cell_states<T, 2, NV>(pm, c0, pp, a, zlo, zhi);
+            if (need_z && !lead) ct_riemann<T, RIEM, 2, NV>(zprev, zlo, bnz, a, fz);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) zprev[v] = zhi[v];
+            for (int v = 0; v < NV; ++v) zprev[v] = zhi[v];
         }
         if (!lead) { XB[5][li] = fz[5]; XB[7][li] = fz[6]; }
         __syncthreads();
@@ -287,9 +296,11 @@ ct_march_kernel(CTMArgs<T> a)
             if (owned && q >= zs + 1) {
                 const int zu = cwrap(q - 1, nz);
                 const size_t ou = col + plane * zu;
-                T r[5];
+                T r[NH], b0[NH];   // hydro values, then species (plane 8 + s at index 5 + s)
 #pragma unroll
-                for (int t = 0; t < 5; ++t) r[t] = gload(a.q[t] + ou) + a.dtdx * (pend[t] + (fzp[t] - fz[t]));
+                for (int t = 0; t < NH; ++t) b0[t] = gload(a.q[t < 5 ? t : t + 3] + ou);
+#pragma unroll
+                for (int t = 0; t < NH; ++t) r[t] = b0[t] + a.dtdx * (pend[t] + (fzp[t] - fz[t < 5 ? t : t + 3]));
                 const T ex0 = Ep[0][lo], ey0 = Ep[1][lo], ez0 = Ep[2][lo];
                 const T bxn = gload(a.q[5] + ou) - a.dtdx * ((Ep[2][lo + CM_PX] - ez0) - (ey - ey0));
                 const T byn = gload(a.q[6] + ou) - a.dtdx * ((ex - ex0) - (Ep[2][lo + 1] - ez0));
@@ -297,6 +308,9 @@ ct_march_kernel(CTMArgs<T> a)
                 if (a.update_floor && r[0] < a.smallr) {
                     T emag = T(0.5) * (pm[5] * pm[5] + pm[6] * pm[6] + pm[7] * pm[7]);
                     r[0] = a.smallr; r[1] = T(0); r[2] = T(0); r[3] = T(0); r[4] = a.smallp * a.igm1 + emag;
+                    // the species keep their fraction: rho X = smallr X_old
+#pragma unroll
+                    for (int t = 5; t < NH; ++t) r[t] = a.smallr * (b0[t] * trcp(tmax(b0[0], a.smallr)));
                 }
                 if (a.do_turb && r[0] >= a.turb_min_rho) {
                     T acc[3];
@@ -313,6 +327,8 @@ ct_march_kernel(CTMArgs<T> a)
                 }
 #pragma unroll
                 for (int t = 0; t < 5; ++t) gstore_nt(r[t], a.o[t] + ou);
+#pragma unroll
+                for (int t = 5; t < NH; ++t) gstore_nt(r[t], a.o[t + 3] + ou);
                 gstore_nt(bxn, a.o[5] + ou);
                 gstore_nt(byn, a.o[6] + ou);
                 gstore_nt(bzn, a.o[7] + ou);
@@ -320,25 +336,28 @@ ct_march_kernel(CTMArgs<T> a)
             // in-plane flux differences of plane q for its update in the next pass
             if (owned) {
 #pragma unroll
-                for (int t = 0; t < 5; ++t) pend[t] = (XB[t][li] - XB[t][li + 1]) + (YB[t][li] - YB[t][li + CM_LX]);
+                for (int t = 0; t < NH; ++t) {
+                    const int v = t < 5 ? t : t + 3;
+                    pend[t] = (XB[v][li] - XB[v][li + 1]) + (YB[v][li] - YB[v][li + CM_LX]);
+                }
             }
 #pragma unroll
-            for (int t = 0; t < 5; ++t) fzp[t] = fz[t];
+            for (int t = 0; t < NH; ++t) fzp[t] = fz[t < 5 ? t : t + 3];
         }
         fxbz = fx7;
         fybz = fy7;
         if (q == ze) break;
 
         // ---- prefetch and publish plane q+1 -------------------------------------------------------
-        T pn[8], hn[8];
-        cm_prim(a, gx, gy, cwrap(q + 2, nz), pn);
-        if (helper) cm_prim(a, hgx, hgy, cwrap(q + 1, nz), hn);
+        T pn[NV], hn[NV];
+        cm_prim<T, NV>(a, gx, gy, cwrap(q + 2, nz), pn);
+        if (helper) cm_prim<T, NV>(a, hgx, hgy, cwrap(q + 1, nz), hn);
         __syncthreads();   // every read of plane q's LDS cells and of the flux exchange is done
 #pragma unroll
-        for (int v = 0; v < 8; ++v) { pm[v] = P[v][lo]; P[v][lo] = pp[v]; pp[v] = pn[v]; }
+        for (int v = 0; v < NV; ++v) { pm[v] = P[v][lo]; P[v][lo] = pp[v]; pp[v] = pn[v]; }
         if (helper) {
 #pragma unroll
-            for (int v = 0; v < 8; ++v) P[v][lo_ring] = hn[v];
+            for (int v = 0; v < NV; ++v) P[v][lo_ring] = hn[v];
         }
         __syncthreads();
     }

@@ -410,6 +410,16 @@ class Solver:
             args.zchunk = self.ct_zchunk
         check(fns[path](self.ctx, qa, oa, C.byref(args), int(face_slope)))
 
+    def ct_step_species(self, qold, qnew, args: GLMParams, nsp: int = 2, face_slope: int = 0):
+        """ct_step (fused march) on (8 + nsp, nz, ny, nx) states whose last nsp planes are the
+        conserved rho*X_s of passive species riding the mass flux (spike_ctm.cu -DNSP)."""
+        qa, kq = self._planes(qold, 8 + nsp)
+        oa, ko = self._planes(qnew, 8 + nsp)
+        if args.zchunk == 0 and self.ct_zchunk:
+            args = GLMParams.from_buffer_copy(args)
+            args.zchunk = self.ct_zchunk
+        check(self.lib.glm_ct_step_species(self.ctx, qa, oa, C.byref(args), int(face_slope), int(nsp)))
+
     def hydro_step(self, qold, qnew, args: GLMParams, nsp: int = 0, cma: bool = False):
         """One step of the hydro march (spike_25d.cu march, build_march.sh scheme) on
         (5 + nsp, nz, ny, nx) conserved states [rho, mx, my, mz, E, rho*s0, rho*s1]; nsp = 2 adds

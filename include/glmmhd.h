@@ -30,12 +30,13 @@ extern "C" {
 /* ===================================================================================== */
 
 /* spike_mhd.cu:321  number of SoA planes: 9 (GLM library), 11 (its -DNSP=2 build, spike_mhd.cu:30-33),
- * 8 (CT library, spike_ctm.cu:301), 5 (hydro library, spike_25d.cu:560) or 7 (its -DSCALARS build,
- * spike_25d.cu:115-119) */
+ * 8 (CT library, spike_ctm.cu:301), 10 (its -DNSP=2 build), 5 (hydro library, spike_25d.cu:560) or 7
+ * (its -DSCALARS build, spike_25d.cu:115-119) */
 int march_nv(void);
 /* spike_mhd.cu:322  grid size N (compile-time NX in the reference; runtime here, see march_set_grid) */
 int march_nx(void);
-/* spike_mhd.cu:324  passive species count (2 in the 11-plane GLM and the 7-plane hydro libraries, else 0) */
+/* spike_mhd.cu:324  passive species count (2 in the 11-plane GLM, the 10-plane CT and the 7-plane hydro
+ * libraries, else 0) */
 int march_nsp(void);
 /* spike_mhd.cu:324  fixed dt/dx of subsequent march_run_dev / ct_run calls */
 void march_set_dtdx(float v);
@@ -57,8 +58,9 @@ int march_set_grid(int nx, int ny, int nz);
 /* FP64 twin of march_run_dev (the FP64 state path; no reference counterpart). */
 double march_run_dev_f64(double* const* qv, double* const* ov, int nsteps);
 
-/* spike_ctm.cu:305-325 CT march: qv/ov = 8 planes [U0..U4, bxf, byf, bzf]; cc = 3 Bcc scratch
- * planes (accepted for ABI compatibility; the cell-centred B is formed in-kernel).  Uses the
+/* spike_ctm.cu:305-325 CT march: qv/ov = 8 planes [U0..U4, bxf, byf, bzf], then in the 10-plane
+ * library the 2 species planes rho X_s (spike_ctm.cu:304, :310-311); cc = 3 Bcc scratch planes
+ * (accepted for ABI compatibility; the cell-centred B is formed in-kernel).  Uses the
  * march_set_dtdx / march_set_gamma globals.  Returns elapsed GPU time in ms. */
 double ct_run(float* const* qv, float* const* ov, float* const* cc, int nsteps);
 /* spike_ctm.cu:326  registers of the CT march kernel (gfx950 arch VGPRs + AGPRs) */
@@ -175,6 +177,14 @@ long long glm_ou_steps(glm_ctx* ctx);
  * or HLLD or LLF), recon (PLM or PPM_SPIKE), update_floor, do_turb, turb_min_rho, smallc, ramp,
  * want_cfl from p; ct_face_slope selects the spike_ctm.cu (0) or the ct_mhd.jl (1) face offset. */
 int glm_ct_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope);
+/* glm_ct_step with nsp passive species (spike_ctm.cu -DNSP, :69-70, :80-119, :178-181, :240-243,
+ * :289-292): q, o = 8 + nsp planes, the CT planes followed by the conserved rho*X_s.  The fractions
+ * X_s = rho*X_s / max(rho, smallr) get the primitives' slopes and PPM edges, the normal-only half
+ * step advects them (X - dt/dx/2 u_n s), and each face carries F_s = F_rho * X_s of the upwind side
+ * of F_rho.  Where the update floor fires, rho*X_s = smallr * X_s(old).  The 8 CT planes come out
+ * bit-identical to glm_ct_step's.  nsp: 0 (same as glm_ct_step) or 2. */
+int glm_ct_step_species(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int ct_face_slope,
+                        int nsp);
 /* The same CT step as the kernel sequence of ct_mhd.jl step! (:174-184): cell-centred B and
  * primitives, face fluxes, edge EMFs, update, each its own pass over a 35-plane scratch.  Same
  * arguments and results (to rounding) as glm_ct_step. */

@@ -305,6 +305,91 @@ int o_ct_step(const real* qin, real* qout, int nx, int ny, int nz, const o_ct_sc
 
 size_t o_ct_step_scratch_reals(int nx, int ny, int nz) { return (size_t)38 * nx * ny * nz; }
 
+/* Passive species in the CT step (spike_ctm.cu -DNSP: fractions in the tile :217-219, :260-263;
+ * slopes and PPM edges :88-90, :108-119; hanc1d's advective predictor X - h un dX (h = dt/dx / 2,
+ * un the cell's normal velocity) :178-181; the CMA flux F_s = F_rho X of the upwind face state
+ * :240-243; the update :289-292).  Species do not feed back on the MHD planes, so their face flux
+ * needs only the fractions, the two cells' normal velocities and the MHD mass flux of the face. */
+static real ct_species_flux(real m2x, real m1x, real c0x, real p1x, real unL, real unR, real f0,
+                            const o_ct_scheme* sc)
+{
+    real dtdx = R(sc->dt) / R(sc->dx);
+    real sL = o_moncen(m1x - m2x, c0x - m1x), sR = o_moncen(c0x - m1x, p1x - c0x);
+    if (sc->face_slope == 1) { sL = R(0.5) * sL; sR = R(0.5) * sR; }
+    real mhL = m1x - R(0.5) * dtdx * unL * sL, mhR = c0x - R(0.5) * dtdx * unR * sR;
+    real L, Rr;
+    if (sc->recon == 2) {
+        L = o_ppm1_spike(m2x, m1x, c0x, R(1)) + mhL - m1x;
+        Rr = o_ppm1_spike(m1x, c0x, p1x, R(-1)) + mhR - c0x;
+    } else {
+        L = mhL + R(0.5) * sL;
+        Rr = mhR - R(0.5) * sR;
+    }
+    return f0 >= R(0) ? f0 * L : f0 * Rr;
+}
+
+/* One CT step of a (8 + nsp)-plane state: o_ct_step on the 8 MHD planes, then the nsp species
+ * planes rho X_s with the mass fluxes that step computed.  Where the update floors the density the
+ * species keep their fraction: rho X = smallr X_old, X_old = rho X / max(rho, smallr) before the step
+ * (the floor of the GLM march's species).  scratch: o_ct_step_sp_scratch_reals. */
+int o_ct_step_sp(const real* qin, real* qout, int nx, int ny, int nz, int nsp, const o_ct_scheme* sc,
+                 const real* afield, real* scratch)
+{
+    if (nsp < 0 || nsp > 2) return 1;
+    int e = o_ct_step(qin, qout, nx, ny, nz, sc, afield, scratch);
+    if (e || nsp == 0) return e;
+    size_t n = (size_t)nx * ny * nz;
+    real smallr = R(sc->smallr), dtdx = R(sc->dt) / R(sc->dx);
+    const real* prim = scratch + 3 * n;      /* o_ct_step's primitives: v_d in planes 1..3 */
+    const real* F = scratch + 11 * n;         /* its fluxes: F[(d*8 + t) n + c] at the lower-d face */
+    real* X = scratch + 38 * n;               /* nsp n fractions */
+    real* FS = X + (size_t)nsp * n;           /* 3 nsp n species fluxes: FS[(d*nsp + s) n + c] */
+    for (int sp = 0; sp < nsp; ++sp)
+#pragma omp parallel for schedule(static)
+        for (size_t c = 0; c < n; ++c) X[(size_t)sp * n + c] = qin[(size_t)(8 + sp) * n + c] / fmax(qin[c], smallr);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = cidx(i, j, k, nx, ny);
+                for (int d = 0; d < 3; ++d) {
+                    int o1 = d == 0, o2 = d == 1, o3 = d == 2;
+                    size_t cm2 = cidx(wrapi(i - 2 * o1, nx), wrapi(j - 2 * o2, ny), wrapi(k - 2 * o3, nz), nx, ny);
+                    size_t cm1 = cidx(wrapi(i - o1, nx), wrapi(j - o2, ny), wrapi(k - o3, nz), nx, ny);
+                    size_t cp1 = cidx(wrapi(i + o1, nx), wrapi(j + o2, ny), wrapi(k + o3, nz), nx, ny);
+                    real unL = prim[(size_t)(1 + d) * n + cm1], unR = prim[(size_t)(1 + d) * n + c];
+                    real f0 = F[(size_t)(d * 8) * n + c];
+                    for (int sp = 0; sp < nsp; ++sp) {
+                        const real* x = X + (size_t)sp * n;
+                        FS[(size_t)(d * nsp + sp) * n + c] = ct_species_flux(x[cm2], x[cm1], x[c], x[cp1], unL, unR, f0, sc);
+                    }
+                }
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t c = cidx(i, j, k, nx, ny);
+                size_t ip = cidx(wrapi(i + 1, nx), j, k, nx, ny), jp = cidx(i, wrapi(j + 1, ny), k, nx, ny);
+                size_t kp = cidx(i, j, wrapi(k + 1, nz), nx, ny);
+                /* the density update of o_ct_step, for its floor decision */
+                real r0 = qin[c] + dtdx * ((F[c] - F[ip]) + (F[(size_t)8 * n + c] - F[(size_t)8 * n + jp]) +
+                                           (F[(size_t)16 * n + c] - F[(size_t)16 * n + kp]));
+                int floored = sc->update_floor && r0 < smallr;
+                for (int sp = 0; sp < nsp; ++sp) {
+                    const real* fx = FS + (size_t)(0 * nsp + sp) * n;
+                    const real* fy = FS + (size_t)(1 * nsp + sp) * n;
+                    const real* fz = FS + (size_t)(2 * nsp + sp) * n;
+                    real r = qin[(size_t)(8 + sp) * n + c] + dtdx * ((fx[c] - fx[ip]) + (fy[c] - fy[jp]) + (fz[c] - fz[kp]));
+                    if (floored) r = smallr * X[(size_t)sp * n + c];
+                    qout[(size_t)(8 + sp) * n + c] = r;
+                }
+            }
+    return 0;
+}
+
+size_t o_ct_step_sp_scratch_reals(int nx, int ny, int nz, int nsp) { return (size_t)(38 + 4 * nsp) * nx * ny * nz; }
+
 /* ct_mhd.jl:191-194 divB_max (face divergence) */
 double o_ct_divb_max(const real* q, int nx, int ny, int nz, double dx)
 {

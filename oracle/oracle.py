@@ -189,6 +189,8 @@ class Oracle:
             "o_ou_afield": (None, [i, vp, vp, C.c_double, vp, vp, vp, vp]),
             "o_ct_step": (i, [vp, vp, i, i, i, vp, vp, vp]),
             "o_ct_step_scratch_reals": (C.c_size_t, [i, i, i]),
+            "o_ct_step_sp": (i, [vp, vp, i, i, i, i, vp, vp, vp]),
+            "o_ct_step_sp_scratch_reals": (C.c_size_t, [i, i, i, i]),
             "o_ct_divb_max": (C.c_double, [vp, i, i, i, C.c_double]),
             "o_ct_ctot_max": (C.c_double, [vp, i, i, i, C.c_double, C.c_double, C.c_double]),
             "o_ct_face_flux": (None, [vp, vp, vp, vp, rl, i, vp, vp]),
@@ -437,6 +439,22 @@ class Oracle:
         cs = sc.to_c()
         af = None if afield is None else self.arr(afield)
         self.lib.o_ct_step(_ptr(q), _ptr(out), nx, ny, nz, C.byref(cs), None if af is None else _ptr(af), _ptr(scratch))
+        return out
+
+    def ct_step_sp(self, q, sc: CTScheme, nsp: int, afield=None):
+        """One CT step of an (8 + nsp)-plane state: the CT planes and nsp passive species rho X
+        (o_ct_step_sp)."""
+        nv, nz, ny, nx = q.shape
+        assert nv == 8 + nsp
+        q = self.arr(q)
+        out = np.zeros_like(q)
+        scratch = np.empty(self.lib.o_ct_step_sp_scratch_reals(nx, ny, nz, nsp), dtype=self.dtype)
+        cs = sc.to_c()
+        af = None if afield is None else self.arr(afield)
+        rc = self.lib.o_ct_step_sp(_ptr(q), _ptr(out), nx, ny, nz, nsp, C.byref(cs), None if af is None else _ptr(af),
+                                   _ptr(scratch))
+        if rc:
+            raise ValueError(f"o_ct_step_sp: nsp must be 0..2, not {nsp}")
         return out
 
     def ct_divb_max(self, q, dx):

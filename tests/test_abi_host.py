@@ -263,6 +263,21 @@ def test_ct_library_exports_only_the_ct_bridge():
     assert lib.march_nv() == 8 and lib.march_nsp() == 0
 
 
+def test_ct_species_library_exports_only_the_ct_bridge():
+    """libct480_nsp2.so (NSP=2 build_ct.sh; spike_ctm.cu built with -DNSP=2): the CT symbols with
+    march_nv = 10 and march_nsp = 2 (spike_ctm.cu:301), no march_run_dev, no march_set_glm."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", "libct480_nsp2.so")
+    assert os.path.exists(path), "build it with NSP=2 glmmhdturb_amd/build_ct.sh"
+    exported = _exported(path)
+    for sym in ("march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_gamma", "ct_run", "ct_regs",
+                "ct_shmem"):
+        assert sym in exported, sym
+    for sym in ("march_run_dev", "march_run_dev_f64", "march_set_glm"):
+        assert sym not in exported, sym
+    lib = C.CDLL(path)
+    assert lib.march_nv() == 10 and lib.march_nsp() == 2 and lib.march_nx() == 480
+
+
 def test_hydro_libraries_have_no_glm_or_ct_entry_points():
     for name in ("libmarch480.so", "libmarch480_cma.so"):
         exported = _exported(os.path.join(ROOT, "glmmhdturb_amd", name))

@@ -84,7 +84,7 @@ GLM_DEV T ct_cf(const T q[8], T bn, T gamma)
 {
     T ir = trcp(q[0]);
     T c2 = gamma * q[4] * ir, b2 = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir, dd = T(0.5) * (b2 + c2);
-    return tsqrt(dd + tsqrt(tmax(dd * dd - c2 * bn * bn * ir, T(0))));
+    return tsqrt_pos(dd + tsqrt_nn(tmax(dd * dd - c2 * bn * bn * ir, T(0))));
 }
 
 // ct_mhd.jl:74-82 / spike_ctm.cu:170-184 hanc1d with offset hs (face = c -/+ hs).  A is any

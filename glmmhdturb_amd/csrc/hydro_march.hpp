@@ -124,7 +124,7 @@ template <typename T, int D, bool FB, int NV, bool CMA>
 GLM_DEV void hy_riemann(const T L[NV], const T R[NV], T F[NV], const HydroArgs<T>& a)
 {
     const T unL = L[1 + D], unR = R[1 + D];
-    const T aL = tsqrt(a.gamma * L[4] * trcp(L[0])), aR = tsqrt(a.gamma * R[4] * trcp(R[0]));
+    const T aL = tsqrt_pos(a.gamma * L[4] * trcp(L[0])), aR = tsqrt_pos(a.gamma * R[4] * trcp(R[0]));
     T FL[NV], FR[NV], UL[NV], UR[NV];
     hy_flux_cons<T, D, NV>(L, FL, UL, a);
     hy_flux_cons<T, D, NV>(R, FR, UR, a);

@@ -82,17 +82,42 @@ GLM_HD float trcp(float x)
 // without the builtin's range scaling, which these operands never need.
 // The float version is the hardware square root (1 ulp).  (The host build, used only by CPU
 // checks, calls the library root.)
-GLM_HD double tsqrt(double a)
+GLM_HD double tsqrt_core(double a, double y)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double y = __builtin_amdgcn_rsq(a);
     double g = a * y, h = 0.5 * y;
     const double r = __builtin_fma(-g, h, 0.5);
     g = __builtin_fma(g, r, g);
     h = __builtin_fma(h, r, h);
     const double d = __builtin_fma(-g, g, a);
-    const double s = __builtin_fma(d, h, g);
+    return __builtin_fma(d, h, g);
+}
+GLM_HD double tsqrt(double a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double s = tsqrt_core(a, __builtin_amdgcn_rsq(a));
     return a == 0.0 ? a : s;   // the estimate of 1/sqrt(0) is infinite; negatives give NaN as sqrt does
+#else
+    return __builtin_sqrt(a);
+#endif
+}
+// The root of a positive operand (no zero test): the outer root of a fast speed, whose operand
+// is at least half the squared sound speed.
+GLM_HD double tsqrt_pos(double a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return tsqrt_core(a, __builtin_amdgcn_rsq(a));
+#else
+    return __builtin_sqrt(a);
+#endif
+}
+// The root of a non-negative operand that is 0 or far above 1e-300 (the inner root of a fast
+// speed, a sum of squares): the estimate is taken at max(a, 1e-300), so a = 0 gives g = 0 and the
+// iteration returns exactly 0 with no select; an operand below 1e-300 would get a root with a
+// poor estimate, which next to the outer sum (>= the squared sound speed) is invisible.
+GLM_HD double tsqrt_nn(double a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return tsqrt_core(a, __builtin_amdgcn_rsq(__builtin_fmax(a, 1e-300)));
 #else
     return __builtin_sqrt(a);
 #endif
@@ -144,6 +169,13 @@ GLM_HD float tsqrt(float a)
     return __builtin_sqrtf(a);
 #endif
 }
+GLM_HD float tsqrt_pos(float a) { return tsqrt(a); }
+GLM_HD float tsqrt_nn(float a) { return tsqrt(a); }
+// Any other scalar type (the dual numbers of the CPU checks): the plain root.
+template <typename T>
+GLM_HD T tsqrt_pos(T a) { return tsqrt(a); }
+template <typename T>
+GLM_HD T tsqrt_nn(T a) { return tsqrt(a); }
 
 // Gas pressure (gamma-1)(E - |m|^2/(2 rho) - |B|^2/2), floored (glmmhd_turb.jl:75).  With FP32 state
 // the subtraction is carried out in FP64: in low-beta or high-Mach cells E is dominated by the
@@ -194,7 +226,7 @@ GLM_HD T fast_speed(T rho, T p, T b2, T bn, const Phys<T>& s)
     T c2 = s.gamma * p * ir;
     T bb = b2 * ir;
     T d2 = T(0.5) * (bb + c2);
-    return tsqrt(d2 + tsqrt(tmax(d2 * d2 - c2 * bn * bn * ir, T(0))));
+    return tsqrt_pos(d2 + tsqrt_nn(tmax(d2 * d2 - c2 * bn * bn * ir, T(0))));
 }
 
 // CFL signal speed sum_d (|v_d| + c_f(B_d)) of a primitive state (glmmhd_turb.jl:1963-1971),
@@ -210,9 +242,9 @@ GLM_DEV double cfl_speed(const T q[9], const Phys<T>& s)
     T bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
     T d2 = T(0.5) * (bb + c2);
     T dd = d2 * d2, k = c2 * ir;
-    T fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], T(0))));
-    T fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], T(0))));
-    T fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], T(0))));
+    T fx = tsqrt_pos(d2 + tsqrt_nn(tmax(dd - k * q[5] * q[5], T(0))));
+    T fy = tsqrt_pos(d2 + tsqrt_nn(tmax(dd - k * q[6] * q[6], T(0))));
+    T fz = tsqrt_pos(d2 + tsqrt_nn(tmax(dd - k * q[7] * q[7], T(0))));
     return (double)((tabs(q[1]) + fx) + (tabs(q[2]) + fy) + (tabs(q[3]) + fz));
 }
 
@@ -226,9 +258,9 @@ GLM_DEV double fast_max_speed(const T q[9], const Phys<T>& s)
     T bb = (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]) * ir;
     T d2 = T(0.5) * (bb + c2);
     T dd = d2 * d2, k = c2 * ir;
-    T fx = tsqrt(d2 + tsqrt(tmax(dd - k * q[5] * q[5], T(0))));
-    T fy = tsqrt(d2 + tsqrt(tmax(dd - k * q[6] * q[6], T(0))));
-    T fz = tsqrt(d2 + tsqrt(tmax(dd - k * q[7] * q[7], T(0))));
+    T fx = tsqrt_pos(d2 + tsqrt_nn(tmax(dd - k * q[5] * q[5], T(0))));
+    T fy = tsqrt_pos(d2 + tsqrt_nn(tmax(dd - k * q[6] * q[6], T(0))));
+    T fz = tsqrt_pos(d2 + tsqrt_nn(tmax(dd - k * q[7] * q[7], T(0))));
     return (double)tmax(fx, tmax(fy, fz));
 }
 
@@ -275,12 +307,21 @@ GLM_HD T minmod(T a, T b)
 {
     return (a * b <= T(0)) ? T(0) : (tabs(a) < tabs(b) ? a : b);
 }
+// Half the MonCen slope without selects: with s the sign of dl + dr, a = s dl and b = s dr,
+// 0.5 moncen(dl, dr) = s max(0, min(a, b, |dl + dr| / 4)).  When dl and dr share a sign, a and b are
+// |dl| and |dr| and the min is 0.5 min(2|dl|, 2|dr|, |dc|) exactly (powers of two); otherwise one of
+// a, b is <= 0 and the max gives 0 (possibly -0, as moncen's own sign convention).
+template <typename T>
+GLM_HD T moncen_half(T dl, T dr) { return T(0.5) * moncen(dl, dr); }
+GLM_HD double moncen_half(double dl, double dr);
+GLM_HD float moncen_half(float dl, float dr);
 // Half slope (glmmhd_turb.jl:153-155 MonCen / :302-303 minmod).
 template <typename T, int SLOPE>
 GLM_HD T half_slope(T l, T m, T r)
 {
     T dl = m - l, dr = r - m;
-    return T(0.5) * (SLOPE == SLOPE_MINMOD ? minmod(dl, dr) : moncen(dl, dr));
+    if (SLOPE == SLOPE_MINMOD) return T(0.5) * minmod(dl, dr);
+    return moncen_half(dl, dr);
 }
 
 // glmmhd_turb.jl:917-923 ppm_mono / ppm_edges (Julia form).
@@ -306,7 +347,7 @@ GLM_HD void ppm_edges(T qm, T q0, T qp, T& lo, T& hi)
 // form.  With a = slope - curve and b = slope + curve the unlimited offsets, ppm_edges' tests are
 // (flat) a*b <= 0 -> both 0; (diff > lim) <=> |a| > 2|b| -> a = 2b; (diff < -lim) <=> |b| > 2|a|
 // -> b = 2a (diff = (a^2 - b^2)/2 and lim = (a + b)^2/6, so diff > lim <=> (a + b)(a - 2b) > 0).
-// With s the sign of a + b that is a' = s max(0, min(s a, 2 s b)) and b' = s max(0, min(s b, 2 s a)):
+// With s the sign of a + b (that is, of the slope) a' = s max(0, min(s a, 2 s b)) and b' = s max(0, min(s b, 2 s a)):
 // the same limited parabola up to rounding, with no selects (the limiter is continuous at each
 // switch, so a switch decided differently by rounding moves the result by rounding only).
 GLM_HD double flip_sign(double x, double ref)
@@ -319,16 +360,29 @@ GLM_HD float flip_sign(float x, float ref)
     const unsigned m = __builtin_bit_cast(unsigned, ref) & 0x80000000u;
     return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) ^ m);
 }
+GLM_HD double moncen_half(double dl, double dr)
+{
+    const double sum = dl + dr;
+    const double a = flip_sign(dl, sum), b = flip_sign(dr, sum);
+    return flip_sign(__builtin_fmax(__builtin_fmin(__builtin_fmin(a, b), 0.25 * __builtin_fabs(sum)), 0.0), sum);
+}
+GLM_HD float moncen_half(float dl, float dr)
+{
+    const float sum = dl + dr;
+    const float a = flip_sign(dl, sum), b = flip_sign(dr, sum);
+    return flip_sign(__builtin_fmaxf(__builtin_fminf(__builtin_fminf(a, b), 0.25f * __builtin_fabsf(sum)), 0.0f), sum);
+}
 template <typename T>
 GLM_HD void ppm_offsets(T qm, T q0, T qp, T& dl, T& dr)
 {
     T slope = T(0.25) * (qp - qm);
     T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
     T a = slope - curve, b = slope + curve;
-    T sum = a + b;
-    T sa = flip_sign(a, sum), sb = flip_sign(b, sum);
-    dl = flip_sign(tmax(tmin(sa, sb + sb), T(0)), sum);
-    dr = flip_sign(tmax(tmin(sb, sa + sa), T(0)), sum);
+    // s = sign(slope): where a and b share a sign it is theirs (a + b = 2 slope up to rounding, which
+    // keeps signs), and where they do not both offsets are 0 whatever s is
+    T sa = flip_sign(a, slope), sb = flip_sign(b, slope);
+    dl = flip_sign(tmax(tmin(sa, sb + sb), T(0)), slope);
+    dr = flip_sign(tmax(tmin(sb, sa + sa), T(0)), slope);
 }
 
 // spike_mhd.cu:164-172 ppm1 (both edges).
@@ -371,8 +425,8 @@ GLM_HD void llf_x(const T L[9], const T R[9], T f[9], const Phys<T>& s)
     T b2R = R[5] * R[5] + R[6] * R[6] + R[7] * R[7];
     T cL, cR;
     if (BOUND) {
-        cL = tsqrt((s.gamma * L[4] + b2L) * trcp(L[0]));
-        cR = tsqrt((s.gamma * R[4] + b2R) * trcp(R[0]));
+        cL = tsqrt_pos((s.gamma * L[4] + b2L) * trcp(L[0]));
+        cR = tsqrt_pos((s.gamma * R[4] + b2R) * trcp(R[0]));
     } else {
         cL = fast_speed(L[0], L[4], b2L, L[5], s);
         cR = fast_speed(R[0], R[4], b2R, R[5], s);

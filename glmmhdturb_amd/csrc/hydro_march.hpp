@@ -215,8 +215,10 @@ inline void hydro_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int&
 }
 
 // Resident waves per SIMD the register allocation is held to: as many as the LDS ring allows
-// (FP64: 2 workgroups per CU of 69 KB, 1 of 97 KB with scalars; FP32: 4 of 35 KB, 3 of 48 KB).
-#define GLM_HYDRO_WAVES_PER_EU(T, NSP) (sizeof(T) == 8 ? ((NSP) ? 1 : 2) : ((NSP) ? 3 : 4))
+// (FP64: 2 workgroups per CU of 69 KB, 1 of 97 KB with scalars; FP32: 4 of 35 KB), except FP32 with
+// scalars, where 3 workgroups of 48 KB would hold it to 168 registers and spill: 2 workgroups and
+// no spills run 1.4x faster (480^3, measured).
+#define GLM_HYDRO_WAVES_PER_EU(T, NSP) (sizeof(T) == 8 ? ((NSP) ? 1 : 2) : ((NSP) ? 2 : 4))
 
 template <typename T, bool FB, int NSP, bool CMA>
 __global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T, NSP))))

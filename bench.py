@@ -745,6 +745,72 @@ def run_hydro(N=480, iters=30, warmup=3, dtdx=0.05):
         s.close()
         torch.cuda.empty_cache()
     out["fp32_over_fp64"] = out["f32"]["value"] / out["f64"]["value"]
+    out["compact"] = run_hydro_compact(N, iters, warmup, dtdx)
+    return out
+
+
+# The reference's figures for its compact-storage builds of the hydro march (Mcell/s, RTX A6000,
+# 480^3, spike_25d.cu:64-95): -DU16STATE 7410 (+11% over the fp32 build), -DF16STATE the same
+# throughput, -DU16SP with 2 species 5340.
+REF_COMPACT_A6000 = {"f16_prim": 7410, "u16_prim": 7410, "u16_log2_species": 5340}
+
+
+def run_hydro_compact(N=480, iters=30, warmup=3, dtdx=0.05):
+    """The hydro march on the compact storage formats (glm_hydro_step_compact), FP32 compute, on the
+    same IC as run_hydro (primitives stored as f16 or uint16 codes; two species fractions
+    0.3 + 0.2 ph and 1e-20 (1 + 0.5 ph), spike_25d.cu:546-547, as uint16 log2 codes)."""
+    import numpy as np
+    import torch
+    from glmmhdturb_amd import Params, Solver, hydro_args
+    s = Solver(Params(N=N, gamma=1.4), "f32")
+    n = N ** 3
+    i = torch.arange(n, device=s.device, dtype=torch.int64)
+    ph = 0.001 * (i % 911).to(torch.float32)
+    del i
+    rho, mx, my, mz, E = 1.0 + 0.3 * ph, 0.1 * ph, -0.1 * ph, 0.05 * ph, 2.5 + 0.5 * ph
+    vx, vy, vz = mx / rho, my / rho, mz / rho
+    p = torch.clamp(0.4 * (E - 0.5 * rho * (vx * vx + vy * vy + vz * vz)), min=1e-6)
+    shape = (N, N, N)
+
+    def u16(x, lo, hi):      # uint16 code of x over [lo, hi] (spike_25d.cu:157-168)
+        t = ((x.clamp(lo, hi) - lo) * (65535.0 / (hi - lo)) + 0.5).floor().to(torch.int32)
+        return t.to(torch.int16).view(shape).contiguous()
+
+    fmts = {
+        "f16_prim": [x.to(torch.float16).view(shape).contiguous() for x in (rho, vx, vy, vz, p)],
+        "u16_prim": [u16(torch.log2(rho), -32.0, 32.0), u16(vx, -8.0, 8.0), u16(vy, -8.0, 8.0), u16(vz, -8.0, 8.0),
+                     u16(torch.log2(p), -32.0, 32.0)],
+        "u16_log2_species": [x.view(shape).contiguous() for x in (rho, mx, my, mz, E)] +
+                            [u16(torch.log2(0.3 + 0.2 * ph), -110.0, 0.0), u16(torch.log2(1e-20 * (1.0 + 0.5 * ph)), -110.0, 0.0)],
+    }
+    del rho, mx, my, mz, E, vx, vy, vz, p, ph
+    bytes_per_cell = {"f16_prim": 20, "u16_prim": 20, "u16_log2_species": 48}
+    out = {"workload": f"hydro line march {N}^3, compact storage, FP32 compute, {warmup} + {iters} steps"}
+    a = hydro_args(dtdx, 1.4)
+    st = torch.cuda.current_stream()
+    for name, q in fmts.items():
+        o = [torch.empty_like(x) for x in q]
+        for _ in range(warmup):
+            s.hydro_step_compact(q, o, a, name)
+            q, o = o, q
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(iters):
+            s.hydro_step_compact(q, o, a, name)
+            q, o = o, q
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        bpc = bytes_per_cell[name]
+        val = n / (ms * 1e-3)
+        out[name] = {"value": val, "unit": "cell-updates/s", "ms_per_step": ms, "bytes_per_cell": bpc,
+                     "achieved_GBs": bpc * n / (ms * 1e-3) / 1e9, "hbm_frac": bpc * n / (ms * 1e-3) / HBM_PEAK,
+                     "reference_a6000_Mcells": REF_COMPACT_A6000[name], "ratio_vs_reference": val / 1e6 / REF_COMPACT_A6000[name]}
+        note(f"hydro compact {name} {N}^3: {ms:.3f} ms per step")
+        del q, o
+        torch.cuda.empty_cache()
+    s.close()
     return out
 
 

@@ -632,7 +632,7 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out)
 // The hydro march (spike_25d.cu:383-516 as build_march.sh builds it), with nsp = 0 or 2 passive
 // scalars (-DSCALARS) whose flux rides the mass flux when cma is set (-DCMA).
 static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma,
-                           const char* who)
+                           const char* who, int fmt = 0)
 {
     char msg[160];
     if (!ctx || !q || !o || !p) {
@@ -667,11 +667,13 @@ static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const g
     h.zchunk = p->zchunk;
     h.nsp = nsp;
     h.cma = nsp > 0 && cma;
-    const int key = 2000 + h.fallback + 2 * nsp + 8 * h.cma;
+    h.fmt = fmt;
+    const int key = 2000 + h.fallback + 2 * nsp + 8 * h.cma + 16 * fmt;
     if (ctx->hydro_slots_key != key) {
         int dev = 0, cus = 0, occ = 0;
-        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback, nsp, h.cma)
-                                        : hydro_march_ptr<float>(h.fallback, nsp, h.cma);
+        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback, nsp, h.cma, fmt)
+                                        : hydro_march_ptr<float>(h.fallback, nsp, h.cma, fmt);
+        if (!fn) return set_err(GLM_ERR_ARG, "hydro step: no kernel for this format and precision");
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, HYDRO_THREADS, 0));
@@ -686,6 +688,15 @@ static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const g
 int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
 {
     return hydro_step_impl(ctx, q, o, p, 0, 0, "glm_hydro_step");
+}
+
+int glm_hydro_step_compact(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int format)
+{
+    if (format != GLM_HYDRO_F16_PRIM && format != GLM_HYDRO_U16_PRIM && format != GLM_HYDRO_U16_LOG2_SPECIES)
+        return set_err(GLM_ERR_ARG, "glm_hydro_step_compact: unknown storage format");
+    if (ctx && ctx->prec != 4) return set_err(GLM_ERR_ARG, "glm_hydro_step_compact: the compact formats need an FP32 context");
+    const int nsp = format == GLM_HYDRO_U16_LOG2_SPECIES ? 2 : 0;
+    return hydro_step_impl(ctx, q, o, p, nsp, nsp > 0, "glm_hydro_step_compact", format);
 }
 
 int glm_hydro_step_scalars(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma)

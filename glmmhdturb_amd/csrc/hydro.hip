@@ -13,28 +13,51 @@ static const void* ptr_fb(int nsp, int cma)
     return cma ? (const void*)&hydro_march_kernel<T, FB, 2, true> : (const void*)&hydro_march_kernel<T, FB, 2, false>;
 }
 
-template <typename T>
-const void* hydro_march_ptr(int fallback, int nsp, int cma)
+// The compact storage formats (FP32 only): f16 and uint16 primitive without scalars, uint16 log2
+// species with CMA.
+template <typename T, bool FB>
+static const void* ptr_fmt(int fmt)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (fmt == HY_F16P) return (const void*)&hydro_march_kernel<T, FB, 0, false, HY_F16P>;
+        if (fmt == HY_U16P) return (const void*)&hydro_march_kernel<T, FB, 0, false, HY_U16P>;
+        if (fmt == HY_U16SP) return (const void*)&hydro_march_kernel<T, FB, 2, true, HY_U16SP>;
+    }
+    return nullptr;
+}
+
+template <typename T>
+const void* hydro_march_ptr(int fallback, int nsp, int cma, int fmt)
+{
+    if (fmt != HY_CONS) return fallback ? ptr_fmt<T, true>(fmt) : ptr_fmt<T, false>(fmt);
     if (nsp != 0 && nsp != 2) return nullptr;
     return fallback ? ptr_fb<T, true>(nsp, cma) : ptr_fb<T, false>(nsp, cma);
 }
 
-template <typename T, bool FB, int NSP, bool CMA>
+template <typename T, bool FB, int NSP, bool CMA, int FMT = HY_CONS>
 static void launch(const HydroArgs<T>& a, dim3 g, hipStream_t st)
 {
-    hipLaunchKernelGGL((hydro_march_kernel<T, FB, NSP, CMA>), g, dim3(HM_NT), 0, st, a);
+    hipLaunchKernelGGL((hydro_march_kernel<T, FB, NSP, CMA, FMT>), g, dim3(HM_NT), 0, st, a);
 }
 
 template <typename T>
 hipError_t launch_hydro_march(void* const* q, void* const* o, const HydroArgsHost& h, int slots, hipStream_t st)
 {
     if (h.nsp != 0 && h.nsp != 2) return hipErrorInvalidValue;
+    if (h.fmt != HY_CONS && (sizeof(T) != 4 || (h.fmt == HY_U16SP ? (h.nsp != 2 || !h.cma) : h.nsp != 0) ||
+                             h.fmt < 0 || h.fmt > HY_U16SP))
+        return hipErrorInvalidValue;
     const int nv = 5 + h.nsp;
     HydroArgs<T> a;
     for (int v = 0; v < HM_NVMAX; ++v) {
-        a.q[v] = v < nv ? (const T*)q[v] : nullptr;
-        a.o[v] = v < nv ? (T*)o[v] : nullptr;
+        // planes of T: every plane of the conserved format, the hydro planes of HY_U16SP;
+        // 16-bit planes: every plane of the primitive formats, the species of HY_U16SP
+        const bool wide = v < nv && (h.fmt == HY_CONS || (h.fmt == HY_U16SP && v < 5));
+        const bool narrow = v < nv && !wide;
+        a.q[v] = wide ? (const T*)q[v] : nullptr;
+        a.o[v] = wide ? (T*)o[v] : nullptr;
+        a.qh[v] = narrow ? (const unsigned short*)q[v] : nullptr;
+        a.oh[v] = narrow ? (unsigned short*)o[v] : nullptr;
     }
     a.nx = h.nx; a.ny = h.ny; a.nz = h.nz;
     hydro_tiling(h.nx, h.ny, h.nz, h.zchunk, slots, a.tiles_x, a.tiles_y, a.nzc, a.zchunk);
@@ -45,6 +68,21 @@ hipError_t launch_hydro_march(void* const* q, void* const* o, const HydroArgsHos
     const long nblk = (long)a.tiles_x * a.tiles_y * a.nzc;
     if (nblk <= 0 || nblk > 0x7fffffffL) return hipErrorInvalidValue;
     const dim3 g((unsigned)nblk);
+    if constexpr (sizeof(T) == 4) {
+        if (h.fmt != HY_CONS) {
+            if (h.fmt == HY_F16P) {
+                if (h.fallback) launch<T, true, 0, false, HY_F16P>(a, g, st);
+                else launch<T, false, 0, false, HY_F16P>(a, g, st);
+            } else if (h.fmt == HY_U16P) {
+                if (h.fallback) launch<T, true, 0, false, HY_U16P>(a, g, st);
+                else launch<T, false, 0, false, HY_U16P>(a, g, st);
+            } else {
+                if (h.fallback) launch<T, true, 2, true, HY_U16SP>(a, g, st);
+                else launch<T, false, 2, true, HY_U16SP>(a, g, st);
+            }
+            return hipGetLastError();
+        }
+    }
     if (h.nsp == 0) {
         if (h.fallback) launch<T, true, 0, false>(a, g, st);
         else launch<T, false, 0, false>(a, g, st);
@@ -60,7 +98,7 @@ hipError_t launch_hydro_march(void* const* q, void* const* o, const HydroArgsHos
 
 template hipError_t launch_hydro_march<double>(void* const*, void* const*, const HydroArgsHost&, int, hipStream_t);
 template hipError_t launch_hydro_march<float>(void* const*, void* const*, const HydroArgsHost&, int, hipStream_t);
-template const void* hydro_march_ptr<double>(int, int, int);
-template const void* hydro_march_ptr<float>(int, int, int);
+template const void* hydro_march_ptr<double>(int, int, int, int);
+template const void* hydro_march_ptr<float>(int, int, int, int);
 
 }  // namespace glm

@@ -33,17 +33,73 @@ constexpr int HM_NT = HM_OX * HM_OY;    // 256 lanes, one per owned column
 
 constexpr int HM_NVMAX = 7;   // 5 hydro variables + 2 passive scalars (-DSCALARS)
 
+// Storage formats of the state in HBM (spike_25d.cu:64-95, the compact-storage builds):
+//   HY_CONS   conserved planes of T (the build_march.sh library);
+//   HY_F16P   -DF16STATE: primitive rho, vx, vy, vz, p as f16;
+//   HY_U16P   -DU16STATE: primitive, rho and p as uint16 log2 over [-32, 32], velocities uint16
+//             linear over [-8, 8];
+//   HY_U16SP  -DU16SP: conserved hydro planes of T, the two species fractions as uint16 log2 over
+//             [-110, 0], reconstructed in log2 (the CMA flux and the conserved rho X decode X).
+// The compact formats compute in FP32.
+enum { HY_CONS = 0, HY_F16P = 1, HY_U16P = 2, HY_U16SP = 3 };
+
 template <typename T>
 struct HydroArgs {
     const T* q[HM_NVMAX];
     T* o[HM_NVMAX];
+    const unsigned short* qh[HM_NVMAX];   // 16-bit planes of the compact formats
+    unsigned short* oh[HM_NVMAX];
     int nx, ny, nz;
     int tiles_x, tiles_y, nzc, zchunk;
     T gm1, igm1, gamma, dtdx, half_dtdx, smallr, smallp, fb_dmin, fb_pmin;
 };
 
-// spike_25d.cu:196-202 prim_from_cons (:183-194 with scalars: s = (rho s) / rho)
-template <typename T, int NV>
+// uint16 codecs (spike_25d.cu:157-168): log2 over [-110, 0] (species), velocity linear over
+// [-8, 8], log2 over [-32, 32] (rho, p); encode rounds to nearest and clamps.  Not contracted:
+// each is a float multiply and a float add, so a codec is a fixed function of its input.
+GLM_DEV float hy_dec_log2(unsigned short u)
+{
+#pragma clang fp contract(off)
+    return -110.f + (float)u * (110.f / 65535.f);
+}
+GLM_DEV unsigned short hy_enc_log2(float l2)
+{
+#pragma clang fp contract(off)
+    float t = (l2 + 110.f) * (65535.f / 110.f);
+    t = fminf(fmaxf(t, 0.f), 65535.f);
+    return (unsigned short)(t + 0.5f);
+}
+GLM_DEV float hy_dec_vel(unsigned short u)
+{
+#pragma clang fp contract(off)
+    return -8.f + (float)u * (16.f / 65535.f);
+}
+GLM_DEV unsigned short hy_enc_vel(float x)
+{
+#pragma clang fp contract(off)
+    const float t = (fminf(fmaxf(x, -8.f), 8.f) + 8.f) * (65535.f / 16.f);
+    return (unsigned short)(t + 0.5f);
+}
+GLM_DEV float hy_dec_log2s(unsigned short u)
+{
+#pragma clang fp contract(off)
+    return -32.f + (float)u * (64.f / 65535.f);
+}
+GLM_DEV unsigned short hy_enc_log2s(float l2)
+{
+#pragma clang fp contract(off)
+    float t = (l2 + 32.f) * (65535.f / 64.f);
+    t = fminf(fmaxf(t, 0.f), 65535.f);
+    return (unsigned short)(t + 0.5f);
+}
+GLM_DEV float hy_f16_to_f32(unsigned short h) { return (float)__builtin_bit_cast(_Float16, h); }
+GLM_DEV unsigned short hy_f32_to_f16(float x) { return __builtin_bit_cast(unsigned short, (_Float16)x); }
+template <typename T>
+GLM_DEV T hy_exp2(T x) { return (T)exp2f((float)x); }
+
+// spike_25d.cu:196-202 prim_from_cons (:183-194 with scalars: s = (rho s) / rho; with LG the scalar
+// planes already hold log2 X, :188-190)
+template <typename T, int NV, bool LG = false>
 GLM_DEV void hy_c2p(const T c[NV], T q[NV], const HydroArgs<T>& a)
 {
 #pragma clang fp contract(off)   // the same bits at every call site of the kernel
@@ -56,7 +112,7 @@ GLM_DEV void hy_c2p(const T c[NV], T q[NV], const HydroArgs<T>& a)
     const T ke = T(0.5) * r * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     q[4] = tmax(a.gm1 * (c[4] - ke), a.smallp);
 #pragma unroll
-    for (int v = 5; v < NV; ++v) q[v] = c[v] * ir;
+    for (int v = 5; v < NV; ++v) q[v] = LG ? c[v] : c[v] * ir;
 }
 
 // spike_25d.cu:207-213 moncen (full slope)
@@ -68,8 +124,9 @@ GLM_DEV T hy_moncen(T l, T m, T r)
     return dl * dr <= T(0) ? T(0) : (dl > T(0) ? s : -s);
 }
 
-// spike_25d.cu:231-244 primFlux and :245-255 toCons
-template <typename T, int D, int NV>
+// spike_25d.cu:231-244 primFlux and :245-255 toCons (with LG the scalars are log2 X, decoded for the
+// flux and the conserved rho X, :248-250)
+template <typename T, int D, int NV, bool LG = false>
 GLM_DEV void hy_flux_cons(const T q[NV], T f[NV], T u[NV], const HydroArgs<T>& a)
 {
     const T un = q[1 + D];
@@ -85,8 +142,9 @@ GLM_DEV void hy_flux_cons(const T q[NV], T f[NV], T u[NV], const HydroArgs<T>& a
     u[4] = E;
 #pragma unroll
     for (int v = 5; v < NV; ++v) {
-        f[v] = mu * q[v];
-        u[v] = q[0] * q[v];
+        const T x = LG ? hy_exp2(q[v]) : q[v];
+        f[v] = mu * x;
+        u[v] = q[0] * x;
     }
 }
 
@@ -120,14 +178,14 @@ GLM_DEV void hy_edge(const T mh[NV], const T s[NV], T sign, T e[NV], const Hydro
 // HLL (spike_25d.cu:302-330); with FB, LLF where a face state is below the fallback floors
 // (riemann_fb, :335-376).  Both combine the same FL, FR, UL, UR.  With CMA the scalar fluxes of
 // the HLL fan are the mass flux times the upwind face value (:321, :367).
-template <typename T, int D, bool FB, int NV, bool CMA>
+template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false>
 GLM_DEV void hy_riemann(const T L[NV], const T R[NV], T F[NV], const HydroArgs<T>& a)
 {
     const T unL = L[1 + D], unR = R[1 + D];
     const T aL = tsqrt_pos(a.gamma * L[4] * trcp(L[0])), aR = tsqrt_pos(a.gamma * R[4] * trcp(R[0]));
     T FL[NV], FR[NV], UL[NV], UR[NV];
-    hy_flux_cons<T, D, NV>(L, FL, UL, a);
-    hy_flux_cons<T, D, NV>(R, FR, UR, a);
+    hy_flux_cons<T, D, NV, LG>(L, FL, UL, a);
+    hy_flux_cons<T, D, NV, LG>(R, FR, UR, a);
     const T sL = tmin(unL - aL, unR - aR), sR = tmax(unL + aL, unR + aR);
     const T inv = trcp(sR - sL), slsr = sL * sR;
     const bool llf = FB && (L[0] < a.fb_dmin || L[4] < a.fb_pmin || R[0] < a.fb_dmin || R[4] < a.fb_pmin);
@@ -136,7 +194,7 @@ GLM_DEV void hy_riemann(const T L[NV], const T R[NV], T F[NV], const HydroArgs<T
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         T hll = v == 0 ? hll0 : (sR * FL[v] - sL * FR[v] + slsr * (UR[v] - UL[v])) * inv;
-        if (CMA && v >= 5) hll = hll0 >= T(0) ? hll0 * L[v] : hll0 * R[v];
+        if (CMA && v >= 5) hll = hll0 >= T(0) ? hll0 * (LG ? hy_exp2(L[v]) : L[v]) : hll0 * (LG ? hy_exp2(R[v]) : R[v]);
         const T h = sL >= T(0) ? FL[v] : (sR <= T(0) ? FR[v] : hll);
         F[v] = llf ? T(0.5) * (FL[v] + FR[v]) - T(0.5) * sm * (UR[v] - UL[v]) : h;
     }
@@ -150,7 +208,7 @@ GLM_DEV void hy_slopes(const T l[NV], const T m[NV], const T r[NV], T s[NV])
 }
 
 // Flux through the face between cells c1 and c2 of the stencil c0 c1 | c2 c3 along D.
-template <typename T, int D, bool FB, int NV, bool CMA>
+template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false>
 GLM_DEV void hy_face(const T c0[NV], const T c1[NV], const T c2[NV], const T c3[NV], T F[NV], const HydroArgs<T>& a)
 {
     T sl[NV], sr[NV], mh[NV], L[NV], R[NV];
@@ -160,13 +218,13 @@ GLM_DEV void hy_face(const T c0[NV], const T c1[NV], const T c2[NV], const T c3[
     hy_edge<T, NV>(mh, sl, T(1), L, a);
     hy_hanc1d<T, D, NV>(c2, sr, mh, a);
     hy_edge<T, NV>(mh, sr, T(-1), R, a);
-    hy_riemann<T, D, FB, NV, CMA>(L, R, F, a);
+    hy_riemann<T, D, FB, NV, CMA, LG>(L, R, F, a);
 }
 
 // div += F(+1/2) - F(-1/2) along D for the cell c0 of the stencil m2 m1 c0 p1 p2: the centre
 // cell's slope and half step are shared by its two faces (spike_25d.cu:462-466 forms them once
 // per face).
-template <typename T, int D, bool FB, int NV, bool CMA>
+template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false>
 GLM_DEV void hy_dir(const T m2[NV], const T m1[NV], const T c0[NV], const T p1[NV], const T p2[NV], T div[NV],
                     const HydroArgs<T>& a)
 {
@@ -177,12 +235,12 @@ GLM_DEV void hy_dir(const T m2[NV], const T m1[NV], const T c0[NV], const T p1[N
     hy_slopes<T, NV>(m1, c0, p1, s);
     hy_hanc1d<T, D, NV>(c0, s, mh, a);
     hy_edge<T, NV>(mh, s, T(-1), R, a);
-    hy_riemann<T, D, FB, NV, CMA>(L, R, Fm, a);
+    hy_riemann<T, D, FB, NV, CMA, LG>(L, R, Fm, a);
     hy_edge<T, NV>(mh, s, T(1), L, a);
     hy_slopes<T, NV>(c0, p1, p2, s);
     hy_hanc1d<T, D, NV>(p1, s, mh, a);
     hy_edge<T, NV>(mh, s, T(-1), R, a);
-    hy_riemann<T, D, FB, NV, CMA>(L, R, Fp, a);
+    hy_riemann<T, D, FB, NV, CMA, LG>(L, R, Fp, a);
 #pragma unroll
     for (int v = 0; v < NV; ++v) div[v] += Fp[v] - Fm[v];
 }
@@ -220,11 +278,15 @@ inline void hydro_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int&
 // no spills run 1.4x faster (480^3, measured).
 #define GLM_HYDRO_WAVES_PER_EU(T, NSP) (sizeof(T) == 8 ? ((NSP) ? 1 : 2) : ((NSP) ? 2 : 4))
 
-template <typename T, bool FB, int NSP, bool CMA>
+template <typename T, bool FB, int NSP, bool CMA, int FMT = HY_CONS>
 __global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T, NSP))))
 hydro_march_kernel(HydroArgs<T> a)
 {
     constexpr int NV = 5 + NSP;
+    constexpr bool LG = FMT == HY_U16SP;      // species held as log2 X
+    constexpr bool PRIM = FMT == HY_F16P || FMT == HY_U16P;   // primitive storage: no conversion on load
+    static_assert(FMT == HY_CONS || (sizeof(T) == 4 && (FMT != HY_U16SP || (NSP == 2 && CMA)) &&
+                                     (FMT == HY_U16SP || NSP == 0)), "compact formats: FP32; U16SP with 2 CMA species");
     // ring of 4 haloed primitive planes: plane j lives in buffer (j - k0 + 2) & 3
     __shared__ T P[4][NV][HM_NPL];
     const int tid = threadIdx.x;
@@ -255,8 +317,23 @@ hydro_march_kernel(HydroArgs<T> a)
             if (c < HM_NPL) {
                 const int lx = c % HM_LX, ly = c / HM_LX;
                 const size_t g = zoff + wrapi(x0 - HM_G + lx, a.nx) + (size_t)a.nx * wrapi(y0 - HM_G + ly, a.ny);
+                if (FMT == HY_CONS) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) pre[r][v] = gload(a.q[v] + g);
+                    for (int v = 0; v < NV; ++v) pre[r][v] = gload(a.q[v] + g);
+                } else if (FMT == HY_F16P) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) pre[r][v] = (T)hy_f16_to_f32(gload(a.qh[v] + g));
+                } else if (FMT == HY_U16P) {   // spike_25d.cu:397-402
+                    pre[r][0] = (T)exp2f(hy_dec_log2s(gload(a.qh[0] + g)));
+#pragma unroll
+                    for (int v = 1; v < 4; ++v) pre[r][v] = (T)hy_dec_vel(gload(a.qh[v] + g));
+                    pre[r][4] = (T)exp2f(hy_dec_log2s(gload(a.qh[4] + g)));
+                } else {                       // HY_U16SP, :409-411
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) pre[r][v] = gload(a.q[v] + g);
+#pragma unroll
+                    for (int v = 5; v < NV; ++v) pre[r][v] = (T)hy_dec_log2(gload(a.qh[v] + g));
+                }
             }
         }
     };
@@ -267,7 +344,14 @@ hydro_march_kernel(HydroArgs<T> a)
             const int c = tid + r * HM_NT;
             if (c < HM_NPL) {
                 T qq[NV];
-                hy_c2p<T, NV>(pre[r], qq, a);
+                if (PRIM) {   // make_prim (spike_25d.cu:204-206)
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qq[v] = pre[r][v];
+                    qq[0] = tmax(qq[0], a.smallr);
+                    qq[4] = tmax(qq[4], a.smallp);
+                } else {
+                    hy_c2p<T, NV, LG>(pre[r], qq, a);
+                }
 #pragma unroll
                 for (int v = 0; v < NV; ++v) P[b][v][c] = qq[v];
             }
@@ -319,25 +403,53 @@ hydro_march_kernel(HydroArgs<T> a)
                 m2[v] = P[b][v][li - 2]; m1[v] = P[b][v][li - 1]; c0[v] = P[b][v][li];
                 p1[v] = P[b][v][li + 1]; p2[v] = P[b][v][li + 2];
             }
-            hy_dir<T, 0, FB, NV, CMA>(m2, m1, c0, p1, p2, div, a);
+            hy_dir<T, 0, FB, NV, CMA, LG>(m2, m1, c0, p1, p2, div, a);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 m2[v] = P[b][v][li - 2 * HM_LX]; m1[v] = P[b][v][li - HM_LX];
                 p1[v] = P[b][v][li + HM_LX]; p2[v] = P[b][v][li + 2 * HM_LX];
             }
-            hy_dir<T, 1, FB, NV, CMA>(m2, m1, c0, p1, p2, div, a);
+            hy_dir<T, 1, FB, NV, CMA, LG>(m2, m1, c0, p1, p2, div, a);
         }
         T Fp[NV];
-        hy_face<T, 2, FB, NV, CMA>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
+        hy_face<T, 2, FB, NV, CMA, LG>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
         if (upd) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) div[v] += Fp[v] - Fz[v];
             T U[NV], f[NV];
-            hy_flux_cons<T, 0, NV>(W[1], f, U, a);   // toCons(prim(U0)); the flux is unused
+            hy_flux_cons<T, 0, NV, LG>(W[1], f, U, a);   // toCons(prim(U0)); the flux is unused
             if (own) {
                 const size_t g = (size_t)k * plane + col;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) a.o[v][g] = U[v] - a.dtdx * div[v];
+                for (int v = 0; v < NV; ++v) U[v] = U[v] - a.dtdx * div[v];
+                if (FMT == HY_CONS) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) a.o[v][g] = U[v];
+                } else if (PRIM) {   // c2p -> store primitive (spike_25d.cu:499-511)
+                    const T ir = T(1) / tmax(U[0], a.smallr);
+                    const T vx = U[1] * ir, vy = U[2] * ir, vz = U[3] * ir;
+                    const T ke = T(0.5) * U[0] * (vx * vx + vy * vy + vz * vz);
+                    const T pp = tmax(a.gm1 * (U[4] - ke), a.smallp);
+                    if (FMT == HY_F16P) {
+                        a.oh[0][g] = hy_f32_to_f16((float)U[0]);
+                        a.oh[1][g] = hy_f32_to_f16((float)vx);
+                        a.oh[2][g] = hy_f32_to_f16((float)vy);
+                        a.oh[3][g] = hy_f32_to_f16((float)vz);
+                        a.oh[4][g] = hy_f32_to_f16((float)pp);
+                    } else {
+                        a.oh[0][g] = hy_enc_log2s(log2f((float)tmax(U[0], a.smallr)));
+                        a.oh[1][g] = hy_enc_vel((float)vx);
+                        a.oh[2][g] = hy_enc_vel((float)vy);
+                        a.oh[3][g] = hy_enc_vel((float)vz);
+                        a.oh[4][g] = hy_enc_log2s(log2f((float)pp));
+                    }
+                } else {             // HY_U16SP: X_new = (rho X)_new / rho_new -> log2 -> uint16 (:514-517)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) a.o[v][g] = U[v];
+                    const T rinv = T(1) / tmax(U[0], a.smallr);
+#pragma unroll
+                    for (int v = 5; v < NV; ++v) a.oh[v][g] = hy_enc_log2(log2f(fmaxf((float)(U[v] * rinv), 7.7e-34f)));
+                }
             }
         }
 #pragma unroll

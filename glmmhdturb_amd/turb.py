@@ -431,6 +431,33 @@ class Solver:
         else:
             check(self.lib.glm_hydro_step_scalars(self.ctx, qa, oa, C.byref(args), int(nsp), int(cma)))
 
+    HYDRO_FORMATS = {"f16_prim": 1, "u16_prim": 2, "u16_log2_species": 3}
+
+    def hydro_step_compact(self, qplanes, oplanes, args: GLMParams, fmt: str):
+        """hydro_step on a compact storage format (spike_25d.cu -DF16STATE / -DU16STATE / -DU16SP,
+        :64-95), FP32 contexts only: "f16_prim" (5 float16 planes rho, vx, vy, vz, p), "u16_prim"
+        (5 16-bit planes, rho and p log2-coded, velocities linear) or "u16_log2_species" (5 float32
+        conserved planes, then 2 16-bit planes of log2-coded species fractions).  qplanes, oplanes:
+        sequences of contiguous (nz, ny, nx) tensors on the context's device."""
+        if fmt not in self.HYDRO_FORMATS:
+            raise ValueError(f"fmt must be one of {sorted(self.HYDRO_FORMATS)}, not {fmt!r}")
+        code = self.HYDRO_FORMATS[fmt]
+        nv = 7 if code == 3 else 5
+        shape = (self.nz, self.ny, self.nx)
+        for planes in (qplanes, oplanes):
+            if len(planes) != nv:
+                raise ValueError(f"{fmt} takes {nv} planes, not {len(planes)}")
+            for v, x in enumerate(planes):
+                wide = code == 3 and v < 5
+                size = 4 if wide else 2
+                if (tuple(x.shape) != shape or not x.is_contiguous() or x.element_size() != size
+                        or x.device != self.device):
+                    raise ValueError(f"{fmt}: plane {v} must be a contiguous {size}-byte tensor of shape {shape} "
+                                     f"on {self.device}")
+        qa, kq = plane_array([x.data_ptr() for x in qplanes])
+        oa, ko = plane_array([x.data_ptr() for x in oplanes])
+        check(self.lib.glm_hydro_step_compact(self.ctx, qa, oa, C.byref(args), int(code)))
+
     def ct_ctot_max(self, q) -> float:
         pa, keep = self._planes(q, 8)
         out = C.c_double()

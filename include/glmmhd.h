@@ -211,6 +211,19 @@ int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_param
  * upwind face value of s (-DCMA, consistent multi-fluid advection, :321, :367); cma = 0 solves
  * rho*s with the same HLL/LLF as the other variables.  nsp = 0 is glm_hydro_step. */
 int glm_hydro_step_scalars(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma);
+/* The same march on the compact storage formats of spike_25d.cu's standalone builds (:64-95; the
+ * reference's bridge library takes FP32 conserved planes only, so this entry has no counterpart in
+ * its C ABI).  FP32 contexts only; computation in FP32.
+ *   GLM_HYDRO_F16_PRIM (-DF16STATE): q, o = 5 planes of IEEE f16 bits [rho, vx, vy, vz, p];
+ *   GLM_HYDRO_U16_PRIM (-DU16STATE): 5 uint16 planes, rho and p as log2 over [-32, 32], the
+ *     velocities linear over [-8, 8] (:157-168);
+ *   GLM_HYDRO_U16_LOG2_SPECIES (-DU16SP): 5 float conserved planes, then 2 uint16 planes of log2 X
+ *     over [-110, 0]; the species ride the mass flux (CMA) and are reconstructed in log2 X.
+ * Each stores its result in its own format (rounded to nearest).  Where a face is supersonic the
+ * reference's -DU16SP build returns the one-sided flux with the mass flux times log2 X; here the
+ * species flux is the mass flux times X on every face. */
+enum { GLM_HYDRO_F16_PRIM = 1, GLM_HYDRO_U16_PRIM = 2, GLM_HYDRO_U16_LOG2_SPECIES = 3 };
+int glm_hydro_step_compact(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int format);
 
 /* ===================================================================================== */
 /* 4. Adjoint (no reference counterpart: the reference computes no derivatives).             */

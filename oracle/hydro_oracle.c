@@ -33,6 +33,8 @@ typedef struct {
     double fb_dmin, fb_pmin;            /* the fallback's floors (1e-3 in :339) */
     int nsp;                            /* passive scalars: 0 or 2 (-DSCALARS, :115-119) */
     int cma;                            /* scalar flux rides the mass flux (-DCMA, :321, :367) */
+    int logsp;                          /* the scalar planes hold log2 X, reconstructed in log space
+                                           (-DU16SP, :64-72, :188-191, :248-250, :317-319) */
 } HyScheme;
 
 static int nvar(const HyScheme* s) { return 5 + s->nsp; }
@@ -48,7 +50,7 @@ void oh_c2p(const real c[HV], real q[HV], const HyScheme* s)
     q[3] = c[3] * ir;
     real ke = R(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     q[4] = fmax((g - R(1)) * (c[4] - ke), sp);
-    for (int v = 5; v < nvar(s); ++v) q[v] = c[v] * ir;   /* :191 */
+    for (int v = 5; v < nvar(s); ++v) q[v] = s->logsp ? c[v] : c[v] * ir;   /* :191; log2 X as is (:188-190) */
 }
 
 /* spike_25d.cu:207-213 moncen: the full monotonized-central slope */
@@ -83,7 +85,9 @@ void oh_flux(const real q[HV], int dir, real f[HV], const HyScheme* s)
     f[0] = q[0] * un;
     for (int c = 0; c < 3; ++c) f[1 + c] = q[0] * q[1 + c] * un + (dir == c ? q[4] : R(0));
     f[4] = (E + q[4]) * un;
-    for (int v = 5; v < nvar(s); ++v) f[v] = q[0] * un * q[v];   /* :241 */
+    /* :241; with log2 X the fraction is decoded (the reference's supersonic return of this flux
+     * multiplies by log2 X itself; here X is used on every face) */
+    for (int v = 5; v < nvar(s); ++v) f[v] = q[0] * un * (s->logsp ? exp2(q[v]) : q[v]);
 }
 
 /* spike_25d.cu:245-255 toCons */
@@ -95,7 +99,7 @@ void oh_tocons(const real q[HV], real c[HV], const HyScheme* s)
     c[2] = q[0] * q[2];
     c[3] = q[0] * q[3];
     c[4] = q[4] / (g - R(1)) + R(0.5) * q[0] * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    for (int v = 5; v < nvar(s); ++v) c[v] = q[0] * q[v];   /* :251 */
+    for (int v = 5; v < nvar(s); ++v) c[v] = q[0] * (s->logsp ? exp2(q[v]) : q[v]);   /* :251, :248-250 */
 }
 
 /* spike_25d.cu:287-301 hanc1d: normal-only Hancock half step in direction dir */
@@ -128,7 +132,10 @@ void oh_hll(const real L[HV], const real Rt[HV], int dir, real F[HV], const HySc
     real inv = R(1) / (sR - sL);
     for (int v = 0; v < nvar(s); ++v) F[v] = (sR * FL[v] - sL * FR[v] + sL * sR * (UR[v] - UL[v])) * inv;
     if (s->cma)   /* :321 */
-        for (int v = 5; v < nvar(s); ++v) F[v] = F[0] >= R(0) ? F[0] * L[v] : F[0] * Rt[v];
+        for (int v = 5; v < nvar(s); ++v) {
+            real xl = s->logsp ? exp2(L[v]) : L[v], xr = s->logsp ? exp2(Rt[v]) : Rt[v];   /* :317-319 */
+            F[v] = F[0] >= R(0) ? F[0] * xl : F[0] * xr;
+        }
 }
 
 /* spike_25d.cu:335-376 riemann_fb: LLF where a face state is below the fallback floors,
@@ -178,11 +185,33 @@ size_t oh_step_scratch_reals(int nx, int ny, int nz) { return (size_t)HV * nx * 
 
 /* One step of the march, spike_25d.cu:430-516 (HANCOCK1D branch :462-466, update :492,
  * store :512).  Returns 0. */
+static int step_from_prim(const real* prim, real* out, int nx, int ny, int nz, const HyScheme* s);
+
 int oh_step(const real* u, real* out, int nx, int ny, int nz, const HyScheme* s, real* scratch)
 {
+    prim_all(u, scratch, nx, ny, nz, s);
+    return step_from_prim(scratch, out, nx, ny, nz, s);
+}
+
+/* The same step from primitive planes [rho, vx, vy, vz, p (, s...)] held as such (the compact
+ * primitive storage of -DF16STATE / -DU16STATE, :74-95: the tile holds make_prim(...), rho and p
+ * floored at SMALLR/SMALLP, :204-206, with no conversion from conserved).  out: conserved. */
+int oh_step_prim(const real* prim_in, real* out, int nx, int ny, int nz, const HyScheme* s, real* scratch)
+{
     size_t n = (size_t)nx * ny * nz;
-    real* prim = scratch;
-    prim_all(u, prim, nx, ny, nz, s);
+    for (int v = 0; v < nvar(s); ++v)
+        for (size_t c = 0; c < n; ++c) {
+            real x = prim_in[(size_t)v * n + c];
+            if (v == 0) x = fmax(x, R(s->smallr));
+            if (v == 4) x = fmax(x, R(s->smallp));
+            scratch[(size_t)v * n + c] = x;
+        }
+    return step_from_prim(scratch, out, nx, ny, nz, s);
+}
+
+static int step_from_prim(const real* prim, real* out, int nx, int ny, int nz, const HyScheme* s)
+{
+    size_t n = (size_t)nx * ny * nz;
     real dtdx = R(s->dtdx);
 #pragma omp parallel for schedule(static) collapse(2)
     for (int k = 0; k < nz; ++k)

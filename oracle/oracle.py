@@ -48,7 +48,8 @@ class OCTScheme(C.Structure):
 
 class OHyScheme(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "dtdx", "smallr", "smallp")] + [
-        ("fallback", C.c_int), ("fb_dmin", C.c_double), ("fb_pmin", C.c_double), ("nsp", C.c_int), ("cma", C.c_int)]
+        ("fallback", C.c_int), ("fb_dmin", C.c_double), ("fb_pmin", C.c_double), ("nsp", C.c_int), ("cma", C.c_int),
+        ("logsp", C.c_int)]
 
 
 @dataclass
@@ -64,10 +65,11 @@ class HydroScheme:
     fb_pmin: float = 1e-3
     nsp: int = 0
     cma: bool = False
+    logsp: bool = False        # the scalar planes hold log2 X (-DU16SP)
 
     def to_c(self) -> OHyScheme:
         return OHyScheme(self.gamma, self.dtdx, self.smallr, self.smallp, int(self.fallback), self.fb_dmin,
-                         self.fb_pmin, self.nsp, int(self.cma))
+                         self.fb_pmin, self.nsp, int(self.cma), int(self.logsp))
 
 
 @dataclass
@@ -205,6 +207,7 @@ class Oracle:
             "oh_hll": (None, [vp, vp, i, vp, vp]),
             "oh_riemann_fb": (None, [vp, vp, i, vp, vp]),
             "oh_step": (i, [vp, vp, i, i, i, vp, vp]),
+            "oh_step_prim": (i, [vp, vp, i, i, i, vp, vp]),
             "oh_step_scratch_reals": (C.c_size_t, [i, i, i]),
         }
         for name, (res, args) in sig.items():
@@ -513,6 +516,65 @@ class Oracle:
         cs = sc.to_c()
         self.lib.oh_step(_ptr(u), _ptr(out), nx, ny, nz, C.byref(cs), _ptr(scratch))
         return out
+
+    def hydro_step_prim(self, q, sc: HydroScheme):
+        """The same step from primitive planes [rho, vx, vy, vz, p] (rho, p floored as the tile's
+        make_prim does); returns the conserved result (oh_step_prim)."""
+        nv, nz, ny, nx = q.shape
+        assert nv == 5 + sc.nsp, nv
+        q = self.arr(q)
+        out = np.zeros_like(q)
+        scratch = np.empty(self.lib.oh_step_scratch_reals(nx, ny, nz), dtype=self.dtype)
+        cs = sc.to_c()
+        self.lib.oh_step_prim(_ptr(q), _ptr(out), nx, ny, nz, C.byref(cs), _ptr(scratch))
+        return out
+
+
+# ---- compact storage codecs of the hydro march (spike_25d.cu:157-168), float32 arithmetic ----
+_F = np.float32
+
+
+def dec_log2(u):
+    """uint16 -> log2 X over [-110, 0]"""
+    return _F(-110) + np.asarray(u).astype(np.float32) * (_F(110) / _F(65535))
+
+
+def enc_log2(l2):
+    t = (np.asarray(l2, dtype=np.float32) + _F(110)) * (_F(65535) / _F(110))
+    t = np.minimum(np.maximum(t, _F(0)), _F(65535))
+    return np.floor(t + _F(0.5)).astype(np.uint16)
+
+
+def dec_vel(u):
+    """uint16 -> velocity over [-8, 8]"""
+    return _F(-8) + np.asarray(u).astype(np.float32) * (_F(16) / _F(65535))
+
+
+def enc_vel(x):
+    t = (np.minimum(np.maximum(np.asarray(x, dtype=np.float32), _F(-8)), _F(8)) + _F(8)) * (_F(65535) / _F(16))
+    return np.floor(t + _F(0.5)).astype(np.uint16)
+
+
+def dec_log2s(u):
+    """uint16 -> log2 of rho or p over [-32, 32]"""
+    return _F(-32) + np.asarray(u).astype(np.float32) * (_F(64) / _F(65535))
+
+
+def enc_log2s(l2):
+    t = (np.asarray(l2, dtype=np.float32) + _F(32)) * (_F(65535) / _F(64))
+    t = np.minimum(np.maximum(t, _F(0)), _F(65535))
+    return np.floor(t + _F(0.5)).astype(np.uint16)
+
+
+def cons_to_stored_prim(U, smallr, smallp, gamma):
+    """The store of the primitive formats (spike_25d.cu:499-511): rho as is, v = m / max(rho,
+    smallr), p = max((gamma - 1)(E - rho v^2 / 2), smallp), in float32."""
+    U = np.asarray(U, dtype=np.float32)
+    ir = _F(1) / np.maximum(U[0], _F(smallr))
+    v = U[1:4] * ir
+    ke = _F(0.5) * U[0] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    p = np.maximum(_F(gamma - 1.0) * (U[4] - ke), _F(smallp))
+    return np.stack([U[0], v[0], v[1], v[2], p])
 
 
 def twiddles(g: int):

@@ -360,17 +360,22 @@ GLM_HD float flip_sign(float x, float ref)
     const unsigned m = __builtin_bit_cast(unsigned, ref) & 0x80000000u;
     return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) ^ m);
 }
+// The sign is applied by multiplying with s = +-1 rather than by flipping the sign bit: a product is
+// a canonical float, so the min/max that follow need no NaN-quieting of their operands (a bit
+// operation's result would get one).  Multiplying by +-1 is exact, so the values are the same.
 GLM_HD double moncen_half(double dl, double dr)
 {
     const double sum = dl + dr;
-    const double a = flip_sign(dl, sum), b = flip_sign(dr, sum);
-    return flip_sign(__builtin_fmax(__builtin_fmin(__builtin_fmin(a, b), 0.25 * __builtin_fabs(sum)), 0.0), sum);
+    const double sg = __builtin_copysign(1.0, sum);
+    const double a = sg * dl, b = sg * dr;
+    return sg * __builtin_fmax(__builtin_fmin(__builtin_fmin(a, b), 0.25 * __builtin_fabs(sum)), 0.0);
 }
 GLM_HD float moncen_half(float dl, float dr)
 {
     const float sum = dl + dr;
-    const float a = flip_sign(dl, sum), b = flip_sign(dr, sum);
-    return flip_sign(__builtin_fmaxf(__builtin_fminf(__builtin_fminf(a, b), 0.25f * __builtin_fabsf(sum)), 0.0f), sum);
+    const float sg = __builtin_copysignf(1.0f, sum);
+    const float a = sg * dl, b = sg * dr;
+    return sg * __builtin_fmaxf(__builtin_fminf(__builtin_fminf(a, b), 0.25f * __builtin_fabsf(sum)), 0.0f);
 }
 template <typename T>
 GLM_HD void ppm_offsets(T qm, T q0, T qp, T& dl, T& dr)
@@ -379,10 +384,12 @@ GLM_HD void ppm_offsets(T qm, T q0, T qp, T& dl, T& dr)
     T curve = (qm - T(2) * q0 + qp) * T(1.0 / 12.0);
     T a = slope - curve, b = slope + curve;
     // s = sign(slope): where a and b share a sign it is theirs (a + b = 2 slope up to rounding, which
-    // keeps signs), and where they do not both offsets are 0 whatever s is
-    T sa = flip_sign(a, slope), sb = flip_sign(b, slope);
-    dl = flip_sign(tmax(tmin(sa, sb + sb), T(0)), slope);
-    dr = flip_sign(tmax(tmin(sb, sa + sa), T(0)), slope);
+    // keeps signs), and where they do not both offsets are 0 whatever s is.  s is applied as a
+    // factor +-1 (exact), whose products need no NaN-quieting before the min/max.
+    const T sg = tcopysign(T(1), slope);
+    T sa = sg * a, sb = sg * b;
+    dl = sg * tmax(tmin(sa, sb + sb), T(0));
+    dr = sg * tmax(tmin(sb, sa + sa), T(0));
 }
 
 // spike_mhd.cu:164-172 ppm1 (both edges).

@@ -818,7 +818,8 @@ def run_hydro_compact(N=480, iters=30, warmup=3, dtdx=0.05):
 # an f16 tile (BENCHMARKS.md:189-195).  Keys: (march, reconstruction, species).
 REF_MATRIX_A6000 = {("glm", "plm", 0): 3175, ("glm", "plm", 2): 2733, ("glm", "ppm", 0): 2064, ("glm", "ppm", 2): 1410,
                     ("ct", "plm", 0): 1255, ("ct", "plm", 2): 1011, ("ct", "ppm", 0): 752, ("ct", "ppm", 2): 558,
-                    ("hydro", "plm", 0): 6865, ("hydro", "plm", 2): 5082}
+                    ("hydro", "plm", 0): 6865, ("hydro", "plm", 2): 5082, ("hydro", "ppm", 0): 3995,
+                    ("hydro", "ppm", 2): 3822}
 
 
 def run_reference_matrix(N=480, iters=30, warmup=3):
@@ -827,7 +828,7 @@ def run_reference_matrix(N=480, iters=30, warmup=3):
     :313-315), fixed dt/dx 0.02, CH = GLMFAC = 1, 3 untimed and 30 timed steps, FP32 state.  The
     GLM and CT marches run the MarchBridge scheme (normal-only Hancock, MonCen or the parabolic-edge
     PPM, HLL, clamped faces; march_args); the hydro march the build_march.sh scheme at dt/dx 0.05
-    (spike_25d.cu:180).  Each configuration takes the fastest z-chunk of a short sweep (the
+    (spike_25d.cu:180), its PPM cells the -DPPM1D parabolic edges (:467-483).  Each configuration takes the fastest z-chunk of a short sweep (the
     reference reports its best tile per configuration)."""
     import torch
     from glmmhdturb_amd import Params, Solver, hydro_args, march_args
@@ -889,7 +890,8 @@ def run_reference_matrix(N=480, iters=30, warmup=3):
                 else:
                     s.ct_step(state["q"], state["o"], a)
             else:
-                s.hydro_step(state["q"], state["o"], hydro_args(0.05, 1.4, zchunk=zc), nsp=nsp, cma=bool(nsp))
+                s.hydro_step(state["q"], state["o"], hydro_args(0.05, 1.4, zchunk=zc, ppm=rc == "ppm"), nsp=nsp,
+                             cma=bool(nsp))
             state["q"], state["o"] = state["o"], state["q"]
 
         best = None
@@ -909,7 +911,6 @@ def run_reference_matrix(N=480, iters=30, warmup=3):
         torch.cuda.empty_cache()
     s.close()
     out["rows"] = rows
-    out["not_built"] = ["hydro_ppm_0sp", "hydro_ppm_2sp"]
     return out
 
 

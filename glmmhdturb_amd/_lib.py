@@ -28,7 +28,7 @@ EXPORTS = (
     "glm_params_default", "glm_create", "glm_destroy", "glm_set_stream", "glm_step", "glm_step_species", "glm_cfl_max",
     "glm_ctot_max", "glm_fast_max", "glm_diagnostics", "glm_ic", "glm_ou_setup", "glm_ou_advance", "glm_ou_afield",
     "glm_ou_steps", "glm_ou_copy_afield", "glm_ct_step", "glm_ct_step_species", "glm_ct_step_passes", "glm_ct_step_tiles", "glm_ct_ctot_max", "glm_ct_divb_max",
-    "glm_hydro_step", "glm_hydro_step_scalars", "glm_hydro_step_compact", "glm_last_error", "glm_error_string",
+    "glm_hydro_step", "glm_hydro_step_scalars", "glm_hydro_step_compact", "glm_hydro_step_donor", "glm_last_error", "glm_error_string",
     "glm_kernel_info",
     "glm_ens_create", "glm_ens_destroy", "glm_ens_stream", "glm_ens_reset", "glm_ens_step", "glm_ens_graph",
     "glm_ens_sync", "glm_ens_sweeps", "glm_ens_current", "glm_ens_log", "glm_ens_timing", "glm_ens_configure",
@@ -103,6 +103,7 @@ def load(path: str | None = None):
         "glm_hydro_step": (i, [vp, pp, pp, C.POINTER(GLMParams)]),
         "glm_hydro_step_scalars": (i, [vp, pp, pp, C.POINTER(GLMParams), i, i]),
         "glm_hydro_step_compact": (i, [vp, pp, pp, C.POINTER(GLMParams), i]),
+        "glm_hydro_step_donor": (i, [vp, pp, pp, C.POINTER(GLMParams), i, i]),
         "glm_last_error": (i, []), "glm_error_string": (C.c_char_p, [i]),
         "glm_kernel_info": (i, [i, i, i, i, i, C.POINTER(i), C.POINTER(i)]),
         "glm_ens_create": (vp, [i, C.POINTER(GLMEnsBox), C.POINTER(GLMParams), i]),

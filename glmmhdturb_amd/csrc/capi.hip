@@ -555,10 +555,12 @@ static int ct_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_
         if (ctx->prec == 8) HIPCHK(launch_ct_tiles<double>(q, o, h, ctx->stream));
         else HIPCHK(launch_ct_tiles<float>(q, o, h, ctx->stream));
     } else if (mode == CT_MARCH) {
-        const int key = 1000 + p->riemann + 10 * nsp;
+        const int ppm = p->recon == GLM_RECON_PPM_SPIKE;
+        const int key = 1000 + p->riemann + 10 * nsp + 100 * ppm;
         if (ctx->ct_slots_key != key) {
             int dev = 0, cus = 0, occ = 0;
-            const void* fn = ctx->prec == 8 ? ct_march_ptr<double>(p->riemann, nsp) : ct_march_ptr<float>(p->riemann, nsp);
+            const void* fn = ctx->prec == 8 ? ct_march_ptr<double>(p->riemann, nsp, p->recon)
+                                            : ct_march_ptr<float>(p->riemann, nsp, p->recon);
             if (!fn) return set_err(GLM_ERR_ARG, "glm_ct_step: unknown Riemann solver");
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -632,7 +634,7 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out)
 // The hydro march (spike_25d.cu:383-516 as build_march.sh builds it), with nsp = 0 or 2 passive
 // scalars (-DSCALARS) whose flux rides the mass flux when cma is set (-DCMA).
 static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma,
-                           const char* who, int fmt = 0)
+                           const char* who, int fmt = 0, int donor = 0)
 {
     char msg[160];
     if (!ctx || !q || !o || !p) {
@@ -668,11 +670,18 @@ static int hydro_step_impl(glm_ctx* ctx, void* const* q, void* const* o, const g
     h.nsp = nsp;
     h.cma = nsp > 0 && cma;
     h.fmt = fmt;
-    const int key = 2000 + h.fallback + 2 * nsp + 8 * h.cma + 16 * fmt;
+    // the scheme variant: donor-cell seams, or the parabolic edges when recon is PPM_SPIKE
+    h.scheme = donor ? 2 : p->recon == GLM_RECON_PPM_SPIKE ? 1 : 0;
+    if (h.scheme != 0 && fmt != 0) {
+        snprintf(msg, sizeof msg, "%s: the PPM1D and donor variants take conserved planes", who);
+        return set_err(GLM_ERR_ARG, msg);
+    }
+    if (h.scheme != 0) h.fallback = 0;   // those builds solve HLL without the fallback (spike_25d.cu:445-483)
+    const int key = 2000 + h.fallback + 2 * nsp + 8 * h.cma + 16 * fmt + 128 * h.scheme;
     if (ctx->hydro_slots_key != key) {
         int dev = 0, cus = 0, occ = 0;
-        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback, nsp, h.cma, fmt)
-                                        : hydro_march_ptr<float>(h.fallback, nsp, h.cma, fmt);
+        const void* fn = ctx->prec == 8 ? hydro_march_ptr<double>(h.fallback, nsp, h.cma, fmt, h.scheme)
+                                        : hydro_march_ptr<float>(h.fallback, nsp, h.cma, fmt, h.scheme);
         if (!fn) return set_err(GLM_ERR_ARG, "hydro step: no kernel for this format and precision");
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -697,6 +706,11 @@ int glm_hydro_step_compact(glm_ctx* ctx, void* const* q, void* const* o, const g
     if (ctx && ctx->prec != 4) return set_err(GLM_ERR_ARG, "glm_hydro_step_compact: the compact formats need an FP32 context");
     const int nsp = format == GLM_HYDRO_U16_LOG2_SPECIES ? 2 : 0;
     return hydro_step_impl(ctx, q, o, p, nsp, nsp > 0, "glm_hydro_step_compact", format);
+}
+
+int glm_hydro_step_donor(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma)
+{
+    return hydro_step_impl(ctx, q, o, p, nsp, cma, "glm_hydro_step_donor", 0, 1);
 }
 
 int glm_hydro_step_scalars(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma)

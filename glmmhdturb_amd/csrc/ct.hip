@@ -77,19 +77,23 @@ const void* ct_tile_ptr()
     return (const void*)&ct_tile_kernel<T>;
 }
 
-template <typename T>
-const void* ct_march_ptr(int riemann, int nsp)
+template <typename T, int NS, int RC>
+static const void* ct_march_fn(int riemann)
 {
-    if (nsp == 2) {
-        if (riemann == RIEM_HLLD) return (const void*)&ct_march_kernel<T, RIEM_HLLD, 2>;
-        if (riemann == RIEM_HLL) return (const void*)&ct_march_kernel<T, RIEM_HLL, 2>;
-        if (riemann == RIEM_LLF) return (const void*)&ct_march_kernel<T, RIEM_LLF, 2>;
-        return nullptr;
-    }
-    if (nsp != 0) return nullptr;
-    if (riemann == RIEM_HLLD) return (const void*)&ct_march_kernel<T, RIEM_HLLD>;
-    if (riemann == RIEM_HLL) return (const void*)&ct_march_kernel<T, RIEM_HLL>;
-    if (riemann == RIEM_LLF) return (const void*)&ct_march_kernel<T, RIEM_LLF>;
+    if (riemann == RIEM_HLLD) return (const void*)&ct_march_kernel<T, RIEM_HLLD, NS, RC>;
+    if (riemann == RIEM_HLL) return (const void*)&ct_march_kernel<T, RIEM_HLL, NS, RC>;
+    if (riemann == RIEM_LLF) return (const void*)&ct_march_kernel<T, RIEM_LLF, NS, RC>;
+    return nullptr;
+}
+
+// The march kernel for a Riemann solver, species count and reconstruction (RECON_PPM_SPIKE the
+// parabolic edges, any other value MonCen PLM), or null.
+template <typename T>
+const void* ct_march_ptr(int riemann, int nsp, int recon)
+{
+    const bool ppm = recon == RECON_PPM_SPIKE;
+    if (nsp == 2) return ppm ? ct_march_fn<T, 2, RECON_PPM_SPIKE>(riemann) : ct_march_fn<T, 2, RECON_PLM>(riemann);
+    if (nsp == 0) return ppm ? ct_march_fn<T, 0, RECON_PPM_SPIKE>(riemann) : ct_march_fn<T, 0, RECON_PLM>(riemann);
     return nullptr;
 }
 
@@ -112,17 +116,11 @@ hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, 
     a.face_slope = h.face_slope; a.recon = h.recon; a.update_floor = h.update_floor;
     a.do_turb = h.do_turb; a.tgs = h.tgs;
     a.afield = (const T*)h.afield;
-    dim3 g((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), b(CM_NT);
-    if (h.nsp == 2) {
-        if (h.riemann == RIEM_HLLD) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLLD, 2>), g, b, 0, st, a);
-        else if (h.riemann == RIEM_HLL) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLL, 2>), g, b, 0, st, a);
-        else if (h.riemann == RIEM_LLF) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_LLF, 2>), g, b, 0, st, a);
-        else return hipErrorInvalidValue;
-    } else if (h.riemann == RIEM_HLLD) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLLD>), g, b, 0, st, a);
-    else if (h.riemann == RIEM_HLL) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_HLL>), g, b, 0, st, a);
-    else if (h.riemann == RIEM_LLF) hipLaunchKernelGGL((ct_march_kernel<T, RIEM_LLF>), g, b, 0, st, a);
-    else return hipErrorInvalidValue;
-    hipError_t e = hipGetLastError();
+    const void* fn = ct_march_ptr<T>(h.riemann, h.nsp, h.recon);
+    if (!fn) return hipErrorInvalidValue;
+    void* args[] = {&a};
+    hipError_t e = hipLaunchKernel(fn, dim3((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), dim3(CM_NT), args, 0, st);
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess || !h.want_cfl) return e;
     Ptr9<const T> o9;
     for (int v = 0; v < 9; ++v) o9.p[v] = v < 8 ? (const T*)o[v] : nullptr;
@@ -163,7 +161,7 @@ const void* ct_kernel_ptr()
                                           unsigned long long*, hipStream_t);                                    \
     template hipError_t launch_ct_divb<T>(void* const*, int, int, int, double, unsigned long long*, hipStream_t); \
     template const void* ct_kernel_ptr<T>();                                                                    \
-    template const void* ct_march_ptr<T>(int, int);                                                             \
+    template const void* ct_march_ptr<T>(int, int, int);                                                        \
     template hipError_t launch_ct_march<T>(void* const*, void* const*, const CTArgsHost&, int, hipStream_t);   \
     template hipError_t launch_ct_tiles<T>(void* const*, void* const*, const CTArgsHost&, hipStream_t);        \
     template const void* ct_tile_ptr<T>();

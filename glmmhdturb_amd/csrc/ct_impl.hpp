@@ -117,7 +117,9 @@ GLM_DEV void ct_hanc1d(const T q[8], const T hs[8], T mh[8], const A& a)
 // NV = 8 + passive species fractions X_s (spike_ctm.cu -DNSP): their slopes and PPM edges are the
 // primitives', and the normal-only half step advects them, X - (dt/dx / 2) un s with s = 2 hs the
 // slope and un the cell's normal velocity (spike_ctm.cu:178-181).
-template <typename T, int D, int NV = 8, typename A>
+//
+// RC: the reconstruction fixed at compile time (RECON_PLM or RECON_PPM_SPIKE), or < 0 for a.recon.
+template <typename T, int D, int NV = 8, int RC = -1, typename A>
 GLM_DEV void ct_cell_states(const T l[NV], const T c[NV], const T r[NV], const A& a, T lo[NV], T hi[NV])
 {
     T hs[NV], mh[NV];
@@ -127,13 +129,14 @@ GLM_DEV void ct_cell_states(const T l[NV], const T c[NV], const T r[NV], const A
     ct_hanc1d<T, D>(c, hs, mh, a);
 #pragma unroll
     for (int k = 8; k < NV; ++k) mh[k] = c[k] - a.half_dtdx * c[1 + D] * (T(2) * hs[k]);
-    if (a.recon == RECON_PPM_SPIKE) {
+    const bool ppm = RC < 0 ? a.recon == RECON_PPM_SPIKE : RC == RECON_PPM_SPIKE;
+    if (ppm) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            T el, eh;
-            ppm_edges_spike(l[k], c[k], r[k], el, eh);
-            hi[k] = eh + mh[k] - c[k];
-            lo[k] = el + mh[k] - c[k];
+        for (int k = 0; k < NV; ++k) {   // the parabola's edges c - dl, c + dr plus mh - c
+            T dl, dr;
+            ppm_offsets_spike(l[k], c[k], r[k], dl, dr);
+            hi[k] = mh[k] + dr;
+            lo[k] = mh[k] - dl;
         }
     } else {
 #pragma unroll

@@ -30,7 +30,7 @@ const void* ct_kernel_ptr();
 template <typename T>
 hipError_t launch_ct_march(void* const* q, void* const* o, const CTArgsHost& h, int slots, hipStream_t st);
 template <typename T>
-const void* ct_march_ptr(int riemann, int nsp = 0);
+const void* ct_march_ptr(int riemann, int nsp = 0, int recon = 0);
 // The same step as one kernel over staged 3-D tiles (ct_tile.hpp) and that kernel (resource queries).
 template <typename T>
 hipError_t launch_ct_tiles(void* const* q, void* const* o, const CTArgsHost& h, hipStream_t st);

@@ -53,6 +53,17 @@ static_assert(CM_NT == CT_MARCH_THREADS, "launch width of the CT march");
 // in FP32 two workgroups with no spills run 1.2-1.4x faster than three held to 168 registers
 // (480^3, measured).
 #define GLM_CT_WAVES_PER_EU_NS(T, NS) ((NS) > 0 ? (sizeof(T) == 8 ? 1 : 2) : GLM_CT_WAVES_PER_EU(T))
+// The parabolic-edge builds without species: FP64 at one wave per SIMD (all 512 registers; at two
+// they spill ~300 bytes per lane and run 8-12% slower, 320^3), FP32 at three like PLM (two: no gain).
+#ifndef GLM_CT_PPM_WAVES_F64
+#define GLM_CT_PPM_WAVES_F64 1
+#endif
+#ifndef GLM_CT_PPM_WAVES_F32
+#define GLM_CT_PPM_WAVES_F32 3
+#endif
+#define GLM_CT_WAVES_RC(T, NS, RC) \
+    ((NS) == 0 && (RC) == RECON_PPM_SPIKE ? (sizeof(T) == 8 ? GLM_CT_PPM_WAVES_F64 : GLM_CT_PPM_WAVES_F32) \
+                                          : GLM_CT_WAVES_PER_EU_NS(T, NS))
 
 template <typename T>
 struct CTMArgs {
@@ -146,9 +157,11 @@ inline void ct_march_tiling(int nx, int ny, int nz, int zchunk_req, int slots, i
 }
 
 // One step of the CT state with NS passive species (spike_ctm.cu -DNSP: their fluxes ride the
-// mass flux and are updated with the hydro planes): workgroup w marches tile xcd_block(w).
-template <typename T, int RIEM, int NS = 0>
-__global__ void __launch_bounds__(CM_NT) __attribute__((amdgpu_waves_per_eu(GLM_CT_WAVES_PER_EU_NS(T, NS))))
+// mass flux and are updated with the hydro planes): workgroup w marches tile xcd_block(w).  RC: the
+// reconstruction (RECON_PLM or RECON_PPM_SPIKE), a template argument so that each build holds the
+// registers of its own path only.
+template <typename T, int RIEM, int NS = 0, int RC = RECON_PLM>
+__global__ void __launch_bounds__(CM_NT) __attribute__((amdgpu_waves_per_eu(GLM_CT_WAVES_RC(T, NS, RC))))
 ct_march_kernel(CTMArgs<T> a)
 {
     constexpr int NV = 8 + NS, NH = 5 + NS;   // values per cell; hydro-updated values (species at 5..)
@@ -229,12 +242,12 @@ ct_march_kernel(CTMArgs<T> a)
             T l[NV], c0[NV], r[NV], hi[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { l[v] = P[v][lo - 1]; c0[v] = P[v][lo]; r[v] = P[v][lo + 1]; }
-            ct_cell_states<T, 0, NV>(l, c0, r, a, xlo, hi);
+            ct_cell_states<T, 0, NV, RC>(l, c0, r, a, xlo, hi);
 #pragma unroll
             for (int v = 0; v < NV; ++v) XB[v][li] = hi[v];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { l[v] = P[v][lo - CM_PX]; r[v] = P[v][lo + CM_PX]; }
-            ct_cell_states<T, 1, NV>(l, c0, r, a, ylo, hi);
+            ct_cell_states<T, 1, NV, RC>(l, c0, r, a, ylo, hi);
 #pragma unroll
             for (int v = 0; v < NV; ++v) YB[v][li] = hi[v];
         }
@@ -275,7 +288,7 @@ ct_march_kernel(CTMArgs<T> a)
 #pragma unroll
             for (int v = 0; v < NV; ++v) c0[v] = P[v][lo];
             T zlo[NV], zhi[NV];
-            ct_cell_states<T, 2, NV>(pm, c0, pp, a, zlo, zhi);
+            ct_cell_states<T, 2, NV, RC>(pm, c0, pp, a, zlo, zhi);
             if (need_z && !lead) ct_riemann<T, RIEM, 2, NV>(zprev, zlo, bnz, a, fz);
 #pragma unroll
             for (int v = 0; v < NV; ++v) zprev[v] = zhi[v];

@@ -115,7 +115,8 @@ GLM_DEV void hy_c2p(const T c[NV], T q[NV], const HydroArgs<T>& a)
     for (int v = 5; v < NV; ++v) q[v] = LG ? c[v] : c[v] * ir;
 }
 
-// spike_25d.cu:207-213 moncen (full slope)
+// spike_25d.cu:207-213 moncen (full slope).  The selects stay: a select-free form (the sign of dc
+// applied as a factor) measured 2-3% slower in this kernel (480^3 FP32).
 template <typename T>
 GLM_DEV T hy_moncen(T l, T m, T r)
 {
@@ -207,40 +208,72 @@ GLM_DEV void hy_slopes(const T l[NV], const T m[NV], const T r[NV], T s[NV])
     for (int v = 0; v < NV; ++v) s[v] = hy_moncen(l[v], m[v], r[v]);
 }
 
+// Scheme variants of the march (spike_25d.cu build flags): HY_PLM the light march (-DHANCOCK1D:
+// MonCen slopes, normal-only Hancock half step, edges clamped at SMALLR/SMALLP, :462-466); HY_PPM1D
+// the parabolic-edge variant (-DPPM1D, :467-483: the monotonized 3-point parabola's edge plus the
+// half step's change mh - c, no clamp, HLL without the fallback); HY_DONOR (-DDONOR, :438-454): the
+// light march with donor-cell HLL of the two cell states at the x and y faces of the 32 x 8 tiles
+// (a face at x = i - 1/2 with i mod 32 = 0, at y = j - 1/2 with j mod 8 = 0; z has none), HLL
+// without the fallback.
+enum { HY_PLM = 0, HY_PPM1D = 1, HY_DONOR = 2 };
+
+// The upper (SIGN = +1) or lower (-1) face state of cell c with neighbours l, r, slope sl and half
+// step mh: the light march's edge mh +- s/2 clamped at SMALLR/SMALLP, or the PPM1D parabola's edge
+// plus mh - c, unclamped (spike_25d.cu:476-479).
+template <typename T, int NV, int SV, int SIGN>
+GLM_DEV void hy_state(const T l[NV], const T c[NV], const T r[NV], const T sl[NV], const T mh[NV], T e[NV],
+                      const HydroArgs<T>& a)
+{
+    if (SV == HY_PPM1D) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T dl, dr;   // the edge is c - dl or c + dr; plus mh - c
+            ppm_offsets_spike(l[v], c[v], r[v], dl, dr);
+            e[v] = SIGN > 0 ? mh[v] + dr : mh[v] - dl;
+        }
+    } else {
+        hy_edge<T, NV>(mh, sl, T(SIGN), e, a);
+    }
+}
+
 // Flux through the face between cells c1 and c2 of the stencil c0 c1 | c2 c3 along D.
-template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false>
+template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false, int SV = HY_PLM>
 GLM_DEV void hy_face(const T c0[NV], const T c1[NV], const T c2[NV], const T c3[NV], T F[NV], const HydroArgs<T>& a)
 {
-    T sl[NV], sr[NV], mh[NV], L[NV], R[NV];
-    hy_slopes<T, NV>(c0, c1, c2, sl);
-    hy_slopes<T, NV>(c1, c2, c3, sr);
-    hy_hanc1d<T, D, NV>(c1, sl, mh, a);
-    hy_edge<T, NV>(mh, sl, T(1), L, a);
-    hy_hanc1d<T, D, NV>(c2, sr, mh, a);
-    hy_edge<T, NV>(mh, sr, T(-1), R, a);
-    hy_riemann<T, D, FB, NV, CMA, LG>(L, R, F, a);
+    T s[NV], mh[NV], L[NV], R[NV];
+    hy_slopes<T, NV>(c0, c1, c2, s);
+    hy_hanc1d<T, D, NV>(c1, s, mh, a);
+    hy_state<T, NV, SV, 1>(c0, c1, c2, s, mh, L, a);
+    hy_slopes<T, NV>(c1, c2, c3, s);
+    hy_hanc1d<T, D, NV>(c2, s, mh, a);
+    hy_state<T, NV, SV, -1>(c1, c2, c3, s, mh, R, a);
+    hy_riemann<T, D, FB && SV == HY_PLM, NV, CMA, LG>(L, R, F, a);
 }
 
 // div += F(+1/2) - F(-1/2) along D for the cell c0 of the stencil m2 m1 c0 p1 p2: the centre
 // cell's slope and half step are shared by its two faces (spike_25d.cu:462-466 forms them once
-// per face).
-template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false>
+// per face).  lob / hib: the lower / upper face is a tile seam (HY_DONOR: donor-cell HLL of the
+// two cell states, :445-448).
+template <typename T, int D, bool FB, int NV, bool CMA, bool LG = false, int SV = HY_PLM>
 GLM_DEV void hy_dir(const T m2[NV], const T m1[NV], const T c0[NV], const T p1[NV], const T p2[NV], T div[NV],
-                    const HydroArgs<T>& a)
+                    const HydroArgs<T>& a, bool lob = false, bool hib = false)
 {
+    constexpr bool fb = FB && SV == HY_PLM;
     T s[NV], mh[NV], L[NV], R[NV], Fm[NV], Fp[NV];
     hy_slopes<T, NV>(m2, m1, c0, s);
     hy_hanc1d<T, D, NV>(m1, s, mh, a);
-    hy_edge<T, NV>(mh, s, T(1), L, a);
+    hy_state<T, NV, SV, 1>(m2, m1, c0, s, mh, L, a);
     hy_slopes<T, NV>(m1, c0, p1, s);
     hy_hanc1d<T, D, NV>(c0, s, mh, a);
-    hy_edge<T, NV>(mh, s, T(-1), R, a);
-    hy_riemann<T, D, FB, NV, CMA, LG>(L, R, Fm, a);
-    hy_edge<T, NV>(mh, s, T(1), L, a);
+    hy_state<T, NV, SV, -1>(m1, c0, p1, s, mh, R, a);
+    if (SV == HY_DONOR && lob) hy_riemann<T, D, false, NV, CMA, LG>(m1, c0, Fm, a);
+    else hy_riemann<T, D, fb, NV, CMA, LG>(L, R, Fm, a);
+    hy_state<T, NV, SV, 1>(m1, c0, p1, s, mh, L, a);
     hy_slopes<T, NV>(c0, p1, p2, s);
     hy_hanc1d<T, D, NV>(p1, s, mh, a);
-    hy_edge<T, NV>(mh, s, T(-1), R, a);
-    hy_riemann<T, D, FB, NV, CMA, LG>(L, R, Fp, a);
+    hy_state<T, NV, SV, -1>(c0, p1, p2, s, mh, R, a);
+    if (SV == HY_DONOR && hib) hy_riemann<T, D, false, NV, CMA, LG>(c0, p1, Fp, a);
+    else hy_riemann<T, D, fb, NV, CMA, LG>(L, R, Fp, a);
 #pragma unroll
     for (int v = 0; v < NV; ++v) div[v] += Fp[v] - Fm[v];
 }
@@ -276,10 +309,19 @@ inline void hydro_tiling(int nx, int ny, int nz, int zchunk_req, int slots, int&
 // (FP64: 2 workgroups per CU of 69 KB, 1 of 97 KB with scalars; FP32: 4 of 35 KB), except FP32 with
 // scalars, where 3 workgroups of 48 KB would hold it to 168 registers and spill: 2 workgroups and
 // no spills run 1.4x faster (480^3, measured).
-#define GLM_HYDRO_WAVES_PER_EU(T, NSP) (sizeof(T) == 8 ? ((NSP) ? 1 : 2) : ((NSP) ? 2 : 4))
+// The PPM1D and DONOR builds in FP32: 3 waves per SIMD without scalars (4 would hold them to 128
+// registers and spill), 1 with scalars (PPM1D with scalars at 1: 20.8 Gcell/s, at 2: 19.9; 480^3).
+#ifndef GLM_HYDRO_SV_WAVES0
+#define GLM_HYDRO_SV_WAVES0 3
+#endif
+#ifndef GLM_HYDRO_SV_WAVES2
+#define GLM_HYDRO_SV_WAVES2 1
+#endif
+#define GLM_HYDRO_WAVES_PER_EU(T, NSP, SV) \
+    (sizeof(T) == 8 ? ((NSP) ? 1 : 2) : (SV) == HY_PLM ? ((NSP) ? 2 : 4) : ((NSP) ? GLM_HYDRO_SV_WAVES2 : GLM_HYDRO_SV_WAVES0))
 
-template <typename T, bool FB, int NSP, bool CMA, int FMT = HY_CONS>
-__global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T, NSP))))
+template <typename T, bool FB, int NSP, bool CMA, int FMT = HY_CONS, int SV = HY_PLM>
+__global__ void __launch_bounds__(HM_NT) __attribute__((amdgpu_waves_per_eu(GLM_HYDRO_WAVES_PER_EU(T, NSP, SV))))
 hydro_march_kernel(HydroArgs<T> a)
 {
     constexpr int NV = 5 + NSP;
@@ -403,16 +445,20 @@ hydro_march_kernel(HydroArgs<T> a)
                 m2[v] = P[b][v][li - 2]; m1[v] = P[b][v][li - 1]; c0[v] = P[b][v][li];
                 p1[v] = P[b][v][li + 1]; p2[v] = P[b][v][li + 2];
             }
-            hy_dir<T, 0, FB, NV, CMA, LG>(m2, m1, c0, p1, p2, div, a);
+            // tile seams of HY_DONOR: faces at x = i - 1/2 with i mod 32 = 0, y = j - 1/2 with j mod 8 = 0
+            // (the periodic face after the last cell included)
+            const bool lox = SV == HY_DONOR && gx % HM_OX == 0, hix = SV == HY_DONOR && ((gx + 1) % a.nx) % HM_OX == 0;
+            const bool loy = SV == HY_DONOR && gy % HM_OY == 0, hiy = SV == HY_DONOR && ((gy + 1) % a.ny) % HM_OY == 0;
+            hy_dir<T, 0, FB, NV, CMA, LG, SV>(m2, m1, c0, p1, p2, div, a, lox, hix);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 m2[v] = P[b][v][li - 2 * HM_LX]; m1[v] = P[b][v][li - HM_LX];
                 p1[v] = P[b][v][li + HM_LX]; p2[v] = P[b][v][li + 2 * HM_LX];
             }
-            hy_dir<T, 1, FB, NV, CMA, LG>(m2, m1, c0, p1, p2, div, a);
+            hy_dir<T, 1, FB, NV, CMA, LG, SV>(m2, m1, c0, p1, p2, div, a, loy, hiy);
         }
         T Fp[NV];
-        hy_face<T, 2, FB, NV, CMA, LG>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
+        hy_face<T, 2, FB, NV, CMA, LG, SV>(W[0], W[1], W[2], W[3], Fp, a);   // face k+1/2
         if (upd) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) div[v] += Fp[v] - Fz[v];

@@ -392,19 +392,33 @@ GLM_HD void ppm_offsets(T qm, T q0, T qp, T& dl, T& dr)
     dr = sg * tmax(tmin(sb, sa + sa), T(0));
 }
 
+// spike_mhd.cu:164-172 ppm1 as edge offsets dl = q0 - lo, dr = hi - q0, in min/max form.  The
+// unlimited edges are ql = q0 - a, qr = q0 + b with a = d + cu, b = d - cu (d = (qp - qm)/4,
+// cu = (qm - 2 q0 + qp)/6).  ppm1's tests, with dqe = a + b and d6 = 3 (a - b): (flat)
+// (qr - q0)(q0 - ql) = a b <= 0 -> both 0; dqe d6 > dqe^2 <=> (a + b)(a - 2b) > 0 -> ql = 3 q0 - 2 qr,
+// i.e. a = 2b; dqe d6 < -dqe^2 <=> (a + b)(2a - b) < 0 -> qr = 3 q0 - 2 ql, i.e. b = 2a (the two
+// exclude each other when a and b share a sign, so the order of ppm1's sequential ifs does not
+// matter).  With s the sign of d (that of a + b = 2d where a and b share a sign):
+// a' = s max(0, min(s a, 2 s b)) and b' = s max(0, min(s b, 2 s a)), the same limited parabola up
+// to rounding (the limiter is continuous at each switch) and with no branch or select.
+template <typename T>
+GLM_HD void ppm_offsets_spike(T qm, T q0, T qp, T& dl, T& dr)
+{
+    const T d = T(0.25) * (qp - qm), cu = (qm - T(2) * q0 + qp) * (T(1) / T(6));
+    const T a = d + cu, b = d - cu;
+    const T sg = tcopysign(T(1), d);
+    const T sa = sg * a, sb = sg * b;
+    dl = sg * tmax(tmin(sa, sb + sb), T(0));
+    dr = sg * tmax(tmin(sb, sa + sa), T(0));
+}
 // spike_mhd.cu:164-172 ppm1 (both edges).
 template <typename T>
 GLM_HD void ppm_edges_spike(T qm, T q0, T qp, T& lo, T& hi)
 {
-    T d = T(0.25) * (qp - qm), cu = (qm - T(2) * q0 + qp) * (T(1) / T(6));
-    T qr = q0 + d - cu, ql = q0 - d - cu;
-    if ((qr - q0) * (q0 - ql) <= T(0)) { qr = q0; ql = q0; }
-    else {
-        T dqe = qr - ql, d6 = T(6) * (q0 - T(0.5) * (ql + qr));
-        if (dqe * d6 > dqe * dqe) ql = T(3) * q0 - T(2) * qr;
-        if (dqe * d6 < -dqe * dqe) qr = T(3) * q0 - T(2) * ql;
-    }
-    lo = ql; hi = qr;
+    T dl, dr;
+    ppm_offsets_spike(qm, q0, qp, dl, dr);
+    lo = q0 - dl;
+    hi = q0 + dr;
 }
 
 // glmmhd_turb.jl:926 spj: the correctly rounded quotient hi / lo exceeds 2, decided without the

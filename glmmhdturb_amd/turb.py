@@ -136,13 +136,15 @@ def step_args(p: Params, dt: float, t: float, *, do_turb=False, riemann="hll", r
 
 
 def hydro_args(dtdx: float, gamma: float = 1.4, *, smallr: float = 1e-6, smallp: float = 1e-6,
-               fallback: tuple[float, float] | None = None, zchunk: int = 0) -> GLMParams:
+               fallback: tuple[float, float] | None = None, zchunk: int = 0, ppm: bool = False) -> GLMParams:
     """Parameters of the 5-variable hydro march (spike_25d.cu:180 constants: gamma 1.4,
     SMALLR = SMALLP = 1e-6).  fallback = (dmin, pmin) selects HLL with the LLF fallback below
-    those floors (-DLLF_FALLBACK, (1e-3, 1e-3) in spike_25d.cu:339)."""
+    those floors (-DLLF_FALLBACK, (1e-3, 1e-3) in spike_25d.cu:339); ppm the parabolic-edge build
+    (-DPPM1D, :467-483, which takes no fallback)."""
     a = GLMParams()
     _lib.load().glm_params_default(C.byref(a))
     a.gamma, a.dt, a.dx, a.smallr, a.pfl = gamma, dtdx, 1.0, smallr, smallp
+    a.recon = _lib.RECON["ppm_spike" if ppm else "plm"]
     a.llf_dmin, a.llf_pmin = fallback if fallback is not None else (0.0, 0.0)
     a.zchunk, a.want_cfl, a.do_turb = int(zchunk), 0, 0
     return a
@@ -420,13 +422,16 @@ class Solver:
             args.zchunk = self.ct_zchunk
         check(self.lib.glm_ct_step_species(self.ctx, qa, oa, C.byref(args), int(face_slope), int(nsp)))
 
-    def hydro_step(self, qold, qnew, args: GLMParams, nsp: int = 0, cma: bool = False):
+    def hydro_step(self, qold, qnew, args: GLMParams, nsp: int = 0, cma: bool = False, donor: bool = False):
         """One step of the hydro march (spike_25d.cu march, build_march.sh scheme) on
         (5 + nsp, nz, ny, nx) conserved states [rho, mx, my, mz, E, rho*s0, rho*s1]; nsp = 2 adds
-        the passive scalars of -DSCALARS, cma their -DCMA flux; see hydro_args."""
+        the passive scalars of -DSCALARS, cma their -DCMA flux, donor the donor-cell tile seams of
+        -DDONOR; see hydro_args."""
         qa, kq = self._planes(qold, 5 + nsp)
         oa, ko = self._planes(qnew, 5 + nsp)
-        if nsp == 0 and not cma:
+        if donor:
+            check(self.lib.glm_hydro_step_donor(self.ctx, qa, oa, C.byref(args), int(nsp), int(cma)))
+        elif nsp == 0 and not cma:
             check(self.lib.glm_hydro_step(self.ctx, qa, oa, C.byref(args)))
         else:
             check(self.lib.glm_hydro_step_scalars(self.ctx, qa, oa, C.byref(args), int(nsp), int(cma)))

@@ -203,7 +203,9 @@ int glm_ct_divb_max(glm_ctx* ctx, void* const* q, double dx, double* out);
  * builds it with -DHANCOCK1D; -DLLF_FALLBACK when llf_dmin or llf_pmin > 0): q, o = 5 planes
  * [rho, mx, my, mz, E] (distinct buffers).  Uses gamma, dt/dx, smallr (SMALLR), pfl (SMALLP;
  * <= 0 selects smallr*smallc^2/gamma), llf_dmin/llf_pmin (the fallback's density and pressure
- * floors, 1e-3 in spike_25d.cu:339) and zchunk from p; the other fields are ignored.
+ * floors, 1e-3 in spike_25d.cu:339) and zchunk from p; recon = GLM_RECON_PPM_SPIKE selects the
+ * parabolic-edge build (-DPPM1D, :467-483: the monotonized 3-point parabola's edges plus the half
+ * step's change, unclamped, HLL without the fallback); the other fields are ignored.
  * Asynchronous on the context stream.  Returns 0 or an error code. */
 int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
 /* The same march with nsp = 2 passive scalars (spike_25d.cu -DSCALARS, :115-119): q, o = 7 planes
@@ -211,6 +213,12 @@ int glm_hydro_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_param
  * upwind face value of s (-DCMA, consistent multi-fluid advection, :321, :367); cma = 0 solves
  * rho*s with the same HLL/LLF as the other variables.  nsp = 0 is glm_hydro_step. */
 int glm_hydro_step_scalars(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma);
+/* The donor-cell build (-DDONOR, spike_25d.cu:438-454): glm_hydro_step_scalars with the faces on the
+ * seams of the 32 x 8 column tiles solved first order, HLL of the two cell states (x faces at
+ * i - 1/2 with i mod 32 = 0, y faces at j - 1/2 with j mod 8 = 0, the periodic face after the last
+ * cell included; z has none), the other faces as the light march; HLL without the fallback.
+ * Exactly conservative.  nsp = 0 or 2 (cma as in glm_hydro_step_scalars); recon is ignored. */
+int glm_hydro_step_donor(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp, int cma);
 /* The same march on the compact storage formats of spike_25d.cu's standalone builds (:64-95; the
  * reference's bridge library takes FP32 conserved planes only, so this entry has no counterpart in
  * its C ABI).  FP32 contexts only; computation in FP32.

@@ -35,7 +35,11 @@ typedef struct {
     int cma;                            /* scalar flux rides the mass flux (-DCMA, :321, :367) */
     int logsp;                          /* the scalar planes hold log2 X, reconstructed in log space
                                            (-DU16SP, :64-72, :188-191, :248-250, :317-319) */
+    int ppm;                            /* parabolic edges (-DPPM1D, :467-483), HLL without fallback */
+    int donor;                          /* donor-cell tile seams (-DDONOR, :438-454), HLL without fallback */
 } HyScheme;
+
+real o_ppm1_spike(real qm, real q0, real qp, real sgn);   /* glm_oracle.c: spike_25d.cu:264-273 ppm1 */
 
 static int nvar(const HyScheme* s) { return 5 + s->nsp; }
 
@@ -238,12 +242,40 @@ static int step_from_prim(const real* prim, real* out, int nx, int ny, int nz, c
                     oh_hanc1d(P[2], s0, d, mh0, s);
                     oh_hanc1d(P[3], sp1, d, mhp, s);
                     real a[HV], b[HV], Fm[HV], Fp[HV];
-                    oh_edge(mhm, sm1, R(1), a, s);
-                    oh_edge(mh0, s0, R(-1), b, s);
-                    solve(a, b, d, Fm, s);
-                    oh_edge(mh0, s0, R(1), a, s);
-                    oh_edge(mhp, sp1, R(-1), b, s);
-                    solve(a, b, d, Fp, s);
+                    if (s->ppm) {   /* :476-481: the parabola's edge plus the half step's change, unclamped */
+                        for (int v = 0; v < nvar(s); ++v) {
+                            a[v] = o_ppm1_spike(P[0][v], P[1][v], P[2][v], R(1)) + mhm[v] - P[1][v];
+                            b[v] = o_ppm1_spike(P[1][v], P[2][v], P[3][v], R(-1)) + mh0[v] - P[2][v];
+                        }
+                        oh_hll(a, b, d, Fm, s);
+                        for (int v = 0; v < nvar(s); ++v) {
+                            a[v] = o_ppm1_spike(P[1][v], P[2][v], P[3][v], R(1)) + mh0[v] - P[2][v];
+                            b[v] = o_ppm1_spike(P[2][v], P[3][v], P[4][v], R(-1)) + mhp[v] - P[3][v];
+                        }
+                        oh_hll(a, b, d, Fp, s);
+                    } else if (s->donor) {   /* :440-448: first order at the faces on the 32 x 8 tile seams */
+                        int lob = d == 0 ? i % 32 == 0 : d == 1 ? j % 8 == 0 : 0;
+                        int hib = d == 0 ? ((i + 1) % nx) % 32 == 0 : d == 1 ? ((j + 1) % ny) % 8 == 0 : 0;
+                        if (lob) oh_hll(P[1], P[2], d, Fm, s);
+                        else {
+                            oh_edge(mhm, sm1, R(1), a, s);
+                            oh_edge(mh0, s0, R(-1), b, s);
+                            oh_hll(a, b, d, Fm, s);
+                        }
+                        if (hib) oh_hll(P[2], P[3], d, Fp, s);
+                        else {
+                            oh_edge(mh0, s0, R(1), a, s);
+                            oh_edge(mhp, sp1, R(-1), b, s);
+                            oh_hll(a, b, d, Fp, s);
+                        }
+                    } else {
+                        oh_edge(mhm, sm1, R(1), a, s);
+                        oh_edge(mh0, s0, R(-1), b, s);
+                        solve(a, b, d, Fm, s);
+                        oh_edge(mh0, s0, R(1), a, s);
+                        oh_edge(mhp, sp1, R(-1), b, s);
+                        solve(a, b, d, Fp, s);
+                    }
                     for (int v = 0; v < nvar(s); ++v) div[v] += Fp[v] - Fm[v];
                 }
                 real U[HV];

@@ -49,7 +49,7 @@ class OCTScheme(C.Structure):
 class OHyScheme(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "dtdx", "smallr", "smallp")] + [
         ("fallback", C.c_int), ("fb_dmin", C.c_double), ("fb_pmin", C.c_double), ("nsp", C.c_int), ("cma", C.c_int),
-        ("logsp", C.c_int)]
+        ("logsp", C.c_int), ("ppm", C.c_int), ("donor", C.c_int)]
 
 
 @dataclass
@@ -66,10 +66,12 @@ class HydroScheme:
     nsp: int = 0
     cma: bool = False
     logsp: bool = False        # the scalar planes hold log2 X (-DU16SP)
+    ppm: bool = False          # parabolic edges (-DPPM1D), HLL without the fallback
+    donor: bool = False        # donor-cell 32 x 8 tile seams (-DDONOR), HLL without the fallback
 
     def to_c(self) -> OHyScheme:
         return OHyScheme(self.gamma, self.dtdx, self.smallr, self.smallp, int(self.fallback), self.fb_dmin,
-                         self.fb_pmin, self.nsp, int(self.cma), int(self.logsp))
+                         self.fb_pmin, self.nsp, int(self.cma), int(self.logsp), int(self.ppm), int(self.donor))
 
 
 @dataclass

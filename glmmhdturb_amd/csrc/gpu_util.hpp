@@ -45,4 +45,27 @@ GLM_DEV void gstore_nt(T v, T* p)
 #endif
 }
 
+// An argument block as a reference into the constant address space: a kernel reads each field with
+// a scalar load where it needs it, instead of holding a copy of the whole block in registers (which
+// cost the ensemble kernel 64 B per lane of spills).
+template <typename S>
+GLM_DEV const auto& constant_ref(const S* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) S*)p;
+#else
+    return *p;   // host pass of the kernel templates: never executed
+#endif
+}
+// A kernel's own argument block read in place from the kernarg segment, where the explicit
+// arguments start: for a kernel whose one argument is an S (at offset 0), each field is then a
+// scalar load where it is read, as the ensemble reads its boxes' blocks, instead of coming through
+// the by-value parameter (the FP64 PPM + HLLD march at 256^3: 44 -> 36 B of spills per lane, 2%
+// faster).
+template <typename S>
+GLM_DEV const auto& kernarg_ref()
+{
+    return constant_ref((const S*)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
 }  // namespace glm

@@ -578,12 +578,13 @@ template <typename T, int PRED, int RIEM, int RECON, int SLOPE, int NS = 0>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU_NS(T, NS)))) march_kernel(MarchArgs<T> a)
 {
     __shared__ MarchLDS<T, 9 + NS> L;
+    const auto& ak = kernarg_ref<MarchArgs<T>>();
     if (threadIdx.x == 0) stage_cold(a, L.ca);
     __syncthreads();
-    const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE, NS>(a, L, xcd_block((int)blockIdx.x, (int)gridDim.x));
-    if (a.want_cfl) {
-        cfl_merge(L, cmax, a.cmax);
-        if (blockIdx.x == 0 && threadIdx.x == 0) *a.cmax_next = 0ull;
+    const double cmax = march_tile<T, PRED, RIEM, RECON, SLOPE, NS>(ak, L, xcd_block((int)blockIdx.x, (int)gridDim.x));
+    if (ak.want_cfl) {
+        cfl_merge(L, cmax, ak.cmax);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *ak.cmax_next = 0ull;
     }
 }
 
@@ -610,18 +611,6 @@ struct EnsLaunch {
     unsigned long long* t1;      // [ENS_LOG][nbox] wall-clock end of its last tile
 };
 
-// A box's argument block as a reference into the constant address space: the march reads each
-// field with a scalar load where it needs it, as it reads a kernel argument, instead of holding a
-// copy of the whole block in registers (which cost the ensemble kernel 64 B per lane of spills).
-template <typename S>
-GLM_DEV const auto& constant_ref(const S* p)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(4))) S*)p;
-#else
-    return *p;   // host pass of the kernel templates: never executed
-#endif
-}
 
 // First flat tile of XCD x's contiguous range (the inverse view of xcd_block) and its length.
 GLM_DEV int xcd_start(int x, int total)

@@ -67,6 +67,12 @@ static_assert(NT <= 256 && NT >= NHELP && NO % 64 == 0, "tile shape");
 #define GLM_WAVES_PER_EU(T) (sizeof(T) == 8 ? GLM_WAVES_F64 : GLM_WAVES_F32)
 // With species the FP64 workgroup's 97 KB of LDS leaves one workgroup per CU: 512 registers per lane.
 #define GLM_WAVES_PER_EU_NS(T, NS) ((NS) > 0 && sizeof(T) == 8 ? 1 : GLM_WAVES_PER_EU(T))
+// FP32 with species: the normal-only builds with PLM or the spike PPM fit three wavefronts per SIMD
+// (168 registers, no spills; the spike-PPM + HLL build took 169 at the default and ran at two:
+// 12.3 -> 14.7 Gcell/s at 480^3 with three); the full-predictor and Julia-PPM builds would spill
+// 16-44 B per lane there and keep two.  FP32 without species fits three at the default already.
+#define GLM_WAVES_PER_EU_SCHEME(T, NS, PRED, RECON) \
+    ((NS) > 0 && sizeof(T) == 4 && (PRED) == PRED_ONED && (RECON) != RECON_PPM ? 3 : GLM_WAVES_PER_EU_NS(T, NS))
 
 // Planes of a state: the 9 GLM-MHD planes and up to two passive species (rho X_s) after them.
 constexpr int NVMAX = 11;
@@ -575,7 +581,7 @@ GLM_DEV void cfl_merge(LDS& L, double cmax, unsigned long long* target)
 // One step of one grid: workgroup w marches tile xcd_block(w) (x fastest, then y, then z-chunk).
 // NS = 0 or 2 passive species after the 9 GLM-MHD planes.
 template <typename T, int PRED, int RIEM, int RECON, int SLOPE, int NS = 0>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU_NS(T, NS)))) march_kernel(MarchArgs<T> a)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU_SCHEME(T, NS, PRED, RECON)))) march_kernel(MarchArgs<T> a)
 {
     __shared__ MarchLDS<T, 9 + NS> L;
     const auto& ak = kernarg_ref<MarchArgs<T>>();

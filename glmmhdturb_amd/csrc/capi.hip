@@ -24,6 +24,14 @@
 #ifndef GLM_LIB_CMA
 #define GLM_LIB_CMA 0
 #endif
+// The scheme of a bridge library's march_run_dev / ct_run, fixed when it is built as the reference
+// fixes it with -D flags: 0 the default build; 1 the parabolic edges (spike_mhd.cu / spike_ctm.cu
+// -DPPM, spike_25d.cu -DPPM1D); 2 the hydro march's donor-cell seams (spike_25d.cu -DDONOR).
+#ifndef GLM_LIB_SCHEME
+#define GLM_LIB_SCHEME 0
+#endif
+static_assert(GLM_LIB_SCHEME >= 0 && GLM_LIB_SCHEME <= 2 && (GLM_LIB_SCHEME != 2 || GLM_LIB_NV == 5 || GLM_LIB_NV == 7),
+              "GLM_LIB_SCHEME: 0, 1 (PPM), or 2 (donor seams, hydro libraries only)");
 
 using namespace glm;
 
@@ -778,8 +786,9 @@ int march_set_grid(int nx, int ny, int nz)
 
 }  // extern "C"
 
-// The spike march scheme (spike_mhd.cu:257-288): normal-only Hancock, PLM MonCen, HLL, GLM pair,
-// psi *= GLMFAC, faces clamped at SMALLR/SMALLP = 1e-6, no predictor revert, no update floor.
+// The spike march scheme (spike_mhd.cu:257-288): normal-only Hancock, PLM MonCen (or with
+// GLM_LIB_SCHEME 1 the parabolic edges), HLL, GLM pair, psi *= GLMFAC, faces clamped at
+// SMALLR/SMALLP = 1e-6, no predictor revert, no update floor.
 static glm_params legacy_params()
 {
     glm_params p;
@@ -792,7 +801,7 @@ static glm_params legacy_params()
     p.ch = g_ch;
     p.glm_fac = g_glmfac;
     p.predictor = GLM_PRED_ONED;
-    p.recon = GLM_RECON_PLM;
+    p.recon = GLM_LIB_SCHEME == 1 ? GLM_RECON_PPM_SPIKE : GLM_RECON_PLM;
     p.slope = GLM_SLOPE_MONCEN;
     p.riemann = GLM_RIEMANN_HLL;
     p.mh_revert = 0;
@@ -819,8 +828,10 @@ static double legacy_run(T* const* qv, T* const* ov, int nsteps, int nv, int kin
     for (int it = 0; it < nsteps && !err; ++it) {
         err = kind == 1   ? glm_ct_step(c, q.data(), o.data(), &p, 0)
               : kind == 5 ? glm_ct_step_species(c, q.data(), o.data(), &p, 0, 2)
-              : kind == 2 ? glm_hydro_step(c, q.data(), o.data(), &p)
-              : kind == 3 ? glm_hydro_step_scalars(c, q.data(), o.data(), &p, 2, GLM_LIB_CMA)
+              : kind == 2 ? (GLM_LIB_SCHEME == 2 ? glm_hydro_step_donor(c, q.data(), o.data(), &p, 0, 0)
+                                                 : glm_hydro_step(c, q.data(), o.data(), &p))
+              : kind == 3 ? (GLM_LIB_SCHEME == 2 ? glm_hydro_step_donor(c, q.data(), o.data(), &p, 2, GLM_LIB_CMA)
+                                                 : glm_hydro_step_scalars(c, q.data(), o.data(), &p, 2, GLM_LIB_CMA))
               : kind == 4 ? glm_step_species(c, q.data(), o.data(), &p, 2)
                           : glm_step(c, q.data(), o.data(), &p);
         std::swap(q, o);

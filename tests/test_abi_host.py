@@ -283,3 +283,20 @@ def test_hydro_libraries_have_no_glm_or_ct_entry_points():
         exported = _exported(os.path.join(ROOT, "glmmhdturb_amd", name))
         for sym in ("march_set_glm", "ct_run", "ct_regs", "ct_shmem"):
             assert sym not in exported, (name, sym)
+
+
+@pytest.mark.parametrize("lib_name,nv,bridge", [("libmhd480_ppm.so", 9, "march"), ("libct480_ppm.so", 8, "ct"),
+                                                ("libmarch480_ppm1d.so", 5, "march"),
+                                                ("libmarch480_donor.so", 5, "march")])
+def test_scheme_bridge_libraries(lib_name, nv, bridge):
+    """The scheme builds (PPM=1 build_mhd.sh / build_ct.sh: -DPPM; SCHEME=ppm1d / donor
+    build_march.sh: -DPPM1D / -DDONOR) export the same bridge as their default builds."""
+    path = os.path.join(ROOT, "glmmhdturb_amd", lib_name)
+    assert os.path.exists(path), "build it with the package's build scripts (__graft_entry__.build)"
+    exported = _exported(path)
+    want = ("march_run_dev", "march_run_dev_f64") if bridge == "march" else ("ct_run", "ct_regs", "ct_shmem")
+    for sym in ("march_nv", "march_nx", "march_set_dtdx", "march_set_gamma") + want:
+        assert sym in exported, sym
+    assert ("march_set_glm" in exported) == (nv == 9)
+    lib = C.CDLL(path)
+    assert lib.march_nv() == nv and lib.march_nx() == 480

@@ -137,8 +137,22 @@ def run_gpu(N, steps, warmup, precision):
     kern_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
     d = s.diagnostics(u1)
     n = N ** 3
+    # after the timed steps: the march's memory floor (spike_mhd.cu -DMEMFLOOR) on the same grid,
+    # state and chunking -- the kernel's loads, staging and stores with no flux computation
+    a = step_args(p, 1e-3, t, do_turb=True, riemann="hlld", recon="ppm", predictor="full", scheme="cube",
+                  want_cfl=False)
+    for _ in range(3):
+        s.march_floor(u1, u2, a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(20):
+        s.march_floor(u1, u2, a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    floor_ms = e0.elapsed_time(e1) / 20
     out = dict(value=n * steps / wall, ms_per_step=wall / steps * 1e3, kernel_ms=kern_ms, t=t, mach=d["mach"],
-               finite=d["finite"], divbmax=d["divbmax"], diag=d, zchunk=tuned)
+               finite=d["finite"], divbmax=d["divbmax"], diag=d, zchunk=tuned, floor_ms=floor_ms)
     s.close()
     return out
 
@@ -959,6 +973,14 @@ def main():
                                          "x1.0 calibrated on a 1 GiB elementwise copy)",
                        "kernel": "glm::march_kernel<double, PRED_FULL, RIEM_HLLD>",
                        "kernel_ms": r64["kernel_ms"], "bytes_per_cell": BYTES_PER_CELL["f64"]}
+    # the march's own memory floor (glm_march_floor, spike_mhd.cu -DMEMFLOOR): its loads, LDS staging
+    # and stores timed alone, as the algorithmic bytes per launch over that time
+    rec["memory_floor"] = {"kernel": "glm::march_floor_kernel<double>", "ms": r64["floor_ms"],
+                           "algorithmic_GBs": bpl / (r64["floor_ms"] * 1e-3) / 1e9,
+                           "frac_of_peak": bpl / (r64["floor_ms"] * 1e-3) / HBM_PEAK,
+                           "march_over_floor": r64["kernel_ms"] / r64["floor_ms"],
+                           "note": "the same launch, tiling and data movement as the march with the flux "
+                                   "computation left out; the march's time over it is its compute share"}
     fpc, busy = profiled_alu(N)
     if fpc is not None:
         tf = fpc * N ** 3 / (r64["kernel_ms"] * 1e-3) / 1e12

@@ -25,7 +25,7 @@ DIAG_KEYS = ("mass", "ke2", "divbmax", "rhomin", "finite", "ekin", "emag", "cros
 EXPORTS = (
     "march_nv", "march_nx", "march_nsp", "march_set_dtdx", "march_set_glm", "march_set_gamma", "march_run_dev",
     "march_set_grid", "march_run_dev_f64", "ct_run", "ct_regs", "ct_shmem",
-    "glm_params_default", "glm_create", "glm_destroy", "glm_set_stream", "glm_step", "glm_step_species", "glm_cfl_max",
+    "glm_params_default", "glm_create", "glm_destroy", "glm_set_stream", "glm_step", "glm_step_species", "glm_march_floor", "glm_cfl_max",
     "glm_ctot_max", "glm_fast_max", "glm_diagnostics", "glm_ic", "glm_ou_setup", "glm_ou_advance", "glm_ou_afield",
     "glm_ou_steps", "glm_ou_copy_afield", "glm_ct_step", "glm_ct_step_species", "glm_ct_step_passes", "glm_ct_step_tiles", "glm_ct_ctot_max", "glm_ct_divb_max",
     "glm_hydro_step", "glm_hydro_step_scalars", "glm_hydro_step_compact", "glm_hydro_step_donor", "glm_last_error", "glm_error_string",
@@ -86,7 +86,8 @@ def load(path: str | None = None):
         "glm_params_default": (None, [C.POINTER(GLMParams)]),
         "glm_create": (vp, [i, i, i, i]), "glm_destroy": (None, [vp]), "glm_set_stream": (i, [vp, vp]),
         "glm_step": (i, [vp, pp, pp, C.POINTER(GLMParams)]),
-        "glm_step_species": (i, [vp, pp, pp, C.POINTER(GLMParams), i]), "glm_cfl_max": (i, [vp, C.POINTER(d)]),
+        "glm_step_species": (i, [vp, pp, pp, C.POINTER(GLMParams), i]),
+        "glm_march_floor": (i, [vp, pp, pp, C.POINTER(GLMParams)]), "glm_cfl_max": (i, [vp, C.POINTER(d)]),
         "glm_ctot_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_fast_max": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),
         "glm_diagnostics": (i, [vp, pp, C.POINTER(GLMParams), C.POINTER(d)]),

@@ -178,9 +178,10 @@ int glm_set_stream(glm_ctx* ctx, void* stream)
 
 }  // extern "C"
 
-// One march step over 9 + nsp planes (nsp = 0 or 2 passive species).
+// One march step over 9 + nsp planes (nsp = 0 or 2 passive species); with floor, the march's
+// memory floor (march_floor_kernel: the same data movement, no flux computation).
 template <typename T>
-static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_params* p, int nsp = 0)
+static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_params* p, int nsp = 0, bool floor = false)
 {
     MarchArgs<T> a;
     memset(&a, 0, sizeof a);
@@ -192,10 +193,10 @@ static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_param
     }
     if (planes_alias(q, nv, o, nv)) return set_err(GLM_ERR_ARG, "glm_step: input and output planes must differ");
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
-    const int key = 1 + (((p->predictor * 4 + p->riemann) * 4 + p->recon) * 4 + p->slope) + 1000 * nsp;
+    const int key = floor ? -1 : 1 + (((p->predictor * 4 + p->riemann) * 4 + p->recon) * 4 + p->slope) + 1000 * nsp;
     if (c->slots_key != key) {
         int dev = 0, cus = 0, occ = 0;
-        const void* fn = march_kernel_ptr<T>(p->predictor, p->riemann, p->recon, p->slope, nsp);
+        const void* fn = floor ? march_floor_ptr<T>() : march_kernel_ptr<T>(p->predictor, p->riemann, p->recon, p->slope, nsp);
         if (!fn) return set_err(GLM_ERR_ARG, "glm_step: unknown scheme");
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -226,6 +227,10 @@ static int step_impl(glm_ctx* c, void* const* q, void* const* o, const glm_param
     a.want_cfl = p->want_cfl;
     a.cmax = c->d_cmax + c->cfl_parity;
     a.cmax_next = c->d_cmax + (1 - c->cfl_parity);
+    if (floor) {
+        HIPCHK(launch_march_floor<T>(a, c->stream));
+        return GLM_OK;
+    }
     HIPCHK(launch_march<T>(a, p->predictor, p->riemann, p->recon, p->slope, c->stream, nsp));
     if (p->want_cfl) {
         c->cfl_slot = c->cfl_parity;
@@ -242,6 +247,14 @@ int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
     int e = check_params(p);
     if (e) return e;
     return ctx->prec == 8 ? step_impl<double>(ctx, q, o, p) : step_impl<float>(ctx, q, o, p);
+}
+
+int glm_march_floor(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p)
+{
+    if (!ctx || !q || !o) return set_err(GLM_ERR_ARG, "glm_march_floor: null argument");
+    int e = check_params(p);
+    if (e) return e;
+    return ctx->prec == 8 ? step_impl<double>(ctx, q, o, p, 0, true) : step_impl<float>(ctx, q, o, p, 0, true);
 }
 
 int glm_step_species(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp)

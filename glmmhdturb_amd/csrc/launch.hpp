@@ -20,6 +20,11 @@ template <typename T, int PRED, int RIEM>
 hipError_t launch_ens_pr(const EnsLaunch<T>& e, int recon, int slope, int grid, hipStream_t st);
 template <typename T, int PRED, int RIEM>
 const void* ens_kernel_ptr_pr(int recon, int slope);
+// The march's memory floor (march_floor_kernel; built in the full-predictor HLLD object).
+template <typename T>
+hipError_t launch_march_floor(const MarchArgs<T>& a, hipStream_t st);
+template <typename T>
+const void* march_floor_ptr();
 
 // nsp = 0: the 9-plane march; nsp = 2: with two passive species (11 planes).
 template <typename T>

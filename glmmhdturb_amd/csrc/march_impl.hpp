@@ -314,8 +314,10 @@ GLM_DEV void stage_cold(const MarchArgs<T>& a, ColdArgs<T>& ca)
 // The march of one tile: the workgroup's lanes march the columns of tile (tile_x, tile_y) through
 // z-chunk zc, tile = tile_x + tiles_x * (tile_y + tiles_y * zc), and write the updated planes.
 // L.ca must hold `a`'s cold arguments, published by a barrier.  Returns the lane's max CFL speed
-// over the cells it updated.
-template <typename T, int PRED, int RIEM, int RECON, int SLOPE, int NS = 0, typename A = MarchArgs<T>>
+// over the cells it updated.  MF (the memory floor, spike_mhd.cu -DMEMFLOOR): the same loads,
+// primitive conversion, LDS staging, update-base read and stores, with no predictor, faces, fluxes
+// or epilogue: each cell is stored as read.
+template <typename T, int PRED, int RIEM, int RECON, int SLOPE, int NS = 0, bool MF = false, typename A = MarchArgs<T>>
 GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 {
     constexpr int NV = 9 + NS;   // the GLM-MHD planes and NS passive species
@@ -387,12 +389,12 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 
         // ---- reconstruction of this column's cell in plane q --------------------------------
         bool up = true;   // PPM strong-pressure-jump fallback flag (glmmhd_turb.jl:959)
-        if (RECON == RECON_PPM) {
+        if (RECON == RECON_PPM && !MF) {
             up = !(spj(P[4][lo_own - 1], p0[4], P[4][lo_own + 1]) || spj(P[4][lo_own - PX], p0[4], P[4][lo_own + PX]) ||
                    spj(pm[4], p0[4], pp[4]));
         }
         T mh[NV];
-        if (PRED == PRED_FULL) {
+        if (PRED == PRED_FULL && !MF) {
             // glmmhd_turb.jl:165-172 hancock, :391-395 revert: U0 - dt/(2dx) sum_d [F_d(m0+s_d) - F_d(m0-s_d)];
             // species: the advective half step X - dt/dx (u hs_x + v hs_y + w hs_z)
             T acc[9], t[9], f[9], hs[NV], sadv[NS > 0 ? NS : 1];
@@ -450,7 +452,9 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 
         // ---- z face q-1/2 and the update of plane q-1 ---------------------------------------
         T fz[NV];
-        if (owned) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) fz[v] = T(0);
+        if (owned && !MF) {
             T zlo[NV], zhi[NV];
             faces_dir<T, PRED, RECON, SLOPE, 2, NV>(pm, p0, pp, mh, up, a, zlo, zhi);
             if (q >= zs) {
@@ -460,7 +464,7 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 #pragma unroll
             for (int v = 0; v < NV; ++v) zprev[v] = zhi[v];
         }
-        if (PRED == PRED_FULL && inplane) {
+        if (PRED == PRED_FULL && inplane && !MF) {
             // park the predicted state in this lane's own +x slot (free until its x face state is
             // stored) across the z solve and the update
 #pragma unroll
@@ -469,12 +473,14 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
         if (do_upd) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) u0[v] = gload(ca.q[v] + col + plane * wrapi(q - 1, nz));
-            T acc[NV];
-#pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] = pend[v] - fz[v];
             const int zq = wrapi(q - 1, nz);
-            double ct = update_cell<NV>(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
-            if (!(ct <= cmax)) cmax = ct;
+            if (!MF) {
+                T acc[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] = pend[v] - fz[v];
+                double ct = update_cell<NV>(u0, acc, ca, s, a.want_cfl != 0, fc, gx, gy, zq);
+                if (!(ct <= cmax)) cmax = ct;
+            }
             const size_t off = col + plane * zq;
 #pragma unroll
             for (int v = 0; v < NV; ++v) gstore_nt(u0[v], ca.o[v] + off);
@@ -483,7 +489,7 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 
 
         // ---- x and y faces of plane q (only for planes this chunk updates) ---------------------
-        if (inplane) {
+        if (inplane && !MF) {
             T xlo[NV], ylo[NV];
             __asm__ volatile("" ::: "memory");   // re-read p0 here instead of keeping it live
 #pragma unroll
@@ -540,7 +546,7 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
         // ---- advance the column window and publish plane q+1 to LDS ------------------------------
         // The last reads of plane q's LDS cells precede the face-state barrier when this plane's
         // faces were solved; otherwise (the lead-in plane) they are the predictor's.
-        if (!inplane) __syncthreads();
+        if (!inplane || MF) __syncthreads();
 #pragma unroll
         for (int v = 0; v < NV; ++v) { pm[v] = P[v][lo_own]; P[v][lo_own] = pp[v]; }
         c2p<NV>(cn, pp, s);
@@ -592,6 +598,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAV
         cfl_merge(L, cmax, ak.cmax);
         if (blockIdx.x == 0 && threadIdx.x == 0) *ak.cmax_next = 0ull;
     }
+}
+
+// The memory floor of the march (spike_mhd.cu -DMEMFLOOR): march_kernel's data movement and
+// launch shape with no flux computation (march_tile with MF); o receives q unchanged.  The layout
+// and LDS are those of the FP64/FP32 march without species; the scheme arguments only select them.
+template <typename T>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GLM_WAVES_PER_EU(T)))) march_floor_kernel(MarchArgs<T> a)
+{
+    __shared__ MarchLDS<T, 9> L;
+    const auto& ak = kernarg_ref<MarchArgs<T>>();
+    if (threadIdx.x == 0) stage_cold(a, L.ca);
+    __syncthreads();
+    (void)march_tile<T, PRED_FULL, RIEM_HLLD, RECON_PPM, SLOPE_MONCEN, 0, true>(ak, L, xcd_block((int)blockIdx.x, (int)gridDim.x));
 }
 
 // ---------------------------------------------------------------------------------------------

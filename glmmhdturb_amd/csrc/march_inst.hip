@@ -91,4 +91,19 @@ const void* ens_kernel_ptr_pr<GLM_T, GLM_PRED, GLM_RIEM>(int recon, int slope)
     return nullptr;
 }
 
+#if GLM_PRED == 0 && GLM_RIEM == 3   // the PRED_FULL, RIEM_HLLD object (the preprocessor sees no enums)
+template <>
+hipError_t launch_march_floor<GLM_T>(const MarchArgs<GLM_T>& a, hipStream_t st)
+{
+    hipLaunchKernelGGL((march_floor_kernel<GLM_T>), dim3((unsigned)(a.tiles_x * a.tiles_y * a.nzc)), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+
+template <>
+const void* march_floor_ptr<GLM_T>()
+{
+    return (const void*)&march_floor_kernel<GLM_T>;
+}
+#endif
+
 }  // namespace glm

@@ -250,6 +250,16 @@ class Solver:
             args.zchunk = self.zchunk
         check(self.lib.glm_step_species(self.ctx, qa, oa, C.byref(args), int(nsp)))
 
+    def march_floor(self, uold, unew, args: GLMParams):
+        """The memory floor of step() (spike_mhd.cu -DMEMFLOOR): the march's loads, staging and
+        stores with no flux computation; unew receives uold unchanged.  For measurement."""
+        qa, kq = self._planes(uold)
+        oa, ko = self._planes(unew)
+        if args.zchunk == 0 and self.zchunk:
+            args = GLMParams.from_buffer_copy(args)
+            args.zchunk = self.zchunk
+        check(self.lib.glm_march_floor(self.ctx, qa, oa, C.byref(args)))
+
     def tune_zchunk(self, uold, unew, args: GLMParams, reps: int = 3, ct: bool = False, face_slope: int = 0) -> dict:
         """Time the march (ct=True: the CT march) for a set of z-chunkings of this grid and keep the
         fastest for later steps whose args leave zchunk at 0.  Every chunking gives the same bits

@@ -124,6 +124,12 @@ int glm_step(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
  * face carries F_s = F_rho * X_s of the upwind side of F_rho.  The MHD planes come out bit-identical
  * to glm_step's.  nsp: 0 (same as glm_step) or 2. */
 int glm_step_species(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p, int nsp);
+/* The memory floor of glm_step (spike_mhd.cu -DMEMFLOOR, :260-299: the march's loads and stores
+ * with the flux computation compiled out): the same launch, tiling, z-chunking, plane loads,
+ * primitive conversion, LDS staging, update-base read and stores as glm_step, with no predictor,
+ * faces, fluxes or epilogue, so o receives q unchanged (9 planes; p as for glm_step, of which the
+ * grid chunking is used).  For measurement: the time of the march's data movement alone. */
+int glm_march_floor(glm_ctx* ctx, void* const* q, void* const* o, const glm_params* p);
 /* Max over cells of sum_d(|v_d| + c_f) of the state written by the last glm_step that had
  * want_cfl set (synchronises the stream).  dt = courant*dx/cmax (compute_dt, :1973-1978). */
 int glm_cfl_max(glm_ctx* ctx, double* cmax);

@@ -70,9 +70,16 @@ static_assert(NT <= 256 && NT >= NHELP && NO % 64 == 0, "tile shape");
 // FP32 with species: the normal-only builds with PLM or the spike PPM fit three wavefronts per SIMD
 // (168 registers, no spills; the spike-PPM + HLL build took 169 at the default and ran at two:
 // 12.3 -> 14.7 Gcell/s at 480^3 with three); the full-predictor and Julia-PPM builds would spill
-// 16-44 B per lane there and keep two.  FP32 without species fits three at the default already.
-#define GLM_WAVES_PER_EU_SCHEME(T, NS, PRED, RECON) \
-    ((NS) > 0 && sizeof(T) == 4 && (PRED) == PRED_ONED && (RECON) != RECON_PPM ? 3 : GLM_WAVES_PER_EU_NS(T, NS))
+// 16-44 B per lane there and keep two.  FP32 without species: four wavefronts per SIMD (40.75 KB
+// of LDS per workgroup, four fit in 160 KB), held to 128 registers with 12-48 B of spills, beat
+// three at ~155 registers: the full-predictor HLLD + PPM march 1.32 -> 1.27 ms at 256^3, the
+// normal-only PLM + HLL 19.9 -> 21.3 Gcell/s at 480^3 (the spike PPM unchanged).
+#ifndef GLM_WAVES_F32_NS0
+#define GLM_WAVES_F32_NS0 4
+#endif
+#define GLM_WAVES_PER_EU_SCHEME(T, NS, PRED, RECON)                                              \
+    ((NS) > 0 && sizeof(T) == 4 && (PRED) == PRED_ONED && (RECON) != RECON_PPM ? 3               \
+     : (NS) == 0 && sizeof(T) == 4 ? GLM_WAVES_F32_NS0 : GLM_WAVES_PER_EU_NS(T, NS))
 
 // Planes of a state: the 9 GLM-MHD planes and up to two passive species (rho X_s) after them.
 constexpr int NVMAX = 11;

@@ -1,7 +1,8 @@
 """ctypes wrapper over the C oracle (oracle/_build/liboracle_{f64,f32,dual}.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, where it is the checker.  The product package never imports it.
+cpu_baseline leg, where it is the reference the GPU results are compared with (or, in the
+cpu_baseline leg, the CPU path that is timed).  The product package never imports it.
 
 Every function mirrors one in glm_oracle.c / ou_oracle.c / ct_oracle.c; see those files for the
 reference lines each one restates.

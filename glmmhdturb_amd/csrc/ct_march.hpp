@@ -376,6 +376,9 @@ ct_march_kernel(CTMArgs<T> a_)
         if (q == ze) break;
 
         // ---- prefetch and publish plane q+1 -------------------------------------------------------
+        // The prefetch and publish run ahead of the other workgroup's wavefront on this SIMD, so the
+        // workgroups reach the plane's last barrier sooner (FP32 PPM +2%, FP64 HLLD + PLM +1.7%).
+        __builtin_amdgcn_s_setprio(1);
         T pn[NV], hn[NV];
         cm_prim<T, NV>(a, gx, gy, cwrap(q + 2, nz), pn);
         if (helper) cm_prim<T, NV>(a, hgx, hgy, cwrap(q + 1, nz), hn);
@@ -387,6 +390,7 @@ ct_march_kernel(CTMArgs<T> a_)
             for (int v = 0; v < NV; ++v) P[v][lo_ring] = hn[v];
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
     }
 }
 

@@ -518,6 +518,11 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) YB[v][lo_own] = hi[v];
             }
+            // From here to the plane's last barrier (face solves, flux exchange, the next planes'
+            // loads and the LDS publish) this wavefront issues ahead of the other workgroup's on its
+            // SIMD, which is then in its reconstruction: the workgroups reach their barriers sooner
+            // (FP64 march -0.9%, FP32 -4% at 256^3).
+            __builtin_amdgcn_s_setprio(1);
             __syncthreads();
             T fx[NV], fy[NV];
 #pragma unroll
@@ -547,6 +552,7 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
             }
         }
 
+        if (!MF) __builtin_amdgcn_s_setprio(1);   // (set already when this plane's faces were solved)
         T cn[NV], hr[NV];
         loadv<NV>(ca, col + plane * wrapi(q + 2, nz), cn);
         if (helper) loadv<NV>(ca, hcol + plane * wrapi(q + 1, nz), hr);
@@ -564,6 +570,7 @@ GLM_DEV double march_tile(const A& a, MarchLDS<T, 9 + NS>& L, int tile)
             for (int v = 0; v < NV; ++v) P[v][lo_ring] = h[v];
         }
         __syncthreads();
+        if (!MF) __builtin_amdgcn_s_setprio(0);   // the reconstruction runs at the base priority
     }
 
     return cmax;
